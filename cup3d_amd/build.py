@@ -1,0 +1,92 @@
+"""Build the native libraries of the package in-tree (cup3d_amd/lib/).
+
+  libcup3d_poisson.so   HIP kernels for gfx950 + C++ host driver + C ABI
+                        (include/cup3d_poisson.h), built with hipcc
+  libcup3d_workloads.so synthetic grids and right-hand sides (g++, OpenMP)
+  libcup3d_adapter.so   C++ PoissonSolverBase adapter over the C ABI (g++)
+
+Run: python -m cup3d_amd.build
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(PKG, "host")
+LIB = os.path.join(PKG, "lib")
+BUILD = os.path.join(PKG, "_build")
+INCLUDE = os.path.join(ROOT, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+ARCH = "gfx950"
+
+# Packed-FP32 instructions are disabled for all device code (they have given
+# silently wrong results on this hardware pool).
+HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-I", INCLUDE, "-I", CSRC]
+
+POISSON_SOURCES = ["kernels.hip", "mixed.hip", "cnn.hip", "topology.cpp", "solver.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {os.path.basename(cmd[-1])}")
+    return r
+
+
+def _compile_hip(src):
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    lang = [] if src.endswith(".hip") else ["-x", "hip"]
+    _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+    return obj
+
+
+def build_poisson(jobs=4):
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIB, exist_ok=True)
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(_compile_hip, POISSON_SOURCES))
+    out = os.path.join(LIB, "libcup3d_poisson.so")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out + ".tmp"] + objs)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_workloads():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libcup3d_workloads.so")
+    _run([CXX, "-std=c++17", "-O3", "-fopenmp", "-fPIC", "-shared", "-Wall",
+          os.path.join(CSRC, "workloads.cpp"), "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_adapter():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libcup3d_adapter.so")
+    _run([CXX, "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-I", INCLUDE, "-I", HOST,
+          os.path.join(HOST, "PoissonSolverHIP.cpp"), "-o", out + ".tmp",
+          "-L", LIB, "-lcup3d_poisson", "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_all():
+    outs = [build_workloads(), build_poisson()]
+    if os.path.exists(os.path.join(HOST, "PoissonSolverHIP.cpp")):
+        outs.append(build_adapter())
+    return outs
+
+
+if __name__ == "__main__":
+    for p in build_all():
+        print(p)

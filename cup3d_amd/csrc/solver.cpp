@@ -1,0 +1,559 @@
+// Solver driver and C ABI of libcup3d_poisson (host side, C++17 + HIP runtime).
+//
+// Mirrors the life cycle of the reference GPU backend:
+//   PoissonSolverExp ctor (P_inv, PoissonSolverExp.cpp:37-111) -> create
+//   getMat + updateAll on mesh change (.cpp:437-442, BiCGSTAB.cu:143-247)
+//                                                   -> set_topology
+//   getVec + BiCGSTABSolver::main + scatter (.cpp:420-473, BiCGSTAB.cu:416-628)
+//                                                   -> solve / solve_device
+// Device memory is allocated once per topology and reused by every solve.
+
+#include "cup3d_poisson.h"
+#include "cup3d_poisson_debug.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cnn.h"
+#include "kernels.h"
+#include "mixed.h"
+#include "poisson_internal.h"
+
+using namespace cup3d;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t alloc(size_t b) {
+    release();
+    bytes = b;
+    return hipMalloc(&p, b ? b : 1);
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+#define HIPCHECK(expr)                                                                     \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      H->err = std::string(#expr) + ": " + hipGetErrorString(e_);                          \
+      return -1;                                                                           \
+    }                                                                                      \
+  } while (0)
+
+}  // namespace
+
+struct cup3d_poisson {
+  cup3d_poisson_config cfg;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+
+  // topology
+  HostTopology topo;
+  std::vector<int> blocks;  // nb x 4
+  int nb = 0;
+  long long n = 0;
+  DevBuf d_nbr, d_h, d_invh, d_h3, d_hf, d_invhf, d_irow_ptr, d_irow_cell, d_irow_ent, d_ent_col,
+      d_ent_val, d_ent_valf, d_blocks;
+  DevGrid<double> g64;
+  DevGrid<float> g32;
+
+  // preconditioner
+  std::vector<double> P;
+  DevBuf d_P64;
+
+  // FP64 Krylov workspace
+  DevBuf d_vec64;  // 11 vectors of n doubles
+  KrylovVectors<double> v64{};
+  DevBuf d_state, d_part;
+  SolverState* h_state = nullptr;  // pinned
+  size_t part_len = 0;
+
+  // mixed precision workspace
+  MixedWorkspace mixed;
+  // learned warm start
+  CnnState cnn;
+};
+
+namespace {
+
+int fail(cup3d_poisson* H, const std::string& m) {
+  H->err = m;
+  return -1;
+}
+
+SolverState make_state(const cup3d_poisson_config& c) {
+  SolverState s;
+  std::memset(&s, 0, sizeof(s));
+  s.alpha = 1.0;  // BiCGSTAB.cu:431
+  s.beta = 1.0;
+  s.omega = 1.0;
+  s.eps = 1e-21;
+  s.rho_prev = 1.0;
+  s.rho_curr = 1.0;
+  s.err_init = s.err_opt = s.err = 1e50;  // :424-426
+  s.tol_abs = c.tol_abs;
+  s.tol_rel = c.tol_rel;
+  s.max_iter = c.max_iter;
+  s.max_restarts = c.max_restarts;
+  s.status = ST_RUNNING;
+  s.p_zero = 1;
+  s.xcur = 0;
+  s.xopt = 0;
+  return s;
+}
+
+int upload(cup3d_poisson* H, DevBuf& d, const void* src, size_t bytes) {
+  if (d.alloc(bytes) != hipSuccess) return fail(H, "hipMalloc failed (" + std::to_string(bytes) + " bytes)");
+  if (bytes && hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(H, "hipMemcpy H2D failed");
+  return 0;
+}
+
+// Run the FP64 classical BiCGSTAB on the device. b64/x64 must already hold
+// getVec's b and x0 (in v64.b and v64.x[0]); the state must be initialised.
+int run_bicgstab64(cup3d_poisson* H) {
+  hipStream_t s = H->stream;
+  SolverState* st = H->d_state.as<SolverState>();
+  double* part = H->d_part.as<double>();
+  const KrylovVectors<double>& v = H->v64;
+  // r = b - A x0; rhat = r (BiCGSTAB.cu:435-467)
+  launch_spmv64(s, SPMV_INIT, H->g64, v.x[0], v.nu, v.b, v.r, v.rhat, st, part);
+  const int poll = H->cfg.poll_every > 0 ? H->cfg.poll_every : 8;
+  int guard = 0;
+  while (true) {
+    for (int k = 0; k < poll; ++k) {
+      launch_precond64(s, PREC_P, H->nb, H->d_P64.as<double>(), H->g64.invh, v, st);
+      launch_spmv64(s, SPMV_DOT1, H->g64, v.z1, v.nu, v.rhat, nullptr, nullptr, st, part);
+      launch_precond64(s, PREC_S, H->nb, H->d_P64.as<double>(), H->g64.invh, v, st);
+      launch_spmv64(s, SPMV_DOT2, H->g64, v.z2, v.t, v.r, nullptr, nullptr, st, part);
+      launch_update64(s, H->n, v, st, part);
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(H->h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (H->h_state->status != ST_RUNNING) break;
+    if (++guard > (H->cfg.max_iter / poll) + 4) return fail(H, "device state machine did not stop");
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cup3d_poisson_create(const cup3d_poisson_config* cfg, cup3d_poisson_t* out) {
+  if (!cfg || !out) {
+    g_create_error = "null argument";
+    return -1;
+  }
+  *out = nullptr;
+  const cup3d_poisson_config& c = *cfg;
+  for (int d = 0; d < 3; ++d) {
+    if (c.bpd[d] <= 0) { g_create_error = "bpd must be positive"; return -1; }
+    if (c.bc[d] < 0 || c.bc[d] > 2) { g_create_error = "bc must be 0 (freespace), 1 (periodic) or 2 (wall)"; return -1; }
+  }
+  if (!(c.h0 > 0)) { g_create_error = "h0 must be positive"; return -1; }
+  if (c.bmean < 0 || c.bmean > 3) { g_create_error = "bmean must be 0..3"; return -1; }
+  if (c.max_iter <= 0) { g_create_error = "max_iter must be positive"; return -1; }
+  if (c.precision < 0 || c.precision > 2) { g_create_error = "unknown precision"; return -1; }
+  if (c.guess < 0 || c.guess > 2) { g_create_error = "unknown guess mode"; return -1; }
+  if (c.bmean == 2) {
+    g_create_error = "bMeanConstraint 2 is not supported by the GPU solver";
+    return -1;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    g_create_error = "no HIP device available";
+    return -1;
+  }
+  if (c.device < 0 || c.device >= ndev) { g_create_error = "invalid device ordinal"; return -1; }
+  std::unique_ptr<cup3d_poisson> H(new cup3d_poisson());
+  H->cfg = c;
+  if (H->cfg.inner_tol <= 0) H->cfg.inner_tol = 1e-4;
+  if (H->cfg.max_refine <= 0) H->cfg.max_refine = 10;
+  H->device = c.device;
+  if (hipSetDevice(c.device) != hipSuccess || hipStreamCreate(&H->stream) != hipSuccess ||
+      hipEventCreate(&H->ev0) != hipSuccess || hipEventCreate(&H->ev1) != hipSuccess ||
+      hipHostMalloc((void**)&H->h_state, sizeof(SolverState), hipHostMallocDefault) != hipSuccess) {
+    g_create_error = "HIP stream/event/pinned allocation failed";
+    return -1;
+  }
+  // block-Jacobi inverse (PoissonSolverExp ctor) and its MFMA packing
+  build_pinv_dst(H->P);
+  std::vector<double> pk;
+  pack_pinv_f64(H->P, pk);
+  if (upload(H.get(), H->d_P64, pk.data(), pk.size() * sizeof(double)) != 0 ||
+      mixed_init_pinv(H->mixed, H->P, H->err) != 0 || cnn_init(H->cnn, H->err) != 0) {
+    g_create_error = H->err;
+    return -1;
+  }
+  if (H->d_state.alloc(sizeof(SolverState)) != hipSuccess) {
+    g_create_error = "hipMalloc(state) failed";
+    return -1;
+  }
+  *out = H.release();
+  return 0;
+}
+
+int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_block_desc* blocks) {
+  if (!H) return -1;
+  if (nblocks <= 0 || !blocks) return fail(H, "empty block list");
+  if (nblocks > (1LL << 22)) return fail(H, "too many blocks (limit 4194304)");
+  HIPCHECK(hipSetDevice(H->device));
+  H->blocks.resize(nblocks * 4);
+  for (int64_t b = 0; b < nblocks; ++b) {
+    H->blocks[4 * b] = blocks[b].level;
+    for (int d = 0; d < 3; ++d) H->blocks[4 * b + 1 + d] = blocks[b].index[d];
+  }
+  TopologyInput in;
+  for (int d = 0; d < 3; ++d) {
+    in.bpd[d] = H->cfg.bpd[d];
+    in.bc[d] = H->cfg.bc[d];
+  }
+  in.level_max = H->cfg.level_max;
+  in.h0 = H->cfg.h0;
+  in.bmean = H->cfg.bmean;
+  in.nb = (int)nblocks;
+  in.blocks = H->blocks.data();
+  std::string e;
+  if (!build_topology(in, H->topo, e)) return fail(H, e);
+  HostTopology& T = H->topo;
+  H->nb = (int)nblocks;
+  H->n = (long long)nblocks * BLEN;
+  std::vector<float> hf(T.h.begin(), T.h.end()), invhf(T.invh.begin(), T.invh.end()),
+      valf(T.ent_val.begin(), T.ent_val.end());
+  if (upload(H, H->d_nbr, T.nbr.data(), T.nbr.size() * sizeof(int)) ||
+      upload(H, H->d_h, T.h.data(), T.h.size() * sizeof(double)) ||
+      upload(H, H->d_invh, T.invh.data(), T.invh.size() * sizeof(double)) ||
+      upload(H, H->d_h3, T.h3.data(), T.h3.size() * sizeof(double)) ||
+      upload(H, H->d_hf, hf.data(), hf.size() * sizeof(float)) ||
+      upload(H, H->d_invhf, invhf.data(), invhf.size() * sizeof(float)) ||
+      upload(H, H->d_irow_ptr, T.irow_ptr.data(), T.irow_ptr.size() * sizeof(int)) ||
+      upload(H, H->d_irow_cell, T.irow_cell.data(), T.irow_cell.size() * sizeof(int)) ||
+      upload(H, H->d_irow_ent, T.irow_ent.data(), T.irow_ent.size() * sizeof(int)) ||
+      upload(H, H->d_ent_col, T.ent_col.data(), T.ent_col.size() * sizeof(int)) ||
+      upload(H, H->d_ent_val, T.ent_val.data(), T.ent_val.size() * sizeof(double)) ||
+      upload(H, H->d_ent_valf, valf.data(), valf.size() * sizeof(float)) ||
+      upload(H, H->d_blocks, H->blocks.data(), H->blocks.size() * sizeof(int)))
+    return -1;
+  const long long origin_row = T.origin_block >= 0 ? (long long)T.origin_block * BLEN : -1;
+  auto fill = [&](auto& g, const void* h, const void* invh, const void* val) {
+    using G = std::remove_reference_t<decltype(g)>;
+    using TT = std::remove_const_t<std::remove_pointer_t<decltype(G::h)>>;
+    g.nb = H->nb;
+    g.n = H->n;
+    g.nbr = H->d_nbr.as<int>();
+    g.h = static_cast<const TT*>(h);
+    g.invh = static_cast<const TT*>(invh);
+    g.h3 = H->d_h3.as<double>();
+    g.irow_ptr = H->d_irow_ptr.as<int>();
+    g.irow_cell = H->d_irow_cell.as<int>();
+    g.irow_ent = H->d_irow_ent.as<int>();
+    g.ent_col = H->d_ent_col.as<int>();
+    g.ent_val = static_cast<const TT*>(val);
+    g.bmean = H->cfg.bmean;
+    g.bmean_row = (H->cfg.bmean == 1 || H->cfg.bmean == 3) ? origin_row : -1;
+  };
+  fill(H->g64, H->d_h.p, H->d_invh.p, H->d_ent_val.p);
+  fill(H->g32, H->d_hf.p, H->d_invhf.p, H->d_ent_valf.p);
+  // FP64 Krylov vectors: x[3], r, rhat, p, nu, t, z1, z2, b
+  const size_t n = (size_t)H->n;
+  if (H->d_vec64.alloc(11 * n * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(vectors) failed");
+  double* base = H->d_vec64.as<double>();
+  KrylovVectors<double>& v = H->v64;
+  for (int k = 0; k < 3; ++k) v.x[k] = base + k * n;
+  v.r = base + 3 * n;
+  v.rhat = base + 4 * n;
+  v.p = base + 5 * n;
+  v.nu = base + 6 * n;
+  v.t = base + 7 * n;
+  v.z1 = base + 8 * n;
+  v.z2 = base + 9 * n;
+  v.b = base + 10 * n;
+  H->part_len = (size_t)((H->nb + 3) / 4 + 8) * 4;
+  if (H->d_part.alloc(H->part_len * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(partials) failed");
+  HIPCHECK(hipMemset(H->d_state.p, 0, sizeof(SolverState)));
+  if (mixed_set_topology(H->mixed, H->nb, H->n, H->err) != 0) return -1;
+  if (cnn_set_topology(H->cnn, H->nb, H->d_nbr.as<int>(), H->err) != 0) return -1;
+  HIPCHECK(hipDeviceSynchronize());
+  return 0;
+}
+
+static int solve_fp64(cup3d_poisson_t H, const double* d_rhs, double* d_x, const double* d_x0,
+                      cup3d_solve_report* rep) {
+  hipStream_t s = H->stream;
+  SolverState init = make_state(H->cfg);
+  *H->h_state = init;
+  SolverState* st = H->d_state.as<SolverState>();
+  double* part = H->d_part.as<double>();
+  HIPCHECK(hipMemcpyAsync(st, H->h_state, sizeof(SolverState), hipMemcpyHostToDevice, s));
+  const long long zero_row = (H->cfg.bmean == 1 || H->cfg.bmean == 3) && H->topo.origin_block >= 0
+                                 ? (long long)H->topo.origin_block * BLEN
+                                 : -1;
+  launch_getvec64(s, H->n, zero_row, d_rhs, d_x0, H->v64.b, H->v64.x[0], st, part, H->cfg.tol_rhs);
+  if (run_bicgstab64(H) != 0) return -1;
+  launch_copy_opt64(s, H->n, H->v64, st, d_x);
+  // true residual ||b - A x_opt|| in FP64 (into nu)
+  launch_spmv64(s, SPMV_RESID, H->g64, d_x, nullptr, H->v64.b, H->v64.nu, nullptr, st, part);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(H->h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  const SolverState& S = *H->h_state;
+  if (rep) {
+    rep->status = S.status;
+    rep->iterations = S.iters_done;
+    rep->restarts = S.restarts;
+    rep->refinements = 1;
+    rep->err_init = S.err_init;
+    rep->err_final = S.err_opt;
+    rep->rhs_norm = std::sqrt(S.bnorm2);
+    rep->true_rel_resid = S.bnorm2 > 0 ? std::sqrt(S.true_rr / S.bnorm2) : std::sqrt(S.true_rr);
+    rep->guess_resid = S.err_init;
+  }
+  return S.status;
+}
+
+int cup3d_poisson_solve_device(cup3d_poisson_t H, const double* d_rhs, double* d_x,
+                               cup3d_solve_report* rep) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called before solve");
+  if (!d_rhs || !d_x) return fail(H, "null vector");
+  HIPCHECK(hipSetDevice(H->device));
+  cup3d_solve_report local;
+  std::memset(&local, 0, sizeof(local));
+  HIPCHECK(hipEventRecord(H->ev0, H->stream));
+  const double* x0 = d_x;
+  int guess_used = H->cfg.guess;
+  double guess_resid = -1;
+  if (H->cfg.guess == CUP3D_GUESS_ZERO) x0 = nullptr;
+  int st;
+  if (H->cfg.guess == CUP3D_GUESS_CNN) {
+    // learned warm start with fallback to the given guess
+    if (cnn_warm_start(H->cnn, H->stream, H->g64, H->v64, H->d_state.as<SolverState>(),
+                       H->d_part.as<double>(), d_rhs, d_x, H->cfg.bmean, H->topo.origin_block,
+                       &guess_used, &guess_resid, H->err) != 0)
+      return -1;
+  }
+  if (H->cfg.precision == CUP3D_PREC_FP64)
+    st = solve_fp64(H, d_rhs, d_x, x0, &local);
+  else
+    st = mixed_solve(H->mixed, H->stream, H->cfg, H->g64, H->g32, H->v64,
+                     H->d_state.as<SolverState>(), H->h_state, H->d_part.as<double>(),
+                     H->topo.origin_block, d_rhs, d_x, x0, &local, H->err);
+  if (st < 0) return st;
+  HIPCHECK(hipEventRecord(H->ev1, H->stream));
+  HIPCHECK(hipEventSynchronize(H->ev1));
+  float ms = 0;
+  HIPCHECK(hipEventElapsedTime(&ms, H->ev0, H->ev1));
+  local.solve_ms = ms;
+  local.guess_used = guess_used;
+  if (guess_resid >= 0) local.guess_resid = guess_resid;
+  if (rep) *rep = local;
+  return st;
+}
+
+int cup3d_poisson_solve(cup3d_poisson_t H, const double* const* rhs_blocks,
+                        double* const* x_blocks, const double* rhs_contig, double* x_contig,
+                        cup3d_solve_report* rep) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called before solve");
+  if ((!rhs_blocks && !rhs_contig) || (!x_blocks && !x_contig)) return fail(H, "null vector");
+  HIPCHECK(hipSetDevice(H->device));
+  const size_t n = (size_t)H->n, bytes = n * sizeof(double);
+  double *hb = nullptr, *hx = nullptr, *db = nullptr, *dx = nullptr;
+  HIPCHECK(hipHostMalloc((void**)&hb, bytes, hipHostMallocDefault));
+  HIPCHECK(hipHostMalloc((void**)&hx, bytes, hipHostMallocDefault));
+  HIPCHECK(hipMalloc(&db, bytes));
+  HIPCHECK(hipMalloc(&dx, bytes));
+  // getVec gather (PoissonSolverExp.cpp:389-417)
+  for (int b = 0; b < H->nb; ++b) {
+    std::memcpy(hb + (size_t)b * BLEN, rhs_blocks ? rhs_blocks[b] : rhs_contig + (size_t)b * BLEN,
+                BLEN * sizeof(double));
+    std::memcpy(hx + (size_t)b * BLEN, x_blocks ? x_blocks[b] : x_contig + (size_t)b * BLEN,
+                BLEN * sizeof(double));
+  }
+  int st = -1;
+  if (hipMemcpy(db, hb, bytes, hipMemcpyHostToDevice) == hipSuccess &&
+      hipMemcpy(dx, hx, bytes, hipMemcpyHostToDevice) == hipSuccess) {
+    st = cup3d_poisson_solve_device(H, db, dx, rep);
+    if (st >= 0 && hipMemcpy(hx, dx, bytes, hipMemcpyDeviceToHost) == hipSuccess) {
+      // scatter into the pressure blocks (.cpp:452-472)
+      for (int b = 0; b < H->nb; ++b)
+        std::memcpy(x_blocks ? x_blocks[b] : x_contig + (size_t)b * BLEN, hx + (size_t)b * BLEN,
+                    BLEN * sizeof(double));
+    } else if (st >= 0) {
+      st = fail(H, "hipMemcpy D2H failed");
+    }
+  } else {
+    st = fail(H, "hipMemcpy H2D failed");
+  }
+  (void)hipFree(db);
+  (void)hipFree(dx);
+  (void)hipHostFree(hb);
+  (void)hipHostFree(hx);
+  return st;
+}
+
+int cup3d_poisson_apply_operator(cup3d_poisson_t H, const double* d_x, double* d_y) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  HIPCHECK(hipSetDevice(H->device));
+  SolverState init = make_state(H->cfg);
+  *H->h_state = init;
+  HIPCHECK(hipMemcpyAsync(H->d_state.p, H->h_state, sizeof(SolverState), hipMemcpyHostToDevice,
+                          H->stream));
+  launch_spmv64(H->stream, SPMV_PLAIN, H->g64, d_x, d_y, nullptr, nullptr, nullptr,
+                H->d_state.as<SolverState>(), H->d_part.as<double>());
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(H->stream));
+  return 0;
+}
+
+int cup3d_poisson_apply_preconditioner(cup3d_poisson_t H, const double* d_p, double* d_z) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  HIPCHECK(hipSetDevice(H->device));
+  if (H->cfg.precision == CUP3D_PREC_FP64) {
+    KrylovVectors<double> v = H->v64;
+    v.p = const_cast<double*>(d_p);
+    v.z1 = d_z;
+    launch_precond64(H->stream, PREC_PLAIN, H->nb, H->d_P64.as<double>(), H->g64.invh, v,
+                     H->d_state.as<SolverState>());
+  } else {
+    if (mixed_apply_precond(H->mixed, H->stream, H->cfg.precision, H->g32, d_p, d_z, H->err) != 0)
+      return -1;
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(H->stream));
+  return 0;
+}
+
+int cup3d_cnn_num_weights(void) { return cnn_num_weights(); }
+
+int cup3d_poisson_cnn_set_weights(cup3d_poisson_t H, const float* w, int nw) {
+  if (!H) return -1;
+  HIPCHECK(hipSetDevice(H->device));
+  return cnn_set_weights(H->cnn, w, nw, H->err);
+}
+
+int cup3d_poisson_cnn_get_weights(cup3d_poisson_t H, float* w, int nw) {
+  if (!H) return -1;
+  HIPCHECK(hipSetDevice(H->device));
+  return cnn_get_weights(H->cnn, w, nw, H->err);
+}
+
+int cup3d_poisson_cnn_predict(cup3d_poisson_t H, const double* d_rhs, const double* d_prev,
+                              double* d_out) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  HIPCHECK(hipSetDevice(H->device));
+  return cnn_predict(H->cnn, H->stream, H->g64, d_rhs, d_prev, d_out, H->err);
+}
+
+int cup3d_poisson_cnn_train_step(cup3d_poisson_t H, const double* d_rhs, const double* d_prev,
+                                 const double* d_target, int nsamples, float lr, double* loss) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  HIPCHECK(hipSetDevice(H->device));
+  return cnn_train_step(H->cnn, H->stream, H->g64, d_rhs, d_prev, d_target, nsamples, lr, loss,
+                        H->err);
+}
+
+int cup3d_debug_time_kernel(cup3d_poisson_t H, int which, int reps, double* avg_ms,
+                            double* bytes, double* flops) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  if (reps <= 0 || which < 0 || which > 2) return fail(H, "bad arguments");
+  HIPCHECK(hipSetDevice(H->device));
+  hipStream_t s = H->stream;
+  SolverState init = make_state(H->cfg);
+  init.tol_abs = -1.0;  // the update kernel never declares convergence here
+  init.tol_rel = -1.0;
+  init.max_iter = 0x7fffffff;
+  init.max_restarts = 0;
+  init.err_init = 1.0;
+  init.iter = 1;
+  *H->h_state = init;
+  SolverState* st = H->d_state.as<SolverState>();
+  double* part = H->d_part.as<double>();
+  HIPCHECK(hipMemcpyAsync(st, H->h_state, sizeof(SolverState), hipMemcpyHostToDevice, s));
+  const double N = (double)H->n;
+  double by = 0, fl = 0;
+  if (H->cfg.precision != CUP3D_PREC_FP64) {
+    if (mixed_time_kernel(H->mixed, s, which, reps, H->g32, st, part, H->ev0, H->ev1, avg_ms, &by,
+                          &fl, H->err) != 0)
+      return -1;
+  } else {
+    auto launch = [&]() {
+      if (which == 0)
+        launch_spmv64(s, SPMV_PLAIN, H->g64, H->v64.z1, H->v64.nu, nullptr, nullptr, nullptr, st,
+                      part);
+      else if (which == 1)
+        launch_precond64(s, PREC_PLAIN, H->nb, H->d_P64.as<double>(), H->g64.invh, H->v64, st);
+      else
+        launch_update64(s, H->n, H->v64, st, part);
+    };
+    launch();  // warm
+    HIPCHECK(hipEventRecord(H->ev0, s));
+    for (int r = 0; r < reps; ++r) launch();
+    HIPCHECK(hipEventRecord(H->ev1, s));
+    HIPCHECK(hipEventSynchronize(H->ev1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, H->ev0, H->ev1));
+    *avg_ms = ms / reps;
+    if (which == 0) { by = 16.0 * N; fl = 8.0 * N; }
+    else if (which == 1) { by = 16.0 * N; fl = 1024.0 * N; }
+    else { by = 64.0 * N; fl = 8.0 * N; }
+  }
+  if (bytes) *bytes = by;
+  if (flops) *flops = fl;
+  return 0;
+}
+
+int64_t cup3d_poisson_num_interface_rows(cup3d_poisson_t H) {
+  return H ? (int64_t)H->topo.irow_cell.size() : -1;
+}
+int64_t cup3d_poisson_num_interface_entries(cup3d_poisson_t H) {
+  return H ? (int64_t)H->topo.ent_col.size() : -1;
+}
+
+void cup3d_poisson_destroy(cup3d_poisson_t H) {
+  if (!H) return;
+  (void)hipSetDevice(H->device);
+  (void)hipStreamSynchronize(H->stream);
+  cnn_free(H->cnn);
+  mixed_free(H->mixed);
+  if (H->h_state) (void)hipHostFree(H->h_state);
+  if (H->ev0) (void)hipEventDestroy(H->ev0);
+  if (H->ev1) (void)hipEventDestroy(H->ev1);
+  if (H->stream) (void)hipStreamDestroy(H->stream);
+  delete H;
+}
+
+const char* cup3d_poisson_last_error(cup3d_poisson_t H) {
+  return H ? H->err.c_str() : g_create_error.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,234 @@
+"""Python binding of libcup3d_poisson (the C ABI in include/cup3d_poisson.h).
+
+Used by tests and bench.py. Device vectors are torch tensors (float64,
+contiguous, on the solver's device): torch only provides the memory; every
+operation on them is a kernel of libcup3d_poisson. There is no fallback: if
+the library is missing or no GPU is present, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import load_library
+
+CONVERGED, MAXITER, BREAKDOWN, NONFINITE = 0, 1, 2, 3
+PRECISIONS = {"fp64": 0, "mixed_fp16": 1, "mixed_bf16": 2}
+GUESSES = {"given": 0, "zero": 1, "cnn": 2}
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("bpd", ctypes.c_int * 3), ("level_max", ctypes.c_int),
+                ("bc", ctypes.c_int * 3), ("h0", ctypes.c_double),
+                ("tol_abs", ctypes.c_double), ("tol_rel", ctypes.c_double),
+                ("tol_rhs", ctypes.c_double), ("bmean", ctypes.c_int),
+                ("max_iter", ctypes.c_int), ("max_restarts", ctypes.c_int),
+                ("precision", ctypes.c_int), ("guess", ctypes.c_int), ("device", ctypes.c_int),
+                ("inner_tol", ctypes.c_double), ("max_refine", ctypes.c_int),
+                ("poll_every", ctypes.c_int)]
+
+
+class BlockDesc(ctypes.Structure):
+    _fields_ = [("level", ctypes.c_int), ("index", ctypes.c_int * 3)]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("restarts", ctypes.c_int), ("refinements", ctypes.c_int),
+                ("err_init", ctypes.c_double), ("err_final", ctypes.c_double),
+                ("rhs_norm", ctypes.c_double), ("true_rel_resid", ctypes.c_double),
+                ("solve_ms", ctypes.c_double), ("guess_used", ctypes.c_int),
+                ("guess_resid", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# Every symbol declared in include/cup3d_poisson.h.
+EXPORTS = [
+    "cup3d_poisson_create", "cup3d_poisson_set_topology", "cup3d_poisson_solve",
+    "cup3d_poisson_solve_device", "cup3d_poisson_apply_operator",
+    "cup3d_poisson_apply_preconditioner", "cup3d_cnn_num_weights",
+    "cup3d_poisson_cnn_set_weights", "cup3d_poisson_cnn_get_weights",
+    "cup3d_poisson_cnn_predict", "cup3d_poisson_cnn_train_step",
+    "cup3d_poisson_num_interface_rows", "cup3d_poisson_num_interface_entries",
+    "cup3d_poisson_destroy", "cup3d_poisson_last_error",
+]
+
+_vp = ctypes.c_void_p
+
+
+def lib():
+    L = load_library("libcup3d_poisson.so")
+    if getattr(L, "_typed", False):
+        return L
+    L.cup3d_poisson_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(_vp)]
+    L.cup3d_poisson_set_topology.argtypes = [_vp, ctypes.c_int64, ctypes.POINTER(BlockDesc)]
+    L.cup3d_poisson_solve.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Report)]
+    L.cup3d_poisson_solve_device.argtypes = [_vp, _vp, _vp, ctypes.POINTER(Report)]
+    L.cup3d_poisson_apply_operator.argtypes = [_vp, _vp, _vp]
+    L.cup3d_poisson_apply_preconditioner.argtypes = [_vp, _vp, _vp]
+    L.cup3d_cnn_num_weights.argtypes = []
+    L.cup3d_poisson_cnn_set_weights.argtypes = [_vp, _vp, ctypes.c_int]
+    L.cup3d_poisson_cnn_get_weights.argtypes = [_vp, _vp, ctypes.c_int]
+    L.cup3d_poisson_cnn_predict.argtypes = [_vp, _vp, _vp, _vp]
+    L.cup3d_poisson_cnn_train_step.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_float,
+                                               ctypes.POINTER(ctypes.c_double)]
+    L.cup3d_poisson_num_interface_rows.argtypes = [_vp]
+    L.cup3d_poisson_num_interface_rows.restype = ctypes.c_int64
+    L.cup3d_poisson_num_interface_entries.argtypes = [_vp]
+    L.cup3d_poisson_num_interface_entries.restype = ctypes.c_int64
+    L.cup3d_poisson_destroy.argtypes = [_vp]
+    L.cup3d_poisson_destroy.restype = None
+    L.cup3d_poisson_last_error.argtypes = [_vp]
+    L.cup3d_poisson_last_error.restype = ctypes.c_char_p
+    L._typed = True
+    return L
+
+
+def _ptr(t):
+    import torch
+    assert isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class PoissonSolver:
+    """One solver handle: a grid topology plus device workspace."""
+
+    def __init__(self, bpd, h0, bc, level_max=0, tol_abs=0.0, tol_rel=1e-7, tol_rhs=0.0,
+                 bmean=1, max_iter=1000, max_restarts=100, precision="fp64", guess="given",
+                 device=0, inner_tol=1e-4, max_refine=10, poll_every=8):
+        self.L = lib()
+        c = Config()
+        for d in range(3):
+            c.bpd[d] = int(bpd[d])
+            c.bc[d] = int(bc[d])
+        c.level_max, c.h0 = int(level_max), float(h0)
+        c.tol_abs, c.tol_rel, c.tol_rhs = float(tol_abs), float(tol_rel), float(tol_rhs)
+        c.bmean, c.max_iter, c.max_restarts = int(bmean), int(max_iter), int(max_restarts)
+        c.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        c.guess = GUESSES[guess] if isinstance(guess, str) else int(guess)
+        c.device, c.inner_tol, c.max_refine = int(device), float(inner_tol), int(max_refine)
+        c.poll_every = int(poll_every)
+        self.cfg = c
+        h = _vp()
+        if self.L.cup3d_poisson_create(ctypes.byref(c), ctypes.byref(h)) != 0:
+            raise RuntimeError("cup3d_poisson_create: " +
+                               self.L.cup3d_poisson_last_error(None).decode())
+        self.h = h
+        self.nb = 0
+        self.device = device
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.cup3d_poisson_destroy(self.h)
+            self.h = None
+
+    def _err(self, what):
+        return RuntimeError(f"{what}: " + self.L.cup3d_poisson_last_error(self.h).decode())
+
+    def set_topology(self, blocks):
+        blocks = np.ascontiguousarray(blocks, np.int32).reshape(-1, 4)
+        arr = (BlockDesc * len(blocks))()
+        for i, (l, a, b, c) in enumerate(blocks):
+            arr[i].level = int(l)
+            arr[i].index[0], arr[i].index[1], arr[i].index[2] = int(a), int(b), int(c)
+        if self.L.cup3d_poisson_set_topology(self.h, len(blocks), arr) != 0:
+            raise self._err("set_topology")
+        self.nb = len(blocks)
+        return self
+
+    @property
+    def n(self):
+        return self.nb * 512
+
+    def interface_rows(self):
+        return self.L.cup3d_poisson_num_interface_rows(self.h)
+
+    def interface_entries(self):
+        return self.L.cup3d_poisson_num_interface_entries(self.h)
+
+    def solve_device(self, rhs, x):
+        """Solve in place on device tensors; x holds the guess. Returns the report dict."""
+        import torch
+        torch.cuda.synchronize(self.device)
+        rep = Report()
+        st = self.L.cup3d_poisson_solve_device(self.h, _ptr(rhs), _ptr(x), ctypes.byref(rep))
+        if st < 0:
+            raise self._err("solve_device")
+        return rep.as_dict()
+
+    def solve_host(self, rhs, x):
+        """Solve from host arrays (contiguous [nb][512] float64); x updated in place."""
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        assert x.dtype == np.float64 and x.flags.c_contiguous
+        rep = Report()
+        st = self.L.cup3d_poisson_solve(self.h, None, None, rhs.ctypes.data_as(_vp),
+                                        x.ctypes.data_as(_vp), ctypes.byref(rep))
+        if st < 0:
+            raise self._err("solve")
+        return rep.as_dict()
+
+    def solve_blocks(self, rhs_blocks, x_blocks):
+        """Solve over per-block host arrays (lists of 512-double arrays), the Cubism layout."""
+        nb = len(rhs_blocks)
+        rp = (ctypes.c_void_p * nb)(*[a.ctypes.data for a in rhs_blocks])
+        xp = (ctypes.c_void_p * nb)(*[a.ctypes.data for a in x_blocks])
+        rep = Report()
+        st = self.L.cup3d_poisson_solve(self.h, rp, xp, None, None, ctypes.byref(rep))
+        if st < 0:
+            raise self._err("solve")
+        return rep.as_dict()
+
+    def apply_operator(self, x, y):
+        import torch
+        torch.cuda.synchronize(self.device)
+        if self.L.cup3d_poisson_apply_operator(self.h, _ptr(x), _ptr(y)) != 0:
+            raise self._err("apply_operator")
+        return y
+
+    def apply_preconditioner(self, p, z):
+        import torch
+        torch.cuda.synchronize(self.device)
+        if self.L.cup3d_poisson_apply_preconditioner(self.h, _ptr(p), _ptr(z)) != 0:
+            raise self._err("apply_preconditioner")
+        return z
+
+    # learned warm start
+    def cnn_num_weights(self):
+        return self.L.cup3d_cnn_num_weights()
+
+    def cnn_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        if self.L.cup3d_poisson_cnn_set_weights(self.h, w.ctypes.data_as(_vp), len(w)) != 0:
+            raise self._err("cnn_set_weights")
+
+    def cnn_get_weights(self):
+        w = np.zeros(self.cnn_num_weights(), np.float32)
+        if self.L.cup3d_poisson_cnn_get_weights(self.h, w.ctypes.data_as(_vp), len(w)) != 0:
+            raise self._err("cnn_get_weights")
+        return w
+
+    def cnn_predict(self, rhs, prev, out):
+        import torch
+        torch.cuda.synchronize(self.device)
+        if self.L.cup3d_poisson_cnn_predict(self.h, _ptr(rhs), _ptr(prev), _ptr(out)) != 0:
+            raise self._err("cnn_predict")
+        return out
+
+    def cnn_train_step(self, rhs, prev, target, lr):
+        import torch
+        torch.cuda.synchronize(self.device)
+        ns = rhs.numel() // self.n
+        loss = ctypes.c_double()
+        if self.L.cup3d_poisson_cnn_train_step(self.h, _ptr(rhs), _ptr(prev), _ptr(target), ns,
+                                               float(lr), ctypes.byref(loss)) != 0:
+            raise self._err("cnn_train_step")
+        return loss.value
+
+
+def solver_for(workload, **kw):
+    """A PoissonSolver on the grid of a cup3d_amd.workloads.Workload."""
+    s = PoissonSolver(workload.bpd, workload.h0, workload.bc, workload.level_max, **kw)
+    return s.set_topology(workload.blocks)

@@ -1,0 +1,161 @@
+/*
+ * cup3d_poisson.h — C ABI of the MI355X pressure-Poisson solver for
+ * CubismUP_3D (libcup3d_poisson.so).
+ *
+ * Drop-in boundary. The reference selects its solver with
+ *   std::shared_ptr<PoissonSolverBase> makePoissonSolver(SimulationData&)
+ *     (source/poisson/PoissonSolverBase.cpp:30-52)
+ * and calls the argument-less
+ *   virtual void PoissonSolverBase::solve()
+ *     (source/poisson/PoissonSolverBase.h:10-17)
+ * once per time step from PressureProjection (PressureProjection.cpp:402).
+ * solve() reads the rhs from the blocks of sim.lhs and the initial guess from
+ * sim.pres, and writes the pressure back into sim.pres
+ * (PoissonSolverExp.cpp:375-418, 452-472). The functions below carry exactly
+ * that data across a plain C boundary: no C++ types, no exceptions, plain
+ * pointers and sizes. A C++ adapter implementing PoissonSolverBase on top of
+ * them is cup3d_amd/host/PoissonSolverHIP.{h,cpp}.
+ *
+ * Layout. A block is 8x8x8 doubles stored [iz][iy][ix] (Cubism ScalarBlock,
+ * source/Definitions.h:524-531). Blocks are numbered by the caller (Cubism's
+ * blockID); the flat cell index is blockID*512 + iz*64 + iy*8 + ix
+ * (PoissonSolverExp.h:69-70).
+ *
+ * Errors. Every function returns an int: >= 0 success (for solves: the
+ * status below), < 0 failure; cup3d_poisson_last_error() gives the message.
+ * Non-convergence is not an error: like the reference (BiCGSTAB.cu:612-625)
+ * the best iterate is returned with status CUP3D_MAXITER.
+ *
+ * Threading. One host thread per handle at a time; each handle owns one HIP
+ * stream on its device; calls block until the result is written.
+ */
+#ifndef CUP3D_POISSON_H
+#define CUP3D_POISSON_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cup3d_poisson* cup3d_poisson_t;
+
+/* BCflag, source/Definitions.h:38 */
+enum { CUP3D_BC_FREESPACE = 0, CUP3D_BC_PERIODIC = 1, CUP3D_BC_WALL = 2 };
+
+/* Solve status (return value of the solve calls). */
+enum {
+  CUP3D_CONVERGED = 0,  /* ||r|| <= tol_abs or ||r||/||r0|| <= tol_rel (BiCGSTAB.cu:595) */
+  CUP3D_MAXITER = 1,    /* iteration cap reached; best iterate returned (:612-625) */
+  CUP3D_BREAKDOWN = 2,  /* restarts exhausted (:496-499); best iterate returned */
+  CUP3D_NONFINITE = 3   /* residual became NaN/Inf; best iterate returned */
+};
+
+/* Arithmetic of the solve. */
+enum {
+  CUP3D_PREC_FP64 = 0,        /* drop-in: FP64 throughout */
+  CUP3D_PREC_MIXED_FP16 = 1,  /* FP16 P_inv MFMA, FP32 Krylov, FP64 refinement */
+  CUP3D_PREC_MIXED_BF16 = 2   /* BF16 P_inv MFMA, FP32 Krylov, FP64 refinement */
+};
+
+/* Initial guess. */
+enum {
+  CUP3D_GUESS_GIVEN = 0,  /* x on input is the guess (sim.pres, the reference contract) */
+  CUP3D_GUESS_ZERO = 1,   /* x on input ignored, guess = 0 */
+  CUP3D_GUESS_CNN = 2     /* learned warm start from (rhs, x on input = previous pressure);
+                             falls back to the given x when its residual is larger */
+};
+
+typedef struct {
+  int bpd[3];        /* blocks per direction at level 0 (-bpdx/-bpdy/-bpdz) */
+  int level_max;     /* -levelMax (informational: any level present in the block list is accepted) */
+  int bc[3];         /* BCflag per axis (-BC_x/-BC_y/-BC_z); freespace and wall are zero-flux */
+  double h0;         /* cell size at level 0 */
+  double tol_abs;    /* sim.PoissonErrorTol (-poissonTol), absolute on ||r||_2 */
+  double tol_rel;    /* sim.PoissonErrorTolRel (-poissonTolRel), relative to ||r0||_2 */
+  double tol_rhs;    /* extra absolute tolerance tol_rhs*||b||_2 (0 = off); the
+                        true-relative-residual stopping test used for benchmarks */
+  int bmean;         /* sim.bMeanConstraint (-bMeanConstraint): 0, 1, 2 or 3 */
+  int max_iter;      /* 1000 in the reference (BiCGSTAB.cu:474) */
+  int max_restarts;  /* 100 in the reference (PoissonSolverExp.cpp:435) */
+  int precision;     /* CUP3D_PREC_* */
+  int guess;         /* CUP3D_GUESS_* */
+  int device;        /* HIP device ordinal */
+  /* mixed precision only: inner FP32 solve tolerance relative to the
+     refinement residual, and the cap on FP64 refinement steps */
+  double inner_tol;
+  int max_refine;
+  int poll_every;    /* iterations between host polls of the device status (0 = default) */
+} cup3d_poisson_config;
+
+typedef struct {
+  int level;     /* BlockInfo::level */
+  int index[3];  /* BlockInfo::index */
+} cup3d_block_desc;
+
+typedef struct {
+  int status;            /* CUP3D_CONVERGED ... */
+  int iterations;        /* Krylov iterations (sum over refinement steps) */
+  int restarts;
+  int refinements;       /* FP64 refinement steps (1 for FP64) */
+  double err_init;       /* ||b - A x0||_2 (BiCGSTAB.cu:461) */
+  double err_final;      /* best recurrence residual ||r||_2 */
+  double rhs_norm;       /* ||b||_2 */
+  double true_rel_resid; /* ||b - A x||_2 / ||b||_2 recomputed in FP64 */
+  double solve_ms;       /* device time of the solve (events around it) */
+  int guess_used;        /* CUP3D_GUESS_* actually used (after CNN fallback) */
+  double guess_resid;    /* ||b - A x0||_2 of the guess used */
+} cup3d_solve_report;
+
+/* Create a solver on cfg->device. Returns 0 or < 0. */
+int cup3d_poisson_create(const cup3d_poisson_config* cfg, cup3d_poisson_t* out);
+
+/* Describe the grid: nblocks leaves in blockID order. Builds neighbour and
+ * coarse-fine interface tables and uploads them. Call again after every mesh
+ * change (the reference rebuilds its matrix when sim.pres->UpdateFluxCorrection
+ * is set, PoissonSolverExp.cpp:437-442). */
+int cup3d_poisson_set_topology(cup3d_poisson_t h, int64_t nblocks, const cup3d_block_desc* blocks);
+
+/* Host solve over Cubism block pointers: rhs_blocks[b] and x_blocks[b] each
+ * point at 512 doubles. If the pointer arrays are NULL, rhs and x are single
+ * contiguous [nblocks][512] arrays passed as rhs_contig / x_contig. */
+int cup3d_poisson_solve(cup3d_poisson_t h, const double* const* rhs_blocks,
+                        double* const* x_blocks, const double* rhs_contig, double* x_contig,
+                        cup3d_solve_report* rep);
+
+/* Device solve: d_rhs and d_x are device pointers to [nblocks][512] doubles
+ * (HBM-resident, on the handle's device). d_x holds the guess on input. */
+int cup3d_poisson_solve_device(cup3d_poisson_t h, const double* d_rhs, double* d_x,
+                               cup3d_solve_report* rep);
+
+/* y = A x with the solver's operator and bMean row (device pointers). */
+int cup3d_poisson_apply_operator(cup3d_poisson_t h, const double* d_x, double* d_y);
+
+/* z = blockdiag(h_b^-1 P_inv) p on the device, in the handle's precision. */
+int cup3d_poisson_apply_preconditioner(cup3d_poisson_t h, const double* d_p, double* d_z);
+
+/* Learned warm start (precision-independent). n_weights = cup3d_cnn_num_weights(). */
+int cup3d_cnn_num_weights(void);
+int cup3d_poisson_cnn_set_weights(cup3d_poisson_t h, const float* w, int n_weights);
+int cup3d_poisson_cnn_get_weights(cup3d_poisson_t h, float* w, int n_weights);
+/* d_out = net(d_rhs, d_prev) on all blocks (device pointers, doubles). */
+int cup3d_poisson_cnn_predict(cup3d_poisson_t h, const double* d_rhs, const double* d_prev,
+                              double* d_out);
+/* One Adam step on a batch of training triples (rhs, p_prev, p) held on the
+ * device as [nsamples][nblocks][512] doubles; returns the mean loss. */
+int cup3d_poisson_cnn_train_step(cup3d_poisson_t h, const double* d_rhs, const double* d_prev,
+                                 const double* d_target, int nsamples, float lr, double* loss);
+
+/* Introspection for tests and reports. */
+int64_t cup3d_poisson_num_interface_rows(cup3d_poisson_t h);
+int64_t cup3d_poisson_num_interface_entries(cup3d_poisson_t h);
+
+void cup3d_poisson_destroy(cup3d_poisson_t h);
+/* Last error message of this handle (or of the last failed create if h is NULL). */
+const char* cup3d_poisson_last_error(cup3d_poisson_t h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUP3D_POISSON_H */

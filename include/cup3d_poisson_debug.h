@@ -1,0 +1,43 @@
+/*
+ * cup3d_poisson_debug.h — host-only diagnostics of libcup3d_poisson.
+ *
+ * No counterpart in the reference. The reference assembles its operator as an
+ * explicit COO matrix (PoissonSolverExp::getMat, PoissonSolverExp.cpp:221-373);
+ * this library applies it matrix-free. cup3d_debug_operator_csr() writes the
+ * operator exactly as the SpMV kernel applies it (7-point stencil over
+ * same-level faces plus the merged coarse-fine interface rows, bMean = 3 row
+ * included; the bMean = 1 row replacement is applied at run time and is not
+ * part of it) as CSR with sorted columns, built on the host from the same
+ * topology tables the device uses. It needs no GPU.
+ *
+ * Call with cap = 0 to get nnz; then with arrays of n+1 / nnz / nnz entries.
+ * Returns nnz, or < 0 on invalid input.
+ */
+#ifndef CUP3D_POISSON_DEBUG_H
+#define CUP3D_POISSON_DEBUG_H
+
+#include <stdint.h>
+
+#include "cup3d_poisson.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int64_t cup3d_debug_operator_csr(const cup3d_poisson_config* cfg, int64_t nblocks,
+                                 const cup3d_block_desc* blocks, int64_t* rowptr, int32_t* col,
+                                 double* val, int64_t cap);
+
+/* Kernel timing for roofline reports: runs one kernel class `reps` times on
+ * the handle's stream between two HIP events, on the data of the last solve,
+ * and returns the average duration in ms. which: 0 = operator (SpMV),
+ * 1 = block-Jacobi GEMM, 2 = fused x/r update (K5), in the handle's
+ * precision. Also returns the algorithmic bytes and flops of one launch.
+ * Must follow a solve on the current topology. Returns 0 or < 0. */
+int cup3d_debug_time_kernel(cup3d_poisson_t h, int which, int reps, double* avg_ms,
+                            double* bytes, double* flops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,160 @@
+"""ctypes binding of the CPU oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker. The product path never loads it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_libs: dict = {}
+
+_i32p = ctypes.POINTER(ctypes.c_int)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+def _load(fast: bool = False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} missing: run `make -C oracle`")
+    lib = ctypes.CDLL(path)
+    lib.orc_create.restype = ctypes.c_void_p
+    lib.orc_create.argtypes = [_i32p, ctypes.c_int, _i32p, ctypes.c_double, ctypes.c_int,
+                               ctypes.c_int, _i32p]
+    lib.orc_destroy.argtypes = [ctypes.c_void_p]
+    lib.orc_nnz.restype = ctypes.c_int64
+    lib.orc_nnz.argtypes = [ctypes.c_void_p]
+    lib.orc_bmean_row.restype = ctypes.c_int
+    lib.orc_bmean_row.argtypes = [ctypes.c_void_p]
+    lib.orc_get_csr.argtypes = [ctypes.c_void_p, _i64p, _i64p, _f64p]
+    lib.orc_get_pinv.argtypes = [ctypes.c_void_p, _f64p]
+    lib.orc_pinv.argtypes = [_f64p]
+    for f in ("orc_spmv", "orc_spmv_raw", "orc_precond", "orc_block_cg", "orc_lhs_cpu"):
+        getattr(lib, f).argtypes = [ctypes.c_void_p, _f64p, _f64p]
+    lib.orc_bicgstab.restype = ctypes.c_int
+    lib.orc_bicgstab.argtypes = [ctypes.c_void_p, _f64p, _f64p, _f64p, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_int, ctypes.c_int, _i32p, _f64p,
+                                 _f64p]
+    lib.orc_pipelined.restype = ctypes.c_int
+    lib.orc_pipelined.argtypes = [ctypes.c_void_p, _f64p, _f64p, _f64p, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int, _i32p, _f64p]
+    lib.orc_last_error.restype = ctypes.c_char_p
+    lib.orc_num_threads.restype = ctypes.c_int
+    _libs[name] = lib
+    return lib
+
+
+def _p(a, t=ctypes.c_double):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def pinv() -> np.ndarray:
+    """P_inv (512x512, row-major) by Cholesky-Crout + Gauss-Jordan."""
+    P = np.zeros((512, 512), np.float64)
+    _load().orc_pinv(_p(P))
+    return P
+
+
+class Oracle:
+    """FP64 CPU restatement of the reference solver on one block grid."""
+
+    def __init__(self, blocks, bpd, h0, bc, bmean=1, level_max=0, fast=False):
+        self.lib = _load(fast)
+        self.blocks = np.ascontiguousarray(blocks, np.int32)
+        bpd = np.ascontiguousarray(bpd, np.int32)
+        bc = np.ascontiguousarray(bc, np.int32)
+        self.n = len(self.blocks) * 512
+        self.h = h0 / (2.0 ** self.blocks[:, 0])
+        self.bmean = bmean
+        self.obj = self.lib.orc_create(_p(bpd, ctypes.c_int), level_max, _p(bc, ctypes.c_int),
+                                       h0, bmean, len(self.blocks), _p(self.blocks, ctypes.c_int))
+        if not self.obj:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "obj", None):
+            self.lib.orc_destroy(self.obj)
+            self.obj = None
+
+    @property
+    def bmean_row(self) -> int:
+        return self.lib.orc_bmean_row(self.obj)
+
+    def csr(self):
+        nnz = self.lib.orc_nnz(self.obj)
+        rowptr = np.zeros(self.n + 1, np.int64)
+        col = np.zeros(nnz, np.int64)
+        val = np.zeros(nnz, np.float64)
+        self.lib.orc_get_csr(self.obj, _p(rowptr, ctypes.c_int64), _p(col, ctypes.c_int64),
+                             _p(val))
+        return rowptr, col, val
+
+    def scipy_matrix(self):
+        import scipy.sparse as sp
+        rowptr, col, val = self.csr()
+        return sp.csr_matrix((val, col, rowptr), shape=(self.n, self.n))
+
+    def _vec(self, x):
+        return np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1))
+
+    def _apply(self, fn, x):
+        x = self._vec(x)
+        y = np.zeros(self.n, np.float64)
+        getattr(self.lib, fn)(self.obj, _p(x), _p(y))
+        return y
+
+    def spmv(self, x):
+        """y = A x with the GPU solver's bMean row (BiCGSTAB.cu:392-413)."""
+        return self._apply("orc_spmv", x)
+
+    def spmv_raw(self, x):
+        return self._apply("orc_spmv_raw", x)
+
+    def precond(self, p):
+        return self._apply("orc_precond", p)
+
+    def block_cg(self, p):
+        return self._apply("orc_block_cg", p)
+
+    def lhs_cpu(self, x):
+        return self._apply("orc_lhs_cpu", x)
+
+    def bicgstab(self, rhs, x0=None, tol_abs=0.0, tol_rel=1e-7, max_restarts=100,
+                 max_iter=1000, history=False):
+        rhs = self._vec(rhs)
+        x0 = None if x0 is None else self._vec(x0)
+        x = np.zeros(self.n, np.float64)
+        ir = np.zeros(2, np.int32)
+        errs = np.zeros(2, np.float64)
+        hist = np.zeros(max_iter, np.float64) if history else None
+        st = self.lib.orc_bicgstab(self.obj, _p(rhs), _p(x0), _p(x), tol_abs, tol_rel,
+                                   max_restarts, max_iter, _p(ir, ctypes.c_int), _p(errs),
+                                   _p(hist))
+        info = dict(status=st, iters=int(ir[0]), restarts=int(ir[1]), err_init=float(errs[0]),
+                    err_final=float(errs[1]))
+        if history:
+            info["history"] = hist[: info["iters"]].copy()
+        return x, info
+
+    def pipelined(self, rhs, x0=None, tol_abs=0.0, tol_rel=1e-7, max_iter=1000):
+        rhs = self._vec(rhs)
+        x0 = None if x0 is None else self._vec(x0)
+        x = np.zeros(self.n, np.float64)
+        ir = np.zeros(2, np.int32)
+        errs = np.zeros(2, np.float64)
+        st = self.lib.orc_pipelined(self.obj, _p(rhs), _p(x0), _p(x), tol_abs, tol_rel, max_iter,
+                                    _p(ir, ctypes.c_int), _p(errs))
+        return x, dict(status=st, iters=int(ir[0]), restarts=int(ir[1]),
+                       err_init=float(errs[0]), err_final=float(errs[1]))
+
+
+def num_threads(fast=True) -> int:
+    return _load(fast).orc_num_threads()

@@ -1,0 +1,120 @@
+"""Small grids and fields shared by the tests."""
+from __future__ import annotations
+
+import numpy as np
+
+from cup3d_amd import workloads as W
+
+PER, WALL, FREE = W.BC_PERIODIC, W.BC_WALL, W.BC_FREESPACE
+
+
+def uniform(bpd, bc, h0=None):
+    h0 = h0 if h0 is not None else 1.0 / (8 * max(bpd))
+    return W.uniform_blocks(*bpd), tuple(bpd), h0, tuple(bc)
+
+
+def amr_sphere(bpd=(4, 4, 4), lmax=1, thr=(0.15,), centre=(0.5, 0.5, 0.5), r=0.2, bc=(2, 2, 2),
+               h0=None):
+    h0 = h0 if h0 is not None else 1.0 / (8 * max(bpd))
+    blocks = W.amr_sphere_blocks(bpd, h0, lmax, thr, centre, r)
+    return blocks, tuple(bpd), h0, tuple(bc)
+
+
+# A family of grids that exercise same-level, coarse and fine faces in all six
+# directions, periodic wrap across coarse-fine faces, and 3 levels.
+def grid_cases():
+    return {
+        "uniform_periodic_2x3x2": uniform((2, 3, 2), (PER, PER, PER)),
+        "uniform_walls_3x2x2": uniform((3, 2, 2), (WALL, WALL, WALL)),
+        "uniform_mixed_bc": uniform((2, 2, 3), (PER, WALL, FREE)),
+        "amr2_walls": amr_sphere((4, 4, 4), 1, (0.15,), (0.5, 0.5, 0.5), 0.2, (WALL,) * 3),
+        "amr2_periodic_offcentre": amr_sphere((4, 4, 4), 1, (0.12,), (0.1, 0.9, 0.15), 0.15,
+                                              (PER,) * 3),
+        "amr3_walls": amr_sphere((2, 2, 2), 2, (0.3, 0.12), (0.45, 0.55, 0.5), 0.15, (WALL,) * 3,
+                                 h0=1.0 / 16),
+    }
+
+
+def random_field(nb, seed=0):
+    return np.random.default_rng(seed).standard_normal((nb, 8, 8, 8))
+
+
+def cell_centres(blocks, h0):
+    """Cell-centre coordinates [nb, 8, 8, 8, 3] and spacing h[nb]."""
+    blocks = np.asarray(blocks)
+    h = h0 / (2.0 ** blocks[:, 0])
+    i = np.arange(8) + 0.5
+    iz, iy, ix = np.meshgrid(i, i, i, indexing="ij")
+    X = (blocks[:, 1, None, None, None] * 8 + ix[None]) * h[:, None, None, None]
+    Y = (blocks[:, 2, None, None, None] * 8 + iy[None]) * h[:, None, None, None]
+    Z = (blocks[:, 3, None, None, None] * 8 + iz[None]) * h[:, None, None, None]
+    return np.stack([X, Y, Z], -1), h
+
+
+def boundary_cells(blocks, bpd, bc):
+    """Mask of cells that sit on a non-periodic domain face."""
+    blocks = np.asarray(blocks)
+    m = np.zeros((len(blocks), 8, 8, 8), bool)
+    for b, (l, i, j, k) in enumerate(blocks):
+        I = (i, j, k)
+        for d in range(3):
+            if bc[d] == PER:
+                continue
+            n = bpd[d] << l
+            sl_lo = [slice(None)] * 3
+            sl_hi = [slice(None)] * 3
+            ax = 2 - d  # arrays are [iz][iy][ix]
+            sl_lo[ax] = 0
+            sl_hi[ax] = 7
+            if I[d] == 0:
+                m[b][tuple(sl_lo)] = True
+            if I[d] == n - 1:
+                m[b][tuple(sl_hi)] = True
+    return m
+
+
+def tiny_amr(periodic=False, three_levels=False):
+    """2x2x2 base blocks with block (0,0,0) refined (15 blocks); with
+    three_levels (walls only) its child at the origin is refined again."""
+    h0 = 1.0 / 16
+    fine = [[1, i, j, k] for k in range(2) for j in range(2) for i in range(2)]
+    if three_levels:
+        fine = fine[1:] + [[2, i, j, k] for k in range(2) for j in range(2) for i in range(2)]
+    coarse = [[0, 1, 0, 0], [0, 0, 1, 0], [0, 1, 1, 0], [0, 0, 0, 1], [0, 1, 0, 1],
+              [0, 0, 1, 1], [0, 1, 1, 1]]
+    blocks = np.array(fine + coarse, np.int32)
+    bc = (PER,) * 3 if periodic else (WALL,) * 3
+    return blocks, (2, 2, 2), h0, bc
+
+
+def global_to_blocks(g, blocks):
+    return np.stack([g[8 * k:8 * k + 8, 8 * j:8 * j + 8, 8 * i:8 * i + 8]
+                     for (_, i, j, k) in blocks])
+
+
+def blocks_to_global(f, blocks, bpd):
+    nx, ny, nz = (8 * b for b in bpd)
+    g = np.zeros((nz, ny, nx))
+    for b, (_, i, j, k) in enumerate(blocks):
+        g[8 * k:8 * k + 8, 8 * j:8 * j + 8, 8 * i:8 * i + 8] = f[b]
+    return g
+
+
+def fft_solution_periodic(rhs, blocks, bpd, h0, row):
+    """Exact solution of the bMean = 1 system on a uniform periodic grid:
+    A x = b on every row but the origin row, sum x = 0. Uses the Fourier
+    eigenvalues h (2 cos(2 pi m_x/n_x) - 2 + ...)."""
+    b = np.asarray(rhs, np.float64).reshape(len(blocks), 8, 8, 8).copy()
+    flat = b.reshape(-1)
+    flat[row] = 0.0
+    flat[row] = -flat.sum()
+    g = blocks_to_global(b, blocks, bpd)
+    n = g.shape
+    lam = sum(np.reshape(2 * np.cos(2 * np.pi * np.arange(n[a]) / n[a]) - 2,
+                         [-1 if d == a else 1 for d in range(3)]) for a in range(3)) * h0
+    gh = np.fft.fftn(g)
+    lam[0, 0, 0] = 1.0
+    xh = gh / lam
+    xh[0, 0, 0] = 0.0
+    x = np.real(np.fft.ifftn(xh))
+    return global_to_blocks(x, blocks).reshape(-1)

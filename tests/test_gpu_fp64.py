@@ -1,0 +1,176 @@
+"""FP64 drop-in on the GPU vs the CPU oracle, through the C ABI.
+
+Bars: operator and preconditioner <= 1e-13 relative (max norm); solver
+<= 1e-10 relative L2 in pressure at the same iteration count wherever the
+recurrence does not amplify rounding (bMean = 0 consistent systems; the first
+iterations of bMean = 1), and for converged bMean = 1 solves both the GPU and
+the oracle within the tolerance-implied distance of the exact solution.
+"""
+import numpy as np
+import pyoracle
+import pytest
+import torch
+
+from cup3d_amd import workloads as W
+from cup3d_amd.poisson import PoissonSolver, solver_for
+from helpers import fft_solution_periodic, grid_cases, random_field, tiny_amr
+
+pytestmark = pytest.mark.gpu
+
+CASES = dict(grid_cases())
+CASES["tiny_amr3_walls"] = tiny_amr(three_levels=True)
+CASES["tiny_amr_periodic"] = tiny_amr(periodic=True)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64).reshape(-1)).cuda()
+
+
+def compatible_rhs(blocks, h0, seed):
+    h3 = np.repeat((h0 / 2.0 ** np.asarray(blocks)[:, 0]) ** 3, 512)
+    b = random_field(len(blocks), seed).reshape(-1)
+    return b - h3 * b.sum() / h3.sum()
+
+
+def pair(name, bmean, **kw):
+    blocks, bpd, h0, bc = CASES[name]
+    S = PoissonSolver(bpd, h0, bc, bmean=bmean, **kw).set_topology(blocks)
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, bmean)
+    return S, O, blocks, h0
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("bmean", [0, 1, 3])
+def test_operator_matches_oracle(gpu, name, bmean):
+    S, O, blocks, h0 = pair(name, bmean)
+    x = random_field(len(blocks), 1).reshape(-1)
+    y = S.apply_operator(dev(x), torch.zeros(O.n, dtype=torch.float64, device=gpu))
+    yo = O.spmv(x)
+    assert torch.isfinite(y).all()
+    assert np.abs(y.cpu().numpy() - yo).max() <= 1e-13 * np.abs(yo).max()
+
+
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "amr3_walls", "amr2_periodic_offcentre"])
+def test_preconditioner_matches_oracle(gpu, name):
+    S, O, blocks, h0 = pair(name, 1)
+    p = random_field(len(blocks), 2).reshape(-1)
+    z = S.apply_preconditioner(dev(p), torch.zeros(O.n, dtype=torch.float64, device=gpu))
+    zo = O.precond(p)
+    assert np.abs(z.cpu().numpy() - zo).max() <= 1e-13 * np.abs(zo).max()
+
+
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "uniform_periodic_2x3x2", "amr2_walls",
+                                  "amr3_walls", "tiny_amr_periodic"])
+def test_solve_bmean0_same_iterations(gpu, name):
+    S, O, blocks, h0 = pair(name, 0, tol_rel=1e-7)
+    b = compatible_rhs(blocks, h0, 3)
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(dev(b), x)
+    xo, info = O.bicgstab(b, tol_rel=1e-7)
+    assert rep["status"] == info["status"] == 0
+    assert rep["iterations"] == info["iters"] and rep["restarts"] == info["restarts"]
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= 1e-10
+    assert abs(rep["err_init"] - info["err_init"]) <= 1e-13 * info["err_init"]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_solve_bmean1_first_iterations(gpu, name):
+    """bMean = 1 amplifies rounding as iterations accumulate (the oracle shows
+    it against itself under 1e-15 rhs perturbations); the first iterates agree."""
+    S, O, blocks, h0 = pair(name, 1, tol_rel=0.0, max_iter=6)
+    b = compatible_rhs(blocks, h0, 4)
+    x0 = random_field(len(blocks), 5).reshape(-1) * 1e-3
+    x = dev(x0)
+    rep = S.solve_device(dev(b), x)
+    xo, info = O.bicgstab(b, x0=x0, tol_rel=0.0, max_iter=6)
+    assert rep["status"] == info["status"] == 1 and rep["iterations"] == info["iters"] == 6
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= 1e-10
+    assert abs(rep["err_final"] - info["err_final"]) <= 1e-10 * info["err_final"]
+
+
+def test_solve_bmean1_converged_c1(gpu):
+    """32^3 periodic Fourier workload, reference default settings: both
+    solvers converge, and both land within the tolerance-implied distance of
+    the exact (FFT) solution."""
+    w = W.c1_periodic_32()
+    S = solver_for(w, tol_rel=1e-7)
+    O = pyoracle.Oracle(w.blocks, w.bpd, w.h0, w.bc, 1)
+    b = w.rhs().reshape(-1)
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(dev(b), x)
+    xo, info = O.bicgstab(b, tol_rel=1e-7)
+    assert rep["status"] == 0 and info["status"] == 0
+    assert abs(rep["iterations"] - info["iters"]) <= max(3, info["iters"] // 5)
+    xe = fft_solution_periodic(b, w.blocks, w.bpd, w.h0, O.bmean_row)
+    for sol in (x.cpu().numpy(), xo):
+        assert np.linalg.norm(sol - xe) / np.linalg.norm(xe) < 1e-5
+    assert rep["true_rel_resid"] < 2e-7 * rep["err_init"] / rep["rhs_norm"]
+
+
+def test_host_paths_match_device_path(gpu):
+    name = "amr2_walls"
+    S, O, blocks, h0 = pair(name, 1, tol_rel=1e-7)
+    b = compatible_rhs(blocks, h0, 6)
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(dev(b), x)
+    xh = np.zeros(O.n)
+    reph = S.solve_host(b, xh)
+    rb = [b[i * 512:(i + 1) * 512].copy() for i in range(len(blocks))]
+    xb = [np.zeros(512) for _ in range(len(blocks))]
+    repb = S.solve_blocks(rb, xb)
+    assert np.array_equal(x.cpu().numpy(), xh)
+    assert np.array_equal(xh, np.concatenate(xb))
+    assert rep["iterations"] == reph["iterations"] == repb["iterations"]
+
+
+def test_deterministic(gpu):
+    S, O, blocks, h0 = pair("amr3_walls", 1, tol_rel=1e-7)
+    b = dev(compatible_rhs(blocks, h0, 7))
+    xs = []
+    for _ in range(3):
+        x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+        S.solve_device(b, x)
+        xs.append(x.cpu().numpy())
+    assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
+
+
+def test_zero_rhs_matches_reference_semantics(gpu):
+    """r0 = 0: the reference never sees an improvement (error < error_opt is
+    false at 0) and runs to max_iter returning x_opt = x0 (BiCGSTAB.cu:590)."""
+    S, O, blocks, h0 = pair("uniform_walls_3x2x2", 1, tol_rel=1e-7, max_iter=20)
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(torch.zeros(O.n, dtype=torch.float64, device=gpu), x)
+    xo, info = O.bicgstab(np.zeros(O.n), tol_rel=1e-7, max_iter=20)
+    assert rep["status"] == info["status"] == 1
+    assert rep["iterations"] == info["iters"] == 20
+    assert (x == 0).all() and (xo == 0).all()
+
+
+def test_nonfinite_rhs_reports_status(gpu):
+    S, O, blocks, h0 = pair("uniform_walls_3x2x2", 1, tol_rel=1e-7)
+    b = compatible_rhs(blocks, h0, 8)
+    b[100] = np.nan
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(dev(b), x)
+    assert rep["status"] == 3
+
+
+def test_c2_full_size_properties(gpu):
+    """256x128x128 walls workload at full size: converges to the true relative
+    residual asked for, deterministically, with the mean constraint met."""
+    w = W.c2_uniform_walls()
+    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-7)
+    b = dev(w.rhs(step=1))
+    runs = []
+    for _ in range(2):
+        x = torch.zeros(w.ncells, dtype=torch.float64, device=gpu)
+        rep = S.solve_device(b, x)
+        runs.append((rep, x.clone()))
+    rep, x = runs[0]
+    assert rep["status"] == 0
+    assert rep["true_rel_resid"] <= 1.05e-7
+    assert torch.equal(runs[0][1], runs[1][1])
+    # the mean-constraint row is part of the residual: |sum h^3 x| <= ||r||
+    assert abs(float(x.sum()) * w.h0 ** 3) <= 1.05e-7 * rep["rhs_norm"]

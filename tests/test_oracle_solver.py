@@ -1,0 +1,163 @@
+"""Pins the oracle's solvers: classical BiCGSTAB (BiCGSTAB.cu:416-628), the
+pipelined CPU solver (PoissonSolverAMR.cpp:13-358) and the block-CG
+preconditioner (PoissonSolverAMRKernels.cpp:268-332)."""
+import numpy as np
+import pyoracle
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+from cup3d_amd import workloads as W
+from helpers import (PER, WALL, fft_solution_periodic, grid_cases, random_field, tiny_amr,
+                     uniform)
+
+
+def constrained_system(O, rhs):
+    """The system the GPU solver solves for bMean = 1: the origin row is
+    replaced by sum_b h_b^3 x (BiCGSTAB.cu:406-407), b and x zeroed there."""
+    A = O.scipy_matrix().tolil()
+    row = O.bmean_row
+    A[row, :] = np.repeat(O.h ** 3, 512)[None, :]
+    b = np.asarray(rhs, np.float64).reshape(-1).copy()
+    b[row] = 0.0
+    return A.tocsr(), b
+
+
+def direct_solution(O, rhs):
+    """Exact solution of the bMean = 1 system for a compatible rhs (sum b = 0,
+    1^T A = 0): pin the origin cell, solve the remaining sparse system
+    directly, then shift by a constant to meet sum h^3 x = 0."""
+    A = O.scipy_matrix().tocsr()
+    row = O.bmean_row
+    keep = np.ones(O.n, bool)
+    keep[row] = False
+    b = np.asarray(rhs, np.float64).reshape(-1)
+    xk = sla.spsolve(A[keep][:, keep].tocsc(), b[keep])
+    x = np.zeros(O.n)
+    x[keep] = xk
+    h3 = np.repeat(O.h ** 3, 512)
+    return x - (h3 @ x) / h3.sum()
+
+
+def compatible_rhs(O, seed):
+    b = random_field(len(O.blocks), seed).reshape(-1)
+    h3 = np.repeat(O.h ** 3, 512)
+    return b - h3 * b.sum() / h3.sum()
+
+
+SMALL = {
+    "uniform_periodic_2x3x2": lambda: grid_cases()["uniform_periodic_2x3x2"],
+    "uniform_walls_3x2x2": lambda: grid_cases()["uniform_walls_3x2x2"],
+    "tiny_amr_walls": lambda: tiny_amr(),
+    "tiny_amr_periodic": lambda: tiny_amr(periodic=True),
+    "tiny_amr3_walls": lambda: tiny_amr(three_levels=True),
+}
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+def test_bicgstab_matches_spsolve(name):
+    blocks, bpd, h0, bc = SMALL[name]()
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 1)
+    rhs = compatible_rhs(O, 3)
+    rhs[O.bmean_row] = 0.0  # getVec zeroes it; keep sum(b) = 0 for the direct solve
+    rhs -= np.repeat(O.h ** 3, 512) * rhs.sum() / np.repeat(O.h ** 3, 512).sum()
+    x, info = O.bicgstab(rhs, tol_abs=0.0, tol_rel=1e-11)
+    assert info["status"] == 0
+    xs = direct_solution(O, rhs)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
+
+
+def test_bicgstab_matches_scipy_recurrence():
+    """scipy's bicgstab is the right-preconditioned algorithm of the Templates
+    book, the same recurrence as BiCGSTAB.cu:474-608. With the same operator,
+    M and x0 = 0 the residual histories agree while rounding has not been
+    amplified (the bMean = 1 system amplifies it: later iterates diverge for
+    both solvers alike), and both end at the requested tolerance."""
+    w = W.c1_periodic_32()
+    O = pyoracle.Oracle(w.blocks, w.bpd, w.h0, w.bc, 1)
+    rhs = w.rhs().reshape(-1)
+    b = rhs.copy()
+    b[O.bmean_row] = 0.0
+    n = O.n
+    A = sla.LinearOperator((n, n), matvec=O.spmv)
+    M = sla.LinearOperator((n, n), matvec=O.precond)
+    x, info = O.bicgstab(rhs, tol_abs=0.0, tol_rel=1e-8, history=True)
+    hist = []
+    xs, flag = sla.bicgstab(A, b, rtol=1e-8, atol=0.0, M=M, maxiter=1000,
+                            callback=lambda xk: hist.append(np.linalg.norm(b - O.spmv(xk))))
+    assert flag == 0 and info["status"] == 0
+    h = info["history"]
+    assert np.allclose(h[:10], hist[:10], rtol=1e-10, atol=0)
+    xd = fft_solution_periodic(b, w.blocks, w.bpd, w.h0, O.bmean_row)
+    assert np.linalg.norm(O.spmv(xd) - b) / np.linalg.norm(b) < 1e-12
+    for sol in (x, xs):
+        assert np.linalg.norm(sol - xd) / np.linalg.norm(xd) < 1e-5
+
+
+def test_bicgstab_deterministic_and_history():
+    blocks, bpd, h0, bc = grid_cases()["amr2_walls"]
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 1)
+    rhs = compatible_rhs(O, 5)
+    x1, i1 = O.bicgstab(rhs, tol_rel=1e-7, history=True)
+    x2, i2 = O.bicgstab(rhs, tol_rel=1e-7, history=True)
+    assert np.array_equal(x1, x2) and i1 == {**i2, "history": i1["history"]}
+    h = i1["history"]
+    assert h[-1] / i1["err_init"] <= 1e-7 and (h[:-1] / i1["err_init"] > 1e-7).all()
+    # the returned iterate is x_opt: its true residual matches the recurrence
+    A, b = constrained_system(O, rhs)
+    assert np.linalg.norm(b - A @ x1) / i1["err_init"] < 2e-7
+
+
+def test_bicgstab_maxiter_returns_best_iterate():
+    blocks, bpd, h0, bc = grid_cases()["uniform_walls_3x2x2"]
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 1)
+    rhs = compatible_rhs(O, 7)
+    x, info = O.bicgstab(rhs, tol_rel=1e-14, max_iter=4, history=True)
+    assert info["status"] == 1 and info["iters"] == 4
+    assert info["err_final"] == info["history"].min()
+
+
+def test_bicgstab_bmean3_row():
+    blocks, bpd, h0, bc = grid_cases()["uniform_periodic_2x3x2"]
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 3)
+    rhs = compatible_rhs(O, 9)
+    x, info = O.bicgstab(rhs, tol_rel=1e-13)
+    assert info["status"] == 0
+    assert abs(x[0]) < 1e-10  # row h*x = 0 pins the origin cell
+    A = O.scipy_matrix()
+    b = rhs.copy()
+    b[0] = 0
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "amr2_walls"])
+def test_pipelined_matches_classical(name):
+    blocks, bpd, h0, bc = grid_cases()[name]
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 1)
+    rhs = compatible_rhs(O, 11)
+    xc, ic = O.bicgstab(rhs, tol_rel=1e-12)
+    xp, ip = O.pipelined(rhs, tol_rel=1e-10)
+    assert ip["status"] == 0
+    assert np.linalg.norm(xp - xc) / np.linalg.norm(xc) < 1e-7
+
+
+def test_block_cg_approximates_exact_inverse():
+    blocks, bpd, h0, bc = grid_cases()["amr3_walls"]
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 1)
+    p = random_field(len(blocks), 13).reshape(-1)
+    z_exact = O.precond(p)
+    z_cg = O.block_cg(p)
+    assert np.linalg.norm(z_cg - z_exact) / np.linalg.norm(z_exact) < 1e-5
+
+
+def test_precond_is_scaled_local_inverse():
+    """M^-1 applied to A_loc-shaped data of one block returns it (P_inv = A_loc^-1)."""
+    blocks, bpd, h0, bc = uniform((2, 2, 2), (WALL,) * 3)
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
+    P = pyoracle.pinv()
+    z = np.random.default_rng(2).standard_normal(512)
+    p = np.zeros(O.n)
+    p[512:1024] = h0 * np.linalg.solve(P, z)  # p_b = h P^-1 z  ->  M^-1 p_b = h^-1 P p_b = z
+    out = O.precond(p)
+    assert np.abs(out[512:1024] - z).max() < 1e-12
+    assert np.abs(out[:512]).max() == 0
