@@ -80,11 +80,17 @@ def build_adapter():
     return out
 
 
+def build_example():
+    out = os.path.join(LIB, "example_projection")
+    _run([CXX, "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, "-I", HOST,
+          os.path.join(HOST, "example_projection.cpp"), "-o", out + ".tmp",
+          "-L", LIB, "-lcup3d_adapter", "-lcup3d_poisson", "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
-    outs = [build_workloads(), build_poisson()]
-    if os.path.exists(os.path.join(HOST, "PoissonSolverHIP.cpp")):
-        outs.append(build_adapter())
-    return outs
+    return [build_workloads(), build_poisson(), build_adapter(), build_example()]
 
 
 if __name__ == "__main__":

@@ -23,291 +23,11 @@
 
 #include "device_common.h"
 #include "kernels.h"
+#include "krylov.h"
 
 namespace cup3d {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
-
-// ---------------------------------------------------------------------------
-// Scalar state machine (single thread of a last workgroup).
-// ---------------------------------------------------------------------------
-
-// Start of iteration k: the breakdown test, beta and the restart of
-// BiCGSTAB.cu:477-515, and the iteration cap of :474-475.
-__device__ void begin_iteration(SolverState* st, double rho, double rr) {
-  if (st->iter >= st->max_iter) {
-    st->status = ST_MAXITER;
-    return;
-  }
-  st->rho_curr = rho;
-  st->rr = rr;
-  const bool serious = rho * rho < 1e-16 * rr * st->rhrh;
-  st->beta = (st->rho_curr / (st->rho_prev + st->eps)) * (st->alpha / (st->omega + st->eps));
-  if (serious && st->max_restarts > 0) {
-    st->restarts += 1;
-    if (st->restarts >= st->max_restarts) {
-      st->status = ST_BREAKDOWN;
-      return;
-    }
-    st->restart_pending = 1;  // rhat = r (done by K1)
-    st->p_zero = 1;           // p = nu = 0
-    st->rho_curr = rr;        // ||rhat||^2 after rhat = r
-    st->rhrh = rr;
-    st->rho_prev = 1.0;       // breakdown_update (:263-269)
-    st->alpha = 1.0;
-    st->omega = 1.0;
-    st->beta = (st->rho_curr / (st->rho_prev + st->eps)) * (st->alpha / (st->omega + st->eps));
-  }
-  st->iter += 1;
-}
-
-// ---------------------------------------------------------------------------
-// getVec (PoissonSolverExp.cpp:375-418): b = rhs, x = x0, both zeroed at the
-// bMean cell for bMeanConstraint 1 and 3; also ||b||^2.
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(WG_THREADS) void k_getvec(long long n, long long zero_row,
-                                                       const double* __restrict__ rhs,
-                                                       const double* __restrict__ x0,
-                                                       T* __restrict__ b, T* __restrict__ x,
-                                                       SolverState* st, double* part,
-                                                       double tol_b) {
-  double v[1] = {0.0};
-  const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
-  for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
-    const long long i = base + e;
-    if (i < n) {
-      double bv = rhs[i], xv = x0 ? x0[i] : 0.0;
-      if (i == zero_row) bv = xv = 0.0;
-      b[i] = (T)bv;
-      x[i] = (T)xv;
-      v[0] += bv * bv;
-    }
-  }
-  double tot[1];
-  if (grid_reduce_last<1>(v, part, &st->counters[0], tot) && threadIdx.x == 0) {
-    st->bnorm2 = tot[0];
-    const double tb = tol_b * sqrt(tot[0]);
-    if (tb > st->tol_abs) st->tol_abs = tb;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Matrix-free operator y = A x (PoissonSolverExp.cpp:305-367 + interface rows)
-// with the bMean handling of hd_cusparseSpMV (BiCGSTAB.cu:392-413) and a
-// fused reduction epilogue.
-//
-// One wavefront per block: lane = (ix, iy), each lane walks its z column in
-// registers; the block is staged in LDS for the x/y neighbours; face halos
-// of same-level neighbours are read straight from HBM (the z faces as
-// coalesced 512-byte rows).
-// ---------------------------------------------------------------------------
-template <typename T, int MODE, int BM>
-__global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __restrict__ x,
-                                                     T* __restrict__ y, const T* __restrict__ a,
-                                                     T* __restrict__ c, T* __restrict__ c2,
-                                                     SolverState* st, double* part) {
-  __shared__ T xs[4][BLEN];
-  __shared__ T ys[4][BLEN];
-  if (MODE != SPMV_PLAIN && MODE != SPMV_RESID && st->status != ST_RUNNING) return;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ix = lane & 7, iy = lane >> 3;
-  const int b = xcd_remap(blockIdx.x, gridDim.x) * 4 + w;
-  const bool valid = b < g.nb;
-  const size_t off = (size_t)b * BLEN;
-  T col[BS];
-  if (valid) {
-#pragma unroll
-    for (int iz = 0; iz < BS; ++iz) {
-      col[iz] = x[off + iz * 64 + lane];
-      xs[w][iz * 64 + lane] = col[iz];
-    }
-  }
-  __syncthreads();
-  double msum = 0.0;
-  if (valid) {
-    const int* nb6 = g.nbr + b * 6;
-    const int nxm = nb6[0], nxp = nb6[1], nym = nb6[2], nyp = nb6[3], nzm = nb6[4], nzp = nb6[5];
-    const T h = g.h[b];
-    const T zlo = nzm >= 0 ? x[(size_t)nzm * BLEN + 7 * 64 + lane] : col[0];
-    const T zhi = nzp >= 0 ? x[(size_t)nzp * BLEN + lane] : col[BS - 1];
-#pragma unroll
-    for (int iz = 0; iz < BS; ++iz) {
-      const T c0 = col[iz];
-      const int li = iz * 64 + lane;
-      T xm, xp, ym, yp;
-      if (ix > 0) xm = xs[w][li - 1];
-      else xm = nxm >= 0 ? x[(size_t)nxm * BLEN + iz * 64 + iy * 8 + 7] : c0;
-      if (ix < 7) xp = xs[w][li + 1];
-      else xp = nxp >= 0 ? x[(size_t)nxp * BLEN + iz * 64 + iy * 8] : c0;
-      if (iy > 0) ym = xs[w][li - 8];
-      else ym = nym >= 0 ? x[(size_t)nym * BLEN + iz * 64 + 56 + ix] : c0;
-      if (iy < 7) yp = xs[w][li + 8];
-      else yp = nyp >= 0 ? x[(size_t)nyp * BLEN + iz * 64 + ix] : c0;
-      const T zm = iz > 0 ? col[iz - 1] : zlo;
-      const T zp = iz < BS - 1 ? col[iz + 1] : zhi;
-      ys[w][li] = h * ((((xm + xp) + (ym + yp)) + (zm + zp)) - (T)6 * c0);
-      msum += (double)c0;
-    }
-  }
-  __syncthreads();
-  if (valid) {
-    const int r0 = g.irow_ptr[b], r1 = g.irow_ptr[b + 1];
-    for (int r = r0 + lane; r < r1; r += 64) {
-      T acc = 0;
-      for (int e = g.irow_ent[r]; e < g.irow_ent[r + 1]; ++e) acc += g.ent_val[e] * x[g.ent_col[e]];
-      ys[w][g.irow_cell[r]] += acc;
-    }
-  }
-  __syncthreads();
-  const long long row = g.bmean_row;  // origin cell for BM 1 and 3, else -1
-  const bool origin = valid && row == (long long)off;
-  T yv[BS];
-  if (valid) {
-#pragma unroll
-    for (int iz = 0; iz < BS; ++iz) yv[iz] = ys[w][iz * 64 + lane];
-    if (BM == 3 && origin && lane == 0) yv[0] = g.h[b] * col[0];  // row h*I (.cpp:311-318)
-  }
-  // epilogue: stores and partial sums (the bMean row is left to the last WG)
-  double v[3] = {0.0, 0.0, 0.0};
-  if (valid) {
-    const double h = (double)g.h[b];
-    v[0] = h * h * h * msum;
-#pragma unroll
-    for (int iz = 0; iz < BS; ++iz) {
-      const size_t i = off + iz * 64 + lane;
-      const bool skip = (BM == 1) && origin && lane == 0 && iz == 0;
-      const T yy = yv[iz];
-      if (MODE == SPMV_PLAIN) {
-        y[i] = yy;
-      } else if (MODE == SPMV_INIT) {
-        y[i] = yy;
-        const T rr = a[i] - yy;
-        c[i] = rr;
-        c2[i] = rr;
-        if (!skip) {
-          v[1] += (double)yy * (double)yy;
-          v[2] += (double)rr * (double)rr;
-        }
-      } else if (MODE == SPMV_DOT1) {
-        y[i] = yy;
-        if (!skip) v[1] += (double)a[i] * (double)yy;
-      } else if (MODE == SPMV_DOT2) {
-        y[i] = yy;
-        if (!skip) {
-          v[1] += (double)a[i] * (double)yy;
-          v[2] += (double)yy * (double)yy;
-        }
-      } else if (MODE == SPMV_RESID) {
-        const T rr = a[i] - yy;
-        c[i] = rr;
-        if (!skip) v[1] += (double)rr * (double)rr;
-      }
-    }
-  }
-  double tot[3];
-  if (!grid_reduce_last<3>(v, part, &st->counters[1], tot)) return;
-  if (threadIdx.x != 0) return;
-  // bMean = 1: y[row] = sum_b h_b^3 sum x  (BiCGSTAB.cu:406-407)
-  if (BM == 1 && row >= 0) {
-    const T yr = (T)tot[0];
-    if (MODE == SPMV_PLAIN) {
-      y[row] = yr;
-    } else if (MODE == SPMV_INIT) {
-      y[row] = yr;
-      const T rr = a[row] - yr;
-      c[row] = rr;
-      c2[row] = rr;
-      tot[1] += (double)yr * (double)yr;
-      tot[2] += (double)rr * (double)rr;
-    } else if (MODE == SPMV_DOT1) {
-      y[row] = yr;
-      tot[1] += (double)a[row] * (double)yr;
-    } else if (MODE == SPMV_DOT2) {
-      y[row] = yr;
-      tot[1] += (double)a[row] * (double)yr;
-      tot[2] += (double)yr * (double)yr;
-    } else if (MODE == SPMV_RESID) {
-      const T rr = a[row] - yr;
-      c[row] = rr;
-      tot[1] += (double)rr * (double)rr;
-    }
-  }
-  st->mean_sum = tot[0];
-  if (MODE == SPMV_INIT) {
-    // initial residual (BiCGSTAB.cu:435-471)
-    st->nu2 = tot[1];
-    st->err = sqrt(tot[2]);
-    st->err_init = st->err;
-    st->err_opt = st->err;
-    st->rhrh = tot[2];
-    begin_iteration(st, tot[2], tot[2]);
-  } else if (MODE == SPMV_DOT1) {
-    st->dot_a = tot[1];
-    st->alpha = st->rho_curr / (tot[1] + st->eps);  // set_alpha (:276-279)
-    st->p_zero = 0;
-    st->restart_pending = 0;
-  } else if (MODE == SPMV_DOT2) {
-    st->dot_ts = tot[1];
-    st->dot_tt = tot[2];
-    st->omega = tot[1] / (tot[2] + st->eps);  // set_omega (:281-284)
-  } else if (MODE == SPMV_RESID) {
-    st->true_rr = tot[1];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// K5: x' = x + alpha z1 + omega z2 (into a free rotating buffer), r = s - omega t,
-// then rho' = (rhat, r), |r|, x_opt, convergence and the next iteration's
-// scalars (BiCGSTAB.cu:543-607).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int third_buffer(int a, int b) {
-  for (int k = 0; k < 3; ++k)
-    if (k != a && k != b) return k;
-  return 0;
-}
-
-template <typename T>
-__global__ __launch_bounds__(WG_THREADS) void k_update(long long n, KrylovVectors<T> v,
-                                                       SolverState* st, double* part) {
-  if (st->status != ST_RUNNING) return;
-  const int xc = st->xcur, xo = st->xopt, xn = third_buffer(xc, xo);
-  const T alpha = (T)st->alpha, omega = (T)st->omega;
-  const T* __restrict__ xin = v.x[xc];
-  T* __restrict__ xout = v.x[xn];
-  double s[2] = {0.0, 0.0};
-  const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
-  for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
-    const long long i = base + e;
-    if (i < n) {
-      xout[i] = fma(omega, v.z2[i], fma(alpha, v.z1[i], xin[i]));
-      const T r = fma(-omega, v.t[i], v.r[i]);
-      v.r[i] = r;
-      s[0] += (double)v.rhat[i] * (double)r;
-      s[1] += (double)r * (double)r;
-    }
-  }
-  double tot[2];
-  if (!grid_reduce_last<2>(s, part, &st->counters[2], tot) || threadIdx.x != 0) return;
-  const double err = sqrt(tot[1]);
-  st->err = err;
-  st->iters_done = st->iter;
-  st->xcur = xn;
-  if (!(err == err) || err > 1.7e308) {
-    st->status = ST_NONFINITE;
-    return;
-  }
-  if (err < st->err_opt) {
-    st->err_opt = err;
-    st->xopt = xn;
-    if (err <= st->tol_abs || err / st->err_init <= st->tol_rel) {
-      st->status = ST_CONVERGED;
-      return;
-    }
-  }
-  st->rho_prev = st->rho_curr;  // set_rho (:607)
-  begin_iteration(st, tot[0], tot[1]);
-}
 
 // ---------------------------------------------------------------------------
 // K1 / K3: fused vector update + block-Jacobi GEMM on the FP64 matrix cores.
@@ -394,18 +114,6 @@ __global__ __launch_bounds__(WG_THREADS) void k_precond_f64(int nb, const double
       for (int t = 0; t < 8; ++t) out[(size_t)b * BLEN + (8 * w + t) * 16 + lr] = acc[t][q] * s;
     }
   }
-}
-
-// Copy the best iterate out: x_user = X[xopt] (PoissonSolverExp.cpp:452-472
-// scatter; BiCGSTAB.cu:625 copies x_opt).
-template <typename T>
-__global__ __launch_bounds__(WG_THREADS) void k_copy_opt(long long n, KrylovVectors<T> v,
-                                                         const SolverState* st,
-                                                         double* __restrict__ out) {
-  const T* __restrict__ src = v.x[st->xopt];
-  for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n;
-       i += (long long)gridDim.x * WG_THREADS)
-    out[i] = (double)src[i];
 }
 
 // ---------------------------------------------------------------------------

@@ -1,30 +1,531 @@
-// Mixed-precision solve (placeholder until the FP32/FP16 kernels land).
+// Mixed-precision form of the solve: an FP64 iterative-refinement loop around
+// an FP32 classical BiCGSTAB whose block-Jacobi preconditioner runs on the
+// FP16 (or BF16) matrix cores with FP32 accumulation.
+//
+//   outer (FP64, on the device):  r = b - A x        (true residual, FP64 dot)
+//                                 stop if |r| <= tol
+//                                 solve A d = r / |r| in FP32 to inner_tol
+//                                 x += |r| d
+//   inner: the five fused kernels of the FP64 solver instantiated for float,
+//          with K1/K3 replaced by the half-precision GEMM below.
+//
+// The half-precision GEMM. P_inv entries span [3.4e-6, 0.237] in magnitude;
+// scaled by 2^14 they sit in FP16's normal range (max 3.9e3, min 5.6e-2).
+// Each block's input vector is scaled by a power of two so its largest
+// entry lies in [0.5, 1) before the cast, and both scales are undone in the
+// FP32 epilogue together with h_b^-1. BF16 needs no P scaling (FP32 exponent
+// range) but uses the same per-block input scaling.
+// MFMA: v_mfma_f32_16x16x32_{f16,bf16}; A = 16 blocks x 32 cells of the
+// input (LDS), B = 32 x 16 of P_inv (fragment-packed in global memory, 1 KiB
+// per fragment), D = 16 blocks x 16 output cells (FP32).
+//
+// Split precision (default). A single FP16 pass perturbs every application
+// of M^-1 by ~5e-4 relative (BF16: ~4e-3) in a way that is not linear in the
+// input, and over hundreds of iterations BiCGSTAB loses its
+// bi-orthogonality and stagnates. Both operands are therefore carried as
+// hi + lo pairs (v = hi + 2^-s lo, s = 11 for FP16, 8 for BF16) and the GEMM
+// runs three MFMA passes, hi*hi + 2^-s (hi*lo + lo*hi), each accumulating in
+// FP32: the preconditioner is then linear to FP32 precision while every
+// product still runs on the FP16/BF16 matrix cores.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "device_common.h"
+#include "kernels.h"
+#include "krylov.h"
 #include "mixed.h"
 
 namespace cup3d {
-int mixed_init_pinv(MixedWorkspace&, const std::vector<double>&, std::string&) { return 0; }
-int mixed_set_topology(MixedWorkspace& M, int nb, long long n, std::string&) {
-  M.nb = nb;
-  M.n = n;
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf168_t __attribute__((ext_vector_type(8)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+template <typename H>
+struct HalfVec;
+template <>
+struct HalfVec<_Float16> {
+  typedef half8_t type;
+};
+template <>
+struct HalfVec<__bf16> {
+  typedef bf168_t type;
+};
+
+__device__ __forceinline__ float4_t mfma16(half8_t a, half8_t b, float4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ float4_t mfma16(bf168_t a, bf168_t b, float4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Accumulator layout of v_mfma_f32_16x16x32_*: register q of lane l holds
+// D[4*(l/16) + q][l%16].
+__device__ __forceinline__ int acc_row(int g, int q) { return 4 * g + q; }
+
+constexpr int HLD = BLEN + 8;  // LDS row stride in halfs (1040 bytes)
+constexpr float P16_SCALE = 16384.0f;
+
+template <typename H>
+struct Split;
+template <>
+struct Split<_Float16> {
+  static constexpr float lo = 2048.0f;  // 2^11
+};
+template <>
+struct Split<__bf16> {
+  static constexpr float lo = 256.0f;  // 2^8
+};
+
+template <int MODE, typename H, int NBW, bool SPLIT>
+__global__ __launch_bounds__(WG_THREADS) void k_precond_h(int nb, const H* __restrict__ Ppk,
+                                                          const H* __restrict__ Ppk_lo,
+                                                          float p_unscale,
+                                                          const float* __restrict__ invh,
+                                                          KrylovVectors<float> v,
+                                                          const float* __restrict__ pin,
+                                                          float* __restrict__ pout,
+                                                          SolverState* st) {
+  typedef typename HalfVec<H>::type HV;
+  constexpr int NS = SPLIT ? 2 : 1;
+  __shared__ __attribute__((aligned(16))) H As[NS][NBW * HLD];
+  __shared__ float sc[NBW];
+  if (MODE != PREC_PLAIN && st->status != ST_RUNNING) return;
+  const int j0 = xcd_remap(blockIdx.x, gridDim.x) * NBW;
+  const float beta = MODE == PREC_P ? (float)st->beta : 0.f;
+  const float omega = MODE == PREC_P ? (float)st->omega : 0.f;
+  const float alpha = MODE == PREC_S ? (float)st->alpha : 0.f;
+  const int pzero = MODE == PREC_P ? st->p_zero : 0;
+  const int restart = MODE == PREC_P ? st->restart_pending : 0;
+  float* __restrict__ out = MODE == PREC_PLAIN ? pout : (MODE == PREC_S ? v.z2 : v.z1);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  // phase 1: vector update, per-block scaling, cast into LDS (one wave per block)
+  for (int j = w; j < NBW; j += 4) {
+    const int b = j0 + j;
+    float vals[BS];
+    float amax = 0.f;
+#pragma unroll
+    for (int e = 0; e < BS; ++e) {
+      float val = 0.f;
+      if (b < nb) {
+        const size_t i = (size_t)b * BLEN + e * 64 + l;
+        if (MODE == PREC_P) {
+          const float r = v.r[i];
+          val = pzero ? r : fmaf(beta, fmaf(-omega, v.nu[i], v.p[i]), r);
+          v.p[i] = val;
+          if (restart) v.rhat[i] = r;
+        } else if (MODE == PREC_S) {
+          val = fmaf(-alpha, v.nu[i], v.r[i]);
+          v.r[i] = val;
+        } else {
+          val = pin[i];
+        }
+      }
+      vals[e] = val;
+      amax = fmaxf(amax, fabsf(val));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
+    int ex = 0;
+    if (amax > 0.f) frexpf(amax, &ex);
+    const float s = amax > 0.f ? ldexpf(1.0f, -ex) : 1.0f;
+#pragma unroll
+    for (int e = 0; e < BS; ++e) {
+      const float x = vals[e] * s;
+      const H hi = (H)x;
+      As[0][j * HLD + e * 64 + l] = hi;
+      if (SPLIT) As[NS - 1][j * HLD + e * 64 + l] = (H)((x - (float)hi) * Split<H>::lo);
+    }
+    if (l == 0) sc[j] = b < nb ? invh[b] * p_unscale / s : 0.f;
+  }
+  __syncthreads();
+  // phase 2: wave w owns output cells [128 w, 128 w + 128) for all NBW blocks
+  constexpr int JT = NBW / 16;
+  const int lr = l & 15, g = l >> 4;
+  float4_t acc[8][JT], cor[8][JT];
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt) {
+      acc[t][jt] = (float4_t){0.f, 0.f, 0.f, 0.f};
+      cor[t][jt] = (float4_t){0.f, 0.f, 0.f, 0.f};
+    }
+  const size_t foff = (size_t)(8 * w) * 16 * 64 + l;
+  const HV* __restrict__ pb = reinterpret_cast<const HV*>(Ppk) + foff;
+  const HV* __restrict__ pl = reinterpret_cast<const HV*>(SPLIT ? Ppk_lo : Ppk) + foff;
+  for (int kt = 0; kt < 16; ++kt) {
+    HV a[JT], al[JT], bq[8], bl[8];
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt) {
+      const int o = (jt * 16 + lr) * HLD + kt * 32 + 8 * g;
+      a[jt] = *reinterpret_cast<const HV*>(&As[0][o]);
+      if (SPLIT) al[jt] = *reinterpret_cast<const HV*>(&As[NS - 1][o]);
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      bq[t] = pb[((size_t)t * 16 + kt) * 64];
+      if (SPLIT) bl[t] = pl[((size_t)t * 16 + kt) * 64];
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int jt = 0; jt < JT; ++jt) {
+        acc[t][jt] = mfma16(a[jt], bq[t], acc[t][jt]);
+        if (SPLIT) {
+          cor[t][jt] = mfma16(a[jt], bl[t], cor[t][jt]);
+          cor[t][jt] = mfma16(al[jt], bq[t], cor[t][jt]);
+        }
+      }
+  }
+  constexpr float inv_lo = 1.0f / Split<H>::lo;
+#pragma unroll
+  for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int jr = jt * 16 + acc_row(g, q);
+      const int b = j0 + jr;
+      if (b < nb) {
+        const float s = sc[jr];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const float z = SPLIT ? fmaf(cor[t][jt][q], inv_lo, acc[t][jt][q]) : acc[t][jt][q];
+          out[(size_t)b * BLEN + (8 * w + t) * 16 + lr] = z * s;
+        }
+      }
+    }
+}
+
+// Inner-solve start: r = rhat = (float)(r64 * scale), x = 0, ||r||^2, and the
+// iteration-0 scalars (BiCGSTAB.cu:435-471 with x0 = 0).
+__global__ __launch_bounds__(WG_THREADS) void k_inner_init(long long n, const double* __restrict__ r64,
+                                                           double scale, KrylovVectors<float> v,
+                                                           SolverState* st, double* part) {
+  double s[1] = {0.0};
+  const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
+  float* __restrict__ x0 = v.x[0];
+  for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
+    const long long i = base + e;
+    if (i < n) {
+      const float r = (float)(r64[i] * scale);
+      v.r[i] = r;
+      v.rhat[i] = r;
+      x0[i] = 0.f;
+      s[0] += (double)r * (double)r;
+    }
+  }
+  double tot[1];
+  if (!grid_reduce_last<1>(s, part, &st->counters[3], tot) || threadIdx.x != 0) return;
+  st->nu2 = 0.0;
+  st->err = sqrt(tot[0]);
+  st->err_init = st->err;
+  st->err_opt = st->err;
+  st->rhrh = tot[0];
+  begin_iteration(st, tot[0], tot[0]);
+}
+
+// x64 += scale * (double) x32[xopt]   (refinement update, FP64)
+__global__ __launch_bounds__(WG_THREADS) void k_refine(long long n, double* __restrict__ x64,
+                                                       KrylovVectors<float> v,
+                                                       const SolverState* st, double scale) {
+  const float* __restrict__ d = v.x[st->xopt];
+  for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n;
+       i += (long long)gridDim.x * WG_THREADS)
+    x64[i] = fma(scale, (double)d[i], x64[i]);
+}
+
+__global__ void k_copy64(long long n, const double* __restrict__ a, double* __restrict__ b) {
+  for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n;
+       i += (long long)gridDim.x * WG_THREADS)
+    b[i] = a[i];
+}
+
+__global__ void k_cast64to32(long long n, const double* __restrict__ a, float* __restrict__ b) {
+  for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n;
+       i += (long long)gridDim.x * WG_THREADS)
+    b[i] = (float)a[i];
+}
+__global__ void k_cast32to64(long long n, const float* __restrict__ a, double* __restrict__ b) {
+  for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n;
+       i += (long long)gridDim.x * WG_THREADS)
+    b[i] = (double)a[i];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace {
+
+int nbw_for(int nb) { return nb >= 512 * 32 ? 32 : 16; }
+
+template <typename H, int NBW, bool SPLIT>
+void launch_precond_hh(hipStream_t s, int mode, int nb, const H* P, const H* Plo, float unscale,
+                       const float* invh, const KrylovVectors<float>& v, const float* pin,
+                       float* pout, SolverState* st) {
+  const int nwg = (nb + NBW - 1) / NBW;
+  if (mode == PREC_P)
+    k_precond_h<PREC_P, H, NBW, SPLIT><<<nwg, WG_THREADS, 0, s>>>(nb, P, Plo, unscale, invh, v, pin, pout, st);
+  else if (mode == PREC_S)
+    k_precond_h<PREC_S, H, NBW, SPLIT><<<nwg, WG_THREADS, 0, s>>>(nb, P, Plo, unscale, invh, v, pin, pout, st);
+  else
+    k_precond_h<PREC_PLAIN, H, NBW, SPLIT><<<nwg, WG_THREADS, 0, s>>>(nb, P, Plo, unscale, invh, v, pin, pout, st);
+}
+
+template <typename H>
+void launch_precond_h(hipStream_t s, int mode, int nb, const void* Ppk, const void* Plo,
+                      float unscale, bool split, const float* invh,
+                      const KrylovVectors<float>& v, const float* pin, float* pout,
+                      SolverState* st) {
+  const H* P = static_cast<const H*>(Ppk);
+  const H* L = static_cast<const H*>(Plo);
+  const bool wide = nbw_for(nb) == 32;
+  if (split) {
+    if (wide) launch_precond_hh<H, 32, true>(s, mode, nb, P, L, unscale, invh, v, pin, pout, st);
+    else launch_precond_hh<H, 16, true>(s, mode, nb, P, L, unscale, invh, v, pin, pout, st);
+  } else {
+    if (wide) launch_precond_hh<H, 32, false>(s, mode, nb, P, L, unscale, invh, v, pin, pout, st);
+    else launch_precond_hh<H, 16, false>(s, mode, nb, P, L, unscale, invh, v, pin, pout, st);
+  }
+}
+
+void launch_precond_mixed(const MixedWorkspace& M, hipStream_t s, int precision, int mode,
+                          const DevGrid<float>& g, const float* pin, float* pout, SolverState* st) {
+  if (precision == CUP3D_PREC_MIXED_FP16)
+    launch_precond_h<_Float16>(s, mode, g.nb, M.d_P16, M.d_P16lo, 1.0f / P16_SCALE, M.split,
+                               g.invh, M.v32, pin, pout, st);
+  else
+    launch_precond_h<__bf16>(s, mode, g.nb, M.d_Pbf, M.d_Pbflo, 1.0f, M.split, g.invh, M.v32,
+                             pin, pout, st);
+}
+
+template <int MODE, int BM>
+void spmv32_bm(hipStream_t s, const DevGrid<float>& g, const float* x, float* y, const float* a,
+               SolverState* st, double* part) {
+  k_spmv<float, MODE, BM><<<(g.nb + 3) / 4, WG_THREADS, 0, s>>>(g, x, y, a, nullptr, nullptr, st, part);
+}
+template <int MODE>
+void spmv32(hipStream_t s, const DevGrid<float>& g, const float* x, float* y, const float* a,
+            SolverState* st, double* part) {
+  if (g.bmean == 1) spmv32_bm<MODE, 1>(s, g, x, y, a, st, part);
+  else if (g.bmean == 3) spmv32_bm<MODE, 3>(s, g, x, y, a, st, part);
+  else spmv32_bm<MODE, 0>(s, g, x, y, a, st, part);
+}
+
+int grid_1d(long long n) { return (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 8192); }
+int grid_vec(long long n) { return (int)((n + 4 * BLEN - 1) / (4 * BLEN)); }
+
+SolverState inner_state(const cup3d_poisson_config& c, double tol_rel) {
+  SolverState s;
+  std::memset(&s, 0, sizeof(s));
+  s.alpha = s.beta = s.omega = 1.0;
+  s.eps = 1e-21;
+  s.rho_prev = s.rho_curr = 1.0;
+  s.err_init = s.err_opt = s.err = 1e50;
+  s.tol_abs = 0.0;
+  s.tol_rel = tol_rel;
+  s.max_iter = c.max_iter;
+  s.max_restarts = c.max_restarts;
+  s.status = ST_RUNNING;
+  s.p_zero = 1;
+  return s;
+}
+
+#define MCHECK(expr)                                              \
+  do {                                                            \
+    hipError_t e_ = (expr);                                       \
+    if (e_ != hipSuccess) {                                       \
+      err = std::string(#expr) + ": " + hipGetErrorString(e_);    \
+      return -1;                                                  \
+    }                                                             \
+  } while (0)
+
+}  // namespace
+
+int mixed_init_pinv(MixedWorkspace& M, const std::vector<double>& P, std::string& err) {
+  // fragment (it, kt): lane l, element e = P[32 kt + 8 (l/16) + e][16 it + l%16]
+  std::vector<_Float16> p16((size_t)BLEN * BLEN), p16lo((size_t)BLEN * BLEN);
+  std::vector<__bf16> pbf((size_t)BLEN * BLEN), pbflo((size_t)BLEN * BLEN);
+  for (int it = 0; it < 32; ++it)
+    for (int kt = 0; kt < 16; ++kt)
+      for (int l = 0; l < 64; ++l)
+        for (int e = 0; e < 8; ++e) {
+          const double v = P[(size_t)(32 * kt + 8 * (l / 16) + e) * BLEN + 16 * it + l % 16];
+          const size_t o = (((size_t)it * 16 + kt) * 64 + l) * 8 + e;
+          const float vs = (float)(v * P16_SCALE);
+          p16[o] = (_Float16)vs;
+          p16lo[o] = (_Float16)((vs - (float)p16[o]) * Split<_Float16>::lo);
+          pbf[o] = (__bf16)(float)v;
+          pbflo[o] = (__bf16)(((float)v - (float)pbf[o]) * Split<__bf16>::lo);
+        }
+  M.p16_scale = P16_SCALE;
+  MCHECK(hipMalloc(&M.d_P16, p16.size() * sizeof(_Float16)));
+  MCHECK(hipMalloc(&M.d_Pbf, pbf.size() * sizeof(__bf16)));
+  MCHECK(hipMemcpy(M.d_P16, p16.data(), p16.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+  MCHECK(hipMemcpy(M.d_Pbf, pbf.data(), pbf.size() * sizeof(__bf16), hipMemcpyHostToDevice));
+  MCHECK(hipMalloc(&M.d_P16lo, p16lo.size() * sizeof(_Float16)));
+  MCHECK(hipMalloc(&M.d_Pbflo, pbflo.size() * sizeof(__bf16)));
+  MCHECK(hipMemcpy(M.d_P16lo, p16lo.data(), p16lo.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+  MCHECK(hipMemcpy(M.d_Pbflo, pbflo.data(), pbflo.size() * sizeof(__bf16), hipMemcpyHostToDevice));
   return 0;
 }
-int mixed_solve(MixedWorkspace&, hipStream_t, const cup3d_poisson_config&, const DevGrid<double>&,
-                const DevGrid<float>&, const KrylovVectors<double>&, SolverState*, SolverState*,
-                double*, int, const double*, double*, const double*, cup3d_solve_report*,
-                std::string& err) {
-  err = "mixed precision not built yet";
-  return -1;
+
+int mixed_set_topology(MixedWorkspace& M, int nb, long long n, std::string& err) {
+  if (M.d_vec32) (void)hipFree(M.d_vec32);
+  if (M.d_res64) (void)hipFree(M.d_res64);
+  M.d_vec32 = M.d_res64 = nullptr;
+  M.nb = nb;
+  M.n = n;
+  MCHECK(hipMalloc(&M.d_vec32, (size_t)11 * n * sizeof(float)));
+  MCHECK(hipMalloc(&M.d_res64, (size_t)2 * n * sizeof(double)));
+  float* base = static_cast<float*>(M.d_vec32);
+  KrylovVectors<float>& v = M.v32;
+  for (int k = 0; k < 3; ++k) v.x[k] = base + k * n;
+  v.r = base + 3 * n;
+  v.rhat = base + 4 * n;
+  v.p = base + 5 * n;
+  v.nu = base + 6 * n;
+  v.t = base + 7 * n;
+  v.z1 = base + 8 * n;
+  v.z2 = base + 9 * n;
+  v.b = base + 10 * n;
+  return 0;
 }
-int mixed_apply_precond(MixedWorkspace&, hipStream_t, int, const DevGrid<float>&, const double*,
-                        double*, std::string& err) {
-  err = "mixed precision not built yet";
-  return -1;
+
+int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cfg,
+                const DevGrid<double>& g64, const DevGrid<float>& g32,
+                const KrylovVectors<double>& v64, SolverState* st, SolverState* h_st,
+                double* part, int origin_block, const double* d_rhs, double* d_x,
+                const double* d_x0, cup3d_solve_report* rep, std::string& err) {
+  const long long n = M.n;
+  const KrylovVectors<float>& v = M.v32;
+  double* x64 = v64.x[0];
+  double* b64 = v64.b;
+  double* r64 = v64.r;
+  // getVec + ||b|| (FP64)
+  SolverState init = inner_state(cfg, 0.0);
+  init.tol_abs = cfg.tol_abs;
+  *h_st = init;
+  MCHECK(hipMemcpyAsync(st, h_st, sizeof(SolverState), hipMemcpyHostToDevice, s));
+  const long long zero_row =
+      (cfg.bmean == 1 || cfg.bmean == 3) && origin_block >= 0 ? (long long)origin_block * BLEN : -1;
+  launch_getvec64(s, n, zero_row, d_rhs, d_x0, b64, x64, st, part, cfg.tol_rhs);
+  MCHECK(hipMemcpyAsync(h_st, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+  MCHECK(hipStreamSynchronize(s));
+  const double bnorm2 = h_st->bnorm2;
+  const double tol_abs = h_st->tol_abs;
+  double err_init = -1, rnorm = 0;
+  int total_iters = 0, total_restarts = 0, steps = 0, status = ST_MAXITER;
+  const int poll = cfg.poll_every > 0 ? cfg.poll_every : 8;
+  for (int k = 0; k <= cfg.max_refine; ++k) {
+    // true residual in FP64
+    launch_spmv64(s, SPMV_RESID, g64, x64, nullptr, b64, r64, nullptr, st, part);
+    MCHECK(hipMemcpyAsync(h_st, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+    MCHECK(hipStreamSynchronize(s));
+    rnorm = std::sqrt(h_st->true_rr);
+    if (k == 0) err_init = rnorm;
+    if (!std::isfinite(rnorm)) {
+      status = ST_NONFINITE;
+      break;
+    }
+    const double tol = std::max(tol_abs, cfg.tol_rel * err_init);
+    if (rnorm <= tol) {
+      status = ST_CONVERGED;
+      break;
+    }
+    if (k == cfg.max_refine) break;
+    // inner FP32 solve of A d = r / |r|
+    const double inner_rel = std::max(cfg.inner_tol, 0.5 * tol / rnorm);
+    *h_st = inner_state(cfg, inner_rel);
+    MCHECK(hipMemcpyAsync(st, h_st, sizeof(SolverState), hipMemcpyHostToDevice, s));
+    k_inner_init<<<grid_vec(n), WG_THREADS, 0, s>>>(n, r64, 1.0 / rnorm, v, st, part);
+    int guard = 0;
+    while (true) {
+      for (int it = 0; it < poll; ++it) {
+        launch_precond_mixed(M, s, cfg.precision, PREC_P, g32, nullptr, nullptr, st);
+        spmv32<SPMV_DOT1>(s, g32, v.z1, v.nu, v.rhat, st, part);
+        launch_precond_mixed(M, s, cfg.precision, PREC_S, g32, nullptr, nullptr, st);
+        spmv32<SPMV_DOT2>(s, g32, v.z2, v.t, v.r, st, part);
+        k_update<float><<<grid_vec(n), WG_THREADS, 0, s>>>(n, v, st, part);
+      }
+      MCHECK(hipGetLastError());
+      MCHECK(hipMemcpyAsync(h_st, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+      MCHECK(hipStreamSynchronize(s));
+      if (h_st->status != ST_RUNNING) break;
+      if (++guard > cfg.max_iter / poll + 4) {
+        err = "inner solve did not stop";
+        return -1;
+      }
+    }
+    total_iters += h_st->iters_done;
+    total_restarts += h_st->restarts;
+    ++steps;
+    k_refine<<<grid_1d(n), WG_THREADS, 0, s>>>(n, x64, v, st, rnorm);
+  }
+  k_copy64<<<grid_1d(n), WG_THREADS, 0, s>>>(n, x64, d_x);
+  MCHECK(hipGetLastError());
+  MCHECK(hipStreamSynchronize(s));
+  if (rep) {
+    rep->status = status;
+    rep->iterations = total_iters;
+    rep->restarts = total_restarts;
+    rep->refinements = steps;
+    rep->err_init = err_init;
+    rep->err_final = rnorm;
+    rep->rhs_norm = std::sqrt(bnorm2);
+    rep->true_rel_resid = bnorm2 > 0 ? rnorm / std::sqrt(bnorm2) : rnorm;
+    rep->guess_resid = err_init;
+  }
+  return status;
 }
-int mixed_time_kernel(MixedWorkspace&, hipStream_t, int, int, const DevGrid<float>&, SolverState*,
-                      double*, hipEvent_t, hipEvent_t, double*, double*, double*,
+
+int mixed_apply_precond(MixedWorkspace& M, hipStream_t s, int precision, const DevGrid<float>& g32,
+                        const double* d_p, double* d_z, std::string& err) {
+  float* in = reinterpret_cast<float*>(M.d_res64);
+  float* out = in + M.n;
+  k_cast64to32<<<grid_1d(M.n), WG_THREADS, 0, s>>>(M.n, d_p, in);
+  launch_precond_mixed(M, s, precision, PREC_PLAIN, g32, in, out, nullptr);
+  k_cast32to64<<<grid_1d(M.n), WG_THREADS, 0, s>>>(M.n, out, d_z);
+  MCHECK(hipGetLastError());
+  return 0;
+}
+
+int mixed_time_kernel(MixedWorkspace& M, hipStream_t s, int which, int reps,
+                      const DevGrid<float>& g32, SolverState* st, double* part, hipEvent_t e0,
+                      hipEvent_t e1, double* avg_ms, double* bytes, double* flops,
                       std::string& err) {
-  err = "mixed precision not built yet";
-  return -1;
+  const KrylovVectors<float>& v = M.v32;
+  const int precision = M.timing_precision;
+  auto launch = [&]() {
+    if (which == 0) spmv32<SPMV_PLAIN>(s, g32, v.z1, v.nu, nullptr, st, part);
+    else if (which == 1) launch_precond_mixed(M, s, precision, PREC_PLAIN, g32, v.p, v.z1, st);
+    else k_update<float><<<grid_vec(M.n), WG_THREADS, 0, s>>>(M.n, v, st, part);
+  };
+  launch();
+  MCHECK(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) launch();
+  MCHECK(hipEventRecord(e1, s));
+  MCHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  MCHECK(hipEventElapsedTime(&ms, e0, e1));
+  *avg_ms = ms / reps;
+  const double N = (double)M.n;
+  if (which == 0) { *bytes = 8.0 * N; *flops = 8.0 * N; }
+  else if (which == 1) { *bytes = 8.0 * N; *flops = 1024.0 * N; }
+  else { *bytes = 32.0 * N; *flops = 8.0 * N; }
+  return 0;
 }
-void mixed_free(MixedWorkspace&) {}
+
+void mixed_free(MixedWorkspace& M) {
+  if (M.d_P16) (void)hipFree(M.d_P16);
+  if (M.d_Pbf) (void)hipFree(M.d_Pbf);
+  if (M.d_P16lo) (void)hipFree(M.d_P16lo);
+  if (M.d_Pbflo) (void)hipFree(M.d_Pbflo);
+  M.d_P16lo = M.d_Pbflo = nullptr;
+  if (M.d_vec32) (void)hipFree(M.d_vec32);
+  if (M.d_res64) (void)hipFree(M.d_res64);
+  M.d_P16 = M.d_Pbf = M.d_vec32 = M.d_res64 = nullptr;
+}
+
 }  // namespace cup3d

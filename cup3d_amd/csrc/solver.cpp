@@ -193,6 +193,7 @@ int cup3d_poisson_create(const cup3d_poisson_config* cfg, cup3d_poisson_t* out) 
   H->cfg = c;
   if (H->cfg.inner_tol <= 0) H->cfg.inner_tol = 1e-4;
   if (H->cfg.max_refine <= 0) H->cfg.max_refine = 10;
+  H->mixed.split = c.gemm_passes != 1;
   H->device = c.device;
   if (hipSetDevice(c.device) != hipSuccess || hipStreamCreate(&H->stream) != hipSuccess ||
       hipEventCreate(&H->ev0) != hipSuccess || hipEventCreate(&H->ev1) != hipSuccess ||
@@ -296,7 +297,8 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
   if (H->d_part.alloc(H->part_len * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(partials) failed");
   HIPCHECK(hipMemset(H->d_state.p, 0, sizeof(SolverState)));
   if (mixed_set_topology(H->mixed, H->nb, H->n, H->err) != 0) return -1;
-  if (cnn_set_topology(H->cnn, H->nb, H->d_nbr.as<int>(), H->err) != 0) return -1;
+  if (cnn_set_topology(H->cnn, H->nb, H->d_nbr.as<int>(), H->d_h.as<double>(), H->err) != 0)
+    return -1;
   HIPCHECK(hipDeviceSynchronize());
   return 0;
 }
@@ -352,8 +354,8 @@ int cup3d_poisson_solve_device(cup3d_poisson_t H, const double* d_rhs, double* d
   if (H->cfg.guess == CUP3D_GUESS_CNN) {
     // learned warm start with fallback to the given guess
     if (cnn_warm_start(H->cnn, H->stream, H->g64, H->v64, H->d_state.as<SolverState>(),
-                       H->d_part.as<double>(), d_rhs, d_x, H->cfg.bmean, H->topo.origin_block,
-                       &guess_used, &guess_resid, H->err) != 0)
+                       H->h_state, H->d_part.as<double>(), d_rhs, d_x, &guess_used, &guess_resid,
+                       H->err) != 0)
       return -1;
   }
   if (H->cfg.precision == CUP3D_PREC_FP64)
@@ -469,16 +471,44 @@ int cup3d_poisson_cnn_predict(cup3d_poisson_t H, const double* d_rhs, const doub
   if (!H) return -1;
   if (H->nb <= 0) return fail(H, "set_topology must be called first");
   HIPCHECK(hipSetDevice(H->device));
-  return cnn_predict(H->cnn, H->stream, H->g64, d_rhs, d_prev, d_out, H->err);
+  return cnn_predict(H->cnn, H->stream, d_rhs, d_prev, d_out, H->err);
 }
 
-int cup3d_poisson_cnn_train_step(cup3d_poisson_t H, const double* d_rhs, const double* d_prev,
-                                 const double* d_target, int nsamples, float lr, double* loss) {
+int cup3d_poisson_cnn_add_sample(cup3d_poisson_t H, const double* d_rhs, const double* d_prev,
+                                 const double* d_p) {
   if (!H) return -1;
-  if (H->nb <= 0) return fail(H, "set_topology must be called first");
   HIPCHECK(hipSetDevice(H->device));
-  return cnn_train_step(H->cnn, H->stream, H->g64, d_rhs, d_prev, d_target, nsamples, lr, loss,
-                        H->err);
+  return cnn_add_sample(H->cnn, H->stream, d_rhs, d_prev, d_p, H->err);
+}
+
+int cup3d_poisson_cnn_clear_samples(cup3d_poisson_t H) {
+  if (!H) return -1;
+  HIPCHECK(hipSetDevice(H->device));
+  cnn_clear_samples(H->cnn);
+  return 0;
+}
+
+int cup3d_poisson_cnn_num_samples(cup3d_poisson_t H) {
+  return H ? (int)H->cnn.samples.size() : -1;
+}
+
+int cup3d_poisson_cnn_train(cup3d_poisson_t H, int steps, int batch, float lr, uint64_t seed,
+                            double* loss) {
+  if (!H) return -1;
+  HIPCHECK(hipSetDevice(H->device));
+  return cnn_train(H->cnn, H->stream, steps, batch, lr, seed, loss, H->err);
+}
+
+int cup3d_poisson_cnn_eval(cup3d_poisson_t H, int k, double* loss) {
+  if (!H || !loss) return -1;
+  HIPCHECK(hipSetDevice(H->device));
+  return cnn_eval(H->cnn, H->stream, k, loss, H->err);
+}
+
+int cup3d_debug_cnn_gradient(cup3d_poisson_t H, int k, float* grad, double* loss) {
+  if (!H || !grad || !loss) return -1;
+  HIPCHECK(hipSetDevice(H->device));
+  return cnn_gradient(H->cnn, H->stream, k, grad, loss, H->err);
 }
 
 int cup3d_debug_time_kernel(cup3d_poisson_t H, int which, int reps, double* avg_ms,
@@ -502,6 +532,7 @@ int cup3d_debug_time_kernel(cup3d_poisson_t H, int which, int reps, double* avg_
   const double N = (double)H->n;
   double by = 0, fl = 0;
   if (H->cfg.precision != CUP3D_PREC_FP64) {
+    H->mixed.timing_precision = H->cfg.precision;
     if (mixed_time_kernel(H->mixed, s, which, reps, H->g32, st, part, H->ev0, H->ev1, avg_ms, &by,
                           &fl, H->err) != 0)
       return -1;

@@ -26,7 +26,7 @@ class Config(ctypes.Structure):
                 ("max_iter", ctypes.c_int), ("max_restarts", ctypes.c_int),
                 ("precision", ctypes.c_int), ("guess", ctypes.c_int), ("device", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("max_refine", ctypes.c_int),
-                ("poll_every", ctypes.c_int)]
+                ("poll_every", ctypes.c_int), ("gemm_passes", ctypes.c_int)]
 
 
 class BlockDesc(ctypes.Structure):
@@ -51,7 +51,9 @@ EXPORTS = [
     "cup3d_poisson_solve_device", "cup3d_poisson_apply_operator",
     "cup3d_poisson_apply_preconditioner", "cup3d_cnn_num_weights",
     "cup3d_poisson_cnn_set_weights", "cup3d_poisson_cnn_get_weights",
-    "cup3d_poisson_cnn_predict", "cup3d_poisson_cnn_train_step",
+    "cup3d_poisson_cnn_predict", "cup3d_poisson_cnn_add_sample",
+    "cup3d_poisson_cnn_clear_samples", "cup3d_poisson_cnn_num_samples", "cup3d_poisson_cnn_train",
+    "cup3d_poisson_cnn_eval",
     "cup3d_poisson_num_interface_rows", "cup3d_poisson_num_interface_entries",
     "cup3d_poisson_destroy", "cup3d_poisson_last_error",
 ]
@@ -73,8 +75,12 @@ def lib():
     L.cup3d_poisson_cnn_set_weights.argtypes = [_vp, _vp, ctypes.c_int]
     L.cup3d_poisson_cnn_get_weights.argtypes = [_vp, _vp, ctypes.c_int]
     L.cup3d_poisson_cnn_predict.argtypes = [_vp, _vp, _vp, _vp]
-    L.cup3d_poisson_cnn_train_step.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_float,
-                                               ctypes.POINTER(ctypes.c_double)]
+    L.cup3d_poisson_cnn_add_sample.argtypes = [_vp, _vp, _vp, _vp]
+    L.cup3d_poisson_cnn_clear_samples.argtypes = [_vp]
+    L.cup3d_poisson_cnn_num_samples.argtypes = [_vp]
+    L.cup3d_poisson_cnn_train.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                          ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+    L.cup3d_poisson_cnn_eval.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.cup3d_poisson_num_interface_rows.argtypes = [_vp]
     L.cup3d_poisson_num_interface_rows.restype = ctypes.c_int64
     L.cup3d_poisson_num_interface_entries.argtypes = [_vp]
@@ -98,7 +104,7 @@ class PoissonSolver:
 
     def __init__(self, bpd, h0, bc, level_max=0, tol_abs=0.0, tol_rel=1e-7, tol_rhs=0.0,
                  bmean=1, max_iter=1000, max_restarts=100, precision="fp64", guess="given",
-                 device=0, inner_tol=1e-4, max_refine=10, poll_every=8):
+                 device=0, inner_tol=1e-4, max_refine=10, poll_every=8, gemm_passes=3):
         self.L = lib()
         c = Config()
         for d in range(3):
@@ -111,6 +117,7 @@ class PoissonSolver:
         c.guess = GUESSES[guess] if isinstance(guess, str) else int(guess)
         c.device, c.inner_tol, c.max_refine = int(device), float(inner_tol), int(max_refine)
         c.poll_every = int(poll_every)
+        c.gemm_passes = int(gemm_passes)
         self.cfg = c
         h = _vp()
         if self.L.cup3d_poisson_create(ctypes.byref(c), ctypes.byref(h)) != 0:
@@ -217,14 +224,29 @@ class PoissonSolver:
             raise self._err("cnn_predict")
         return out
 
-    def cnn_train_step(self, rhs, prev, target, lr):
+    def cnn_add_sample(self, rhs, prev, p):
         import torch
         torch.cuda.synchronize(self.device)
-        ns = rhs.numel() // self.n
+        if self.L.cup3d_poisson_cnn_add_sample(self.h, _ptr(rhs), _ptr(prev), _ptr(p)) != 0:
+            raise self._err("cnn_add_sample")
+
+    def cnn_clear_samples(self):
+        self.L.cup3d_poisson_cnn_clear_samples(self.h)
+
+    def cnn_num_samples(self):
+        return self.L.cup3d_poisson_cnn_num_samples(self.h)
+
+    def cnn_train(self, steps, batch=256, lr=1e-3, seed=0):
+        loss = (ctypes.c_double * 2)()
+        if self.L.cup3d_poisson_cnn_train(self.h, int(steps), int(batch), float(lr), int(seed),
+                                          loss) != 0:
+            raise self._err("cnn_train")
+        return loss[0], loss[1]
+
+    def cnn_eval(self, k):
         loss = ctypes.c_double()
-        if self.L.cup3d_poisson_cnn_train_step(self.h, _ptr(rhs), _ptr(prev), _ptr(target), ns,
-                                               float(lr), ctypes.byref(loss)) != 0:
-            raise self._err("cnn_train_step")
+        if self.L.cup3d_poisson_cnn_eval(self.h, int(k), ctypes.byref(loss)) != 0:
+            raise self._err("cnn_eval")
         return loss.value
 
 

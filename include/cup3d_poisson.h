@@ -87,6 +87,8 @@ typedef struct {
   double inner_tol;
   int max_refine;
   int poll_every;    /* iterations between host polls of the device status (0 = default) */
+  int gemm_passes;   /* mixed only: 3 (or 0) = split hi/lo operands, three MFMA passes,
+                        FP32-accurate; 1 = one FP16/BF16 pass */
 } cup3d_poisson_config;
 
 typedef struct {
@@ -135,17 +137,31 @@ int cup3d_poisson_apply_operator(cup3d_poisson_t h, const double* d_x, double* d
 /* z = blockdiag(h_b^-1 P_inv) p on the device, in the handle's precision. */
 int cup3d_poisson_apply_preconditioner(cup3d_poisson_t h, const double* d_p, double* d_z);
 
-/* Learned warm start (precision-independent). n_weights = cup3d_cnn_num_weights(). */
+/* Learned warm start (precision-independent). A 3-layer 3-D CNN per 8^3
+ * block with a one-cell halo predicts the pressure from (rhs, previous
+ * pressure); inference runs on FP16 matrix cores, training in FP32, both by
+ * this library's kernels. Training triples are copied into a device-resident
+ * training set owned by the handle.
+ * n_weights = cup3d_cnn_num_weights(). */
 int cup3d_cnn_num_weights(void);
 int cup3d_poisson_cnn_set_weights(cup3d_poisson_t h, const float* w, int n_weights);
 int cup3d_poisson_cnn_get_weights(cup3d_poisson_t h, float* w, int n_weights);
 /* d_out = net(d_rhs, d_prev) on all blocks (device pointers, doubles). */
 int cup3d_poisson_cnn_predict(cup3d_poisson_t h, const double* d_rhs, const double* d_prev,
                               double* d_out);
-/* One Adam step on a batch of training triples (rhs, p_prev, p) held on the
- * device as [nsamples][nblocks][512] doubles; returns the mean loss. */
-int cup3d_poisson_cnn_train_step(cup3d_poisson_t h, const double* d_rhs, const double* d_prev,
-                                 const double* d_target, int nsamples, float lr, double* loss);
+/* Append one triple (rhs, p_prev, p) (device pointers) to the training set. */
+int cup3d_poisson_cnn_add_sample(cup3d_poisson_t h, const double* d_rhs, const double* d_prev,
+                                 const double* d_p);
+int cup3d_poisson_cnn_clear_samples(cup3d_poisson_t h);
+int cup3d_poisson_cnn_num_samples(cup3d_poisson_t h);
+/* `steps` Adam steps on minibatches of `batch` random (sample, block) items
+ * drawn with `seed`; loss = 0.5 mean (net - target)^2 in normalised units.
+ * loss[0] = first minibatch loss, loss[1] = last. */
+int cup3d_poisson_cnn_train(cup3d_poisson_t h, int steps, int batch, float lr, uint64_t seed,
+                            double* loss);
+/* Mean loss of the current weights over every block of training sample k
+ * (FP32 forward), for validation. */
+int cup3d_poisson_cnn_eval(cup3d_poisson_t h, int k, double* loss);
 
 /* Introspection for tests and reports. */
 int64_t cup3d_poisson_num_interface_rows(cup3d_poisson_t h);

@@ -101,3 +101,11 @@ def test_topology_rejects_unbalanced_grid():
     blocks = np.array([[2, 0, 0, 0]] + [[0, 1, 0, 0]], np.int32)
     with pytest.raises(AssertionError):
         debug_csr(blocks, (2, 1, 1), 1 / 16, (2, 2, 2), 0)
+
+
+def test_adapter_library_links():
+    """The C++ adapter resolves every C ABI symbol it uses."""
+    L = ctypes.CDLL(os.path.join(ROOT, "cup3d_amd", "lib", "libcup3d_adapter.so"))
+    assert L is not None
+    exe = os.path.join(ROOT, "cup3d_amd", "lib", "example_projection")
+    assert os.access(exe, os.X_OK)

@@ -1,0 +1,123 @@
+"""Learned warm start on the GPU: FP16-MFMA inference and FP32 training
+kernels against the FP64 CNN oracle (oracle/cnn_oracle.py), and the fallback
+rule of the warm start."""
+import ctypes
+
+import cnn_oracle as C
+import numpy as np
+import pytest
+import torch
+
+from cup3d_amd import workloads as W
+from cup3d_amd.poisson import PoissonSolver, lib, solver_for
+from helpers import grid_cases, random_field
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64).reshape(-1)).cuda()
+
+
+def nbr_table(blocks, bpd, bc):
+    """Same-level face neighbours (x-, x+, y-, y+, z-, z+), -1 otherwise."""
+    idx = {tuple(b): i for i, b in enumerate(np.asarray(blocks).tolist())}
+    out = -np.ones((len(blocks), 6), np.int64)
+    for i, (l, a, b, c) in enumerate(np.asarray(blocks).tolist()):
+        I = [a, b, c]
+        for f in range(6):
+            d, s = f // 2, (1 if f % 2 else -1)
+            n = bpd[d] << l
+            J = list(I)
+            J[d] += s
+            if J[d] < 0 or J[d] >= n:
+                if bc[d] != 1:
+                    continue
+                J[d] %= n
+            out[i, f] = idx.get((l, *J), -1)
+    return out
+
+
+def smooth_fields(blocks, h0, seed):
+    from helpers import cell_centres
+    X, h = cell_centres(blocks, h0)
+    rng = np.random.default_rng(seed)
+    k = rng.uniform(1, 3, (2, 3))
+    prev = np.cos(2 * np.pi * (X @ k[0])) + 0.1 * rng.standard_normal(X.shape[:-1])
+    rhs = (h ** 3)[:, None, None, None] * np.sin(2 * np.pi * (X @ k[1]))
+    return rhs, prev, h
+
+
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "amr2_periodic_offcentre", "amr3_walls"])
+def test_predict_matches_oracle(gpu, name):
+    blocks, bpd, h0, bc = grid_cases()[name]
+    S = PoissonSolver(bpd, h0, bc).set_topology(blocks)
+    w = C.init_weights(5)
+    w[C.NW - 1] = 0.05
+    S.cnn_set_weights(w.astype(np.float32))
+    rhs, prev, h = smooth_fields(blocks, h0, 1)
+    out = torch.zeros(len(blocks) * 512, dtype=torch.float64, device=gpu)
+    S.cnn_predict(dev(rhs), dev(prev), out)
+    ref = C.predict(w, rhs, prev, h, nbr_table(blocks, bpd, bc))
+    inc_g = out.cpu().numpy().reshape(ref.shape) - prev
+    inc_o = ref - prev
+    # FP16 weights and activations, FP32 accumulation: ~1e-3 of the increment
+    rel = np.linalg.norm(inc_g - inc_o) / np.linalg.norm(inc_o)
+    assert rel < 1e-2, rel
+
+
+def test_gradient_matches_oracle(gpu):
+    blocks, bpd, h0, bc = grid_cases()["amr2_periodic_offcentre"]
+    S = PoissonSolver(bpd, h0, bc).set_topology(blocks)
+    w = C.init_weights(7)
+    w[C.NW - 1] = 0.02
+    S.cnn_set_weights(w.astype(np.float32))
+    rhs, prev, h = smooth_fields(blocks, h0, 2)
+    p = prev + 0.3 * random_field(len(blocks), 3)
+    S.cnn_add_sample(dev(rhs), dev(prev), dev(p))
+    L = lib()
+    f = L.cup3d_debug_cnn_gradient
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+    g = np.zeros(C.NW, np.float32)
+    loss = ctypes.c_double()
+    assert f(S.h, 0, g.ctypes.data, ctypes.byref(loss)) == 0
+    x, s = C.inputs(rhs, prev, h, nbr_table(blocks, bpd, bc))
+    target = (p - prev) / s
+    lo, go = C.loss_and_grad(w.astype(np.float32).astype(np.float64), x, target)
+    assert abs(loss.value - lo) < 1e-4 * lo
+    assert np.linalg.norm(g - go) < 1e-3 * np.linalg.norm(go)
+
+
+def test_training_reduces_loss_and_fallback(gpu):
+    """Train on a short sequence of solver triples at small size; the loss
+    falls, and the warm start never uses a guess worse than p_prev."""
+    w = W.c4_uniform_sequence((8, 4, 4))
+    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-9, bmean=1)
+    prev = torch.zeros(w.ncells, dtype=torch.float64, device=gpu)
+    rhs0 = dev(w.rhs(step=0))
+    S.solve_device(rhs0, prev)
+    sols = []
+    for t in range(1, 11):
+        b = dev(w.rhs(step=t))
+        x = prev.clone()
+        S.solve_device(b, x)
+        sols.append((b, prev.clone(), x.clone()))
+        prev = x
+    for b, pp, x in sols[:8]:
+        S.cnn_add_sample(b, pp, x)
+    l0, _ = S.cnn_train(1, batch=64, lr=1e-3, seed=1)
+    _, l1 = S.cnn_train(200, batch=64, lr=1e-3, seed=2)
+    assert l1 < 0.5 * l0, (l0, l1)
+    # held-out step: warm start with fallback
+    Sg = solver_for(w, tol_rel=0.0, tol_rhs=1e-7, guess="cnn")
+    Sg.cnn_set_weights(S.cnn_get_weights())
+    b, pp, _ = sols[9]
+    x = pp.clone()
+    rep = Sg.solve_device(b, x)
+    Sp = solver_for(w, tol_rel=0.0, tol_rhs=1e-7)
+    x2 = pp.clone()
+    rep2 = Sp.solve_device(b, x2)
+    assert rep["status"] == 0 and rep2["status"] == 0
+    assert rep["guess_resid"] <= rep2["err_init"] * (1 + 1e-12)
+    print("guess used", rep["guess_used"], "resid net/prev", rep["guess_resid"] / rep2["err_init"],
+          "iters", rep["iterations"], "vs", rep2["iterations"])
