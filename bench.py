@@ -1,20 +1,27 @@
 """Benchmark: pressure-Poisson solves per second on one MI355X.
 
-A step is one complete solve (rhs resident in HBM in Cubism block layout ->
-pressure written back in that layout) to the true relative residual
-||b - A x|| / ||b|| <= 1e-7, warm-started from the previous step's pressure.
+A step is one complete solve — rhs resident in HBM in Cubism's 8^3 block
+layout, pressure written back in that layout — to the true relative residual
+||b - A x|| / ||b|| <= 1e-7 (recomputed in FP64 on the device), warm-started
+from the previous step's pressure.
 
-Default workload: the FP64 drop-in on the uniform 256x128x128 walls grid
-(32x16x16 blocks of 8^3, 256 Gaussian blobs + penalised sphere). The line
-carries the roofline of the dominant kernel (timed live with HIP events on
-the library's stream) and a CPU baseline (the oracle's restatement of the
-reference CPU solver on a bounded sample of the same workload).
+Default workload: uniform 256x128x128 cells (32x16x16 blocks), domain
+[0,2]x[0,1]^2, walls, rhs = divergence of 256 Gaussian blobs with a
+penalised sphere; the previous pressure is the solution of the step before
+(blob centres 0.005 earlier). Two forms are timed on the same input:
+  * the FP64 drop-in (the reference algorithm, FP64 MFMA block-Jacobi GEMM);
+  * the chosen form (headline `value`): FP16 split-precision MFMA GEMM with
+    FP32 accumulation, FP32 Krylov vectors, FP64 iterative refinement.
+The line carries the roofline of the headline form's dominant kernel (HIP
+events on the library's stream) and a CPU baseline (the oracle's restatement
+of the reference CPU solver on a bounded sample of the same workload).
 
-  python bench.py [--steps K] [--warmup W] [--form fp64|mixed_fp16|mixed_bf16]
+  python bench.py [--steps K] [--warmup W] [--form mixed_fp16|mixed_bf16|fp64]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -25,8 +32,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Poisson solves/sec at rel. residual 1e-7 (mixed+learned vs FP64); HBM GB/s"
 HBM_PEAK = 8000.0       # GB/s, MI355X
-FP64_MFMA_PEAK = 78.6   # TFLOP/s, MI355X dense FP64 matrix
-F16_MFMA_PEAK = 2500.0  # TFLOP/s, MI355X dense FP16/BF16 matrix
+FP64_MFMA_PEAK = 78.6   # TFLOP/s, dense FP64 matrix
+F16_MFMA_PEAK = 2500.0  # TFLOP/s, dense FP16/BF16 matrix
+KERNELS = {0: "spmv", 1: "precond_gemm", 2: "update_K5", 3: "precond_update_K1",
+           4: "spmv_dot_K2"}
+CALLS_PER_ITER = {2: 1, 3: 2, 4: 2}  # K1 and K3 share the GEMM kernel, K2/K4 the SpMV
 
 
 def parse():
@@ -34,38 +44,76 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--form", default="fp64", choices=["fp64", "mixed_fp16", "mixed_bf16"])
-    p.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    p.add_argument("--form", default="mixed_fp16", choices=["fp64", "mixed_fp16", "mixed_bf16"])
+    p.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    p.add_argument("--bmean", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    p.add_argument("--cpu-iters", type=int, default=12)
+    p.add_argument("--cpu-iters", type=int, default=10)
     return p.parse_args()
 
 
 def make_workload(name):
     from cup3d_amd import workloads as W
-    return W.c1_periodic_32() if name == "c1" else W.c2_uniform_walls()
+    return {"c1": W.c1_periodic_32, "c2": W.c2_uniform_walls, "c3": W.c3_amr_sphere}[name]()
 
 
-def cpu_baseline(w, rhs, x0, gpu_iters, iters):
+def time_form(w, form, bmean, rhs, p_prev, steps, warmup):
+    import torch
+    from cup3d_amd.poisson import solver_for
+    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-7, precision=form, bmean=bmean, max_iter=3000)
+    x = torch.empty_like(p_prev)
+    reps = []
+    for _ in range(warmup):
+        x.copy_(p_prev)
+        reps.append(S.solve_device(rhs, x))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        x.copy_(p_prev)
+        reps.append(S.solve_device(rhs, x))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rep = reps[-1]
+    ok = all(r["status"] == 0 and r["true_rel_resid"] <= 1.0e-7 for r in reps)
+    return S, x.clone(), rep, dt / steps * 1e3, ok
+
+
+def kernel_times(S, which_list, reps=20):
+    from cup3d_amd.poisson import lib
+    L = lib()
+    f = L.cup3d_debug_time_kernel
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + [ctypes.POINTER(ctypes.c_double)] * 3
+    out = {}
+    for which in which_list:
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        if f(S.h, which, reps, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)) != 0:
+            raise RuntimeError(L.cup3d_poisson_last_error(S.h).decode())
+        sec = a.value * 1e-3
+        out[which] = {"avg_us": a.value * 1e3, "bytes": b.value, "flops": c.value,
+                      "GB/s": b.value / sec / 1e9, "TFLOP/s": c.value / sec / 1e12}
+    return out
+
+
+def cpu_baseline(w, bmean, rhs, x0, gpu_iters, iters):
     """Reference CPU algorithm (pipelined BiCGSTAB + per-block CG), restated in
     oracle/oracle.cpp and built with the reference's release flags; timed for
     `iters` iterations on the full workload. solves/s = 1 / (seconds per
-    iteration * iterations the FP64 solve needs)."""
+    iteration * iterations of the FP64 GPU solve)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    O = pyoracle.Oracle(w.blocks, w.bpd, w.h0, w.bc, 1, fast=True)
-    O.pipelined(rhs, x0, tol_rel=0.0, max_iter=1)  # warm-up (allocations, first touch)
+    O = pyoracle.Oracle(w.blocks, w.bpd, w.h0, w.bc, bmean, fast=True)
+    O.pipelined(rhs, x0, tol_rel=0.0, max_iter=1)  # first touch of every vector
     t0 = time.perf_counter()
     O.pipelined(rhs, x0, tol_rel=0.0, max_iter=iters)
     dt = (time.perf_counter() - t0) / iters
     return {
-        "value": 1.0 / (dt * gpu_iters),
+        "value": round(1.0 / (dt * gpu_iters), 4),
         "unit": "solves/s",
         "cores": pyoracle.num_threads(fast=True),
         "kind": "port",
         "sample": (f"{iters} iterations of the pipelined BiCGSTAB + block-CG CPU solver on the "
-                   f"full {w.ncells}-cell workload ({dt:.3f} s/iteration), scaled by the "
-                   f"{gpu_iters} iterations of the FP64 solve"),
+                   f"full {w.ncells}-cell workload ({dt:.3f} s/iteration), scaled to one solve "
+                   f"by the {gpu_iters} iterations the FP64 GPU solve needs"),
     }
 
 
@@ -75,56 +123,42 @@ def main():
         print("bench.py measures one GPU: every box in this run has a single MI355X; "
               "the multi-GPU sharding is described in DESIGN.md, not launched.", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
-    from cup3d_amd.poisson import solver_for, lib
+    from cup3d_amd.poisson import solver_for
 
     dev = torch.device("cuda:0")
     w = make_workload(args.workload)
     rhs_prev = torch.from_numpy(w.rhs(step=0).reshape(-1)).to(dev)
     rhs = torch.from_numpy(w.rhs(step=1).reshape(-1)).to(dev)
-    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-7, precision=args.form)
-    # previous-step pressure (untimed setup)
+    # previous-step pressure and an FP64 reference solution (untimed setup)
+    R = solver_for(w, tol_rel=0.0, tol_rhs=1e-10, bmean=args.bmean, max_iter=5000)
     p_prev = torch.zeros(w.ncells, dtype=torch.float64, device=dev)
-    S.solve_device(rhs_prev, p_prev)
-    x = torch.empty_like(p_prev)
-    reps = []
-    for _ in range(args.warmup):
-        x.copy_(p_prev)
-        reps.append(S.solve_device(rhs, x))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        x.copy_(p_prev)
-        reps.append(S.solve_device(rhs, x))
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    rep = reps[-1]
-    assert rep["status"] == 0 and rep["true_rel_resid"] <= 1.05e-7, rep
-    ms = dt / args.steps * 1e3
-    iters = rep["iterations"]
+    R.solve_device(rhs_prev, p_prev)
+    x_ref = p_prev.clone()
+    R.solve_device(rhs, x_ref)
 
-    # per-kernel timing (HIP events on the library stream, data of the last solve)
-    import ctypes
-    L = lib()
-    f = L.cup3d_debug_time_kernel
-    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + [ctypes.POINTER(ctypes.c_double)] * 3
-    kern = {}
-    for which, name in ((0, "spmv"), (1, "precond_gemm"), (2, "update")):
-        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        if f(S.h, which, 20, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)) != 0:
-            raise RuntimeError(L.cup3d_poisson_last_error(S.h).decode())
-        kern[name] = {"avg_us": a.value * 1e3, "GB/s": b.value / (a.value * 1e-3) / 1e9,
-                      "TFLOP/s": c.value / (a.value * 1e-3) / 1e12}
-    g = kern["precond_gemm"]
+    S64, x64, rep64, ms64, ok64 = time_form(w, "fp64", args.bmean, rhs, p_prev, args.steps,
+                                            args.warmup)
     if args.form == "fp64":
-        roof = {"bound": "mfma", "achieved": round(g["TFLOP/s"], 3), "peak": FP64_MFMA_PEAK,
-                "unit": "TFLOP/s", "frac": round(g["TFLOP/s"] / FP64_MFMA_PEAK, 4),
-                "traffic": None, "kernel": "k_precond_f64 (block-Jacobi GEMM, 1024 flop/cell)"}
+        S, x, rep, ms, ok = S64, x64, rep64, ms64, ok64
     else:
-        roof = {"bound": "hbm", "achieved": round(g["GB/s"], 1), "peak": HBM_PEAK, "unit": "GB/s",
-                "frac": round(g["GB/s"] / HBM_PEAK, 4), "traffic": None,
-                "kernel": "precond GEMM (8 B/cell FP32 in+out)"}
+        S, x, rep, ms, ok = time_form(w, args.form, args.bmean, rhs, p_prev, args.steps,
+                                      args.warmup)
+    if not ok:
+        raise SystemExit(f"solve did not reach the tolerance: {rep}")
+
+    def rel(a, b):
+        return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+    kern = kernel_times(S, [2, 3, 4])
+    dom = max(kern, key=lambda k: kern[k]["avg_us"] * CALLS_PER_ITER[k])
+    kd = kern[dom]
+    roof = {"bound": "hbm", "achieved": round(kd["GB/s"], 1), "peak": HBM_PEAK, "unit": "GB/s",
+            "frac": round(kd["GB/s"] / HBM_PEAK, 4), "traffic": None,
+            "kernel": KERNELS[dom], "avg_us": round(kd["avg_us"], 2),
+            "alg_bytes_per_launch": kd["bytes"]}
+    k64 = kernel_times(S64, [1, 2, 3, 4])
+    g64 = k64[1]
     out = {
         "metric": METRIC,
         "value": round(1e3 / ms, 3),
@@ -136,19 +170,32 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "fp64" else "f32+f16" if args.form == "mixed_fp16" else "f32+bf16",
+        "dtype": {"fp64": "f64", "mixed_fp16": "f32+f16mfma(f64 refine)",
+                  "mixed_bf16": "f32+bf16mfma(f64 refine)"}[args.form],
         "data": "synthetic",
         "config": {"workload": w.name, "form": args.form, "cells": w.ncells,
                    "blocks": int(len(w.blocks)), "tolerance": "||b-Ax||/||b|| <= 1e-7",
-                   "warm_start": "previous-step pressure", "parallelism": "single GPU"},
-        "iterations": iters,
-        "cell_iterations_per_s": round(w.ncells * iters / (ms * 1e-3), 1),
+                   "warm_start": "previous-step pressure", "bMeanConstraint": args.bmean,
+                   "parallelism": "single GPU"},
+        "iterations": rep["iterations"],
+        "refinements": rep["refinements"],
+        "cell_iterations_per_s": round(w.ncells * rep["iterations"] / (ms * 1e-3), 1),
         "true_rel_resid": rep["true_rel_resid"],
-        "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kern.items()},
+        "err_vs_fp64_reference": rel(x, x_ref),
+        "fp64": {"solves_per_s": round(1e3 / ms64, 3), "ms_per_step": round(ms64, 3),
+                 "iterations": rep64["iterations"], "true_rel_resid": rep64["true_rel_resid"],
+                 "err_vs_fp64_reference": rel(x64, x_ref),
+                 "gemm_TFLOP/s": round(g64["TFLOP/s"], 2),
+                 "gemm_mfma_frac": round(g64["TFLOP/s"] / FP64_MFMA_PEAK, 4),
+                 "kernels_us": {KERNELS[k]: round(v["avg_us"], 2) for k, v in k64.items()}},
+        "speedup_vs_fp64": round(ms64 / ms, 3),
+        "kernels": {KERNELS[k]: {kk: round(vv, 3) for kk, vv in v.items()}
+                    for k, v in kern.items()},
         "roofline": roof,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, rhs.cpu().numpy(), p_prev.cpu().numpy(), iters,
+        out["cpu_baseline"] = cpu_baseline(w, args.bmean, rhs.cpu().numpy(),
+                                           p_prev.cpu().numpy(), rep64["iterations"],
                                            args.cpu_iters)
     print(json.dumps(out))
 

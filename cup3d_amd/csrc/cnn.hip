@@ -78,8 +78,14 @@ __global__ __launch_bounds__(WG_THREADS) void k_cnn_rms(long long n, const doubl
     const long long i = base + e;
     if (i < n) v[0] += f[i] * f[i];
   }
+  grid_store_partial<1>(v, part);
+}
+
+__global__ __launch_bounds__(WG_THREADS) void k_cnn_rms_fin(long long n, double* scal,
+                                                            const double* part, int nparts) {
   double tot[1];
-  if (grid_reduce_last<1>(v, part, counter, tot) && threadIdx.x == 0) {
+  grid_sum_partials<1>(part, nparts, tot);
+  if (threadIdx.x == 0) {
     const double s = sqrt(tot[0] / (double)n);
     scal[0] = s > 0 ? s : 1.0;
   }
@@ -509,6 +515,7 @@ int cnn_get_weights(CnnState& C, float* w, int nw, std::string& err) {
 static int rms(CnnState& C, hipStream_t s, const double* f, std::string& err) {
   const int nwg = (int)((C.n + 4 * BLEN - 1) / (4 * BLEN));
   k_cnn_rms<<<nwg, WG_THREADS, 0, s>>>(C.n, f, C.d_scal, C.d_counter, C.d_part);
+  k_cnn_rms_fin<<<1, WG_THREADS, 0, s>>>(C.n, C.d_scal, C.d_part, nwg);
   CCHECK(hipGetLastError());
   return 0;
 }

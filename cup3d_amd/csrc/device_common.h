@@ -92,4 +92,40 @@ __device__ bool grid_reduce_last(double (&v)[K], double* __restrict__ part,
   return true;
 }
 
+// Producer side of the split reduction: each workgroup stores its K partial
+// sums (plain stores; the kernel boundary makes them visible to the
+// finalize kernel that follows on the same stream).
+template <int K>
+__device__ __forceinline__ void grid_store_partial(double (&v)[K], double* __restrict__ part) {
+  __shared__ double red[K][4];
+  wg_sum<K>(v, red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) part[(size_t)blockIdx.x * K + k] = v[k];
+  }
+}
+
+// Consumer side (one workgroup): sums the partials of `nparts` producer
+// workgroups in workgroup order; the totals are valid in every thread.
+template <int K>
+__device__ __forceinline__ void grid_sum_partials(const double* __restrict__ part, int nparts,
+                                                  double (&tot)[K]) {
+  __shared__ double red[K][4];
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] += part[(size_t)i * K + k];
+  }
+  wg_sum<K>(acc, red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[k][0] = acc[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) tot[k] = red[k][0];
+}
+
 }  // namespace cup3d

@@ -89,19 +89,26 @@ __global__ __launch_bounds__(WG_THREADS) void k_precond_f64(int nb, const double
 #pragma unroll
   for (int t = 0; t < 8; ++t) acc[t] = (double4_t){0.0, 0.0, 0.0, 0.0};
   const double* __restrict__ pb = Ppk + (size_t)(8 * w) * 128 * 64 + l;
-  double bq[8];
+  // P fragments stream from L2 through a 4-stage register ring: the loads for
+  // k-step kt+4 are issued while the MFMAs of kt run, hiding L2 latency.
+  constexpr int NST = 4;
+  double bq[NST][8];
 #pragma unroll
-  for (int t = 0; t < 8; ++t) bq[t] = pb[(size_t)t * 128 * 64];
-  for (int kt = 0; kt < 128; ++kt) {
-    const double av = Vs[lr * PC_LD + 4 * kt + lk];
-    double bn[8];
-    const int kn = kt + 1 < 128 ? kt + 1 : kt;
+  for (int st_ = 0; st_ < NST; ++st_)
 #pragma unroll
-    for (int t = 0; t < 8; ++t) bn[t] = pb[((size_t)t * 128 + kn) * 64];
+    for (int t = 0; t < 8; ++t) bq[st_][t] = pb[((size_t)t * 128 + st_) * 64];
+  for (int k0 = 0; k0 < 128; k0 += NST) {
 #pragma unroll
-    for (int t = 0; t < 8; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[t], acc[t], 0, 0, 0);
+    for (int st_ = 0; st_ < NST; ++st_) {
+      const int kt = k0 + st_;
+      const double av = Vs[lr * PC_LD + 4 * kt + lk];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) bq[t] = bn[t];
+      for (int t = 0; t < 8; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[st_][t], acc[t], 0, 0, 0);
+      const int kn = kt + NST < 128 ? kt + NST : kt;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) bq[st_][t] = pb[((size_t)t * 128 + kn) * 64];
+    }
   }
   // D layout of v_mfma_f64_16x16x4f64: register q of lane l holds
   // D[l/16 + 4q][l%16]
@@ -126,33 +133,17 @@ void launch_getvec64(hipStream_t s, long long n, long long zero_row, const doubl
                      double tol_b) {
   const int nwg = (int)((n + 4 * BLEN - 1) / (4 * BLEN));
   k_getvec<double><<<nwg, WG_THREADS, 0, s>>>(n, zero_row, rhs, x0, b, x, st, part, tol_b);
-}
-
-template <int MODE, int BM>
-static void spmv_bm(hipStream_t s, const DevGrid<double>& g, const double* x, double* y,
-                    const double* a, double* c, double* c2, SolverState* st, double* part) {
-  k_spmv<double, MODE, BM><<<wg_for_blocks(g.nb, 4), WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st,
-                                                                         part);
-}
-
-template <int MODE>
-static void spmv_mode(hipStream_t s, const DevGrid<double>& g, const double* x, double* y,
-                      const double* a, double* c, double* c2, SolverState* st, double* part) {
-  switch (g.bmean) {
-    case 1: spmv_bm<MODE, 1>(s, g, x, y, a, c, c2, st, part); break;
-    case 3: spmv_bm<MODE, 3>(s, g, x, y, a, c, c2, st, part); break;
-    default: spmv_bm<MODE, 0>(s, g, x, y, a, c, c2, st, part); break;
-  }
+  k_getvec_fin<<<1, WG_THREADS, 0, s>>>(st, part, nwg, tol_b);
 }
 
 void launch_spmv64(hipStream_t s, int mode, const DevGrid<double>& g, const double* x, double* y,
                    const double* a, double* c, double* c2, SolverState* st, double* part) {
   switch (mode) {
-    case SPMV_PLAIN: spmv_mode<SPMV_PLAIN>(s, g, x, y, a, c, c2, st, part); break;
-    case SPMV_INIT: spmv_mode<SPMV_INIT>(s, g, x, y, a, c, c2, st, part); break;
-    case SPMV_DOT1: spmv_mode<SPMV_DOT1>(s, g, x, y, a, c, c2, st, part); break;
-    case SPMV_DOT2: spmv_mode<SPMV_DOT2>(s, g, x, y, a, c, c2, st, part); break;
-    case SPMV_RESID: spmv_mode<SPMV_RESID>(s, g, x, y, a, c, c2, st, part); break;
+    case SPMV_PLAIN: spmv_launch<double, SPMV_PLAIN>(s, g, x, y, a, c, c2, st, part); break;
+    case SPMV_INIT: spmv_launch<double, SPMV_INIT>(s, g, x, y, a, c, c2, st, part); break;
+    case SPMV_DOT1: spmv_launch<double, SPMV_DOT1>(s, g, x, y, a, c, c2, st, part); break;
+    case SPMV_DOT2: spmv_launch<double, SPMV_DOT2>(s, g, x, y, a, c, c2, st, part); break;
+    case SPMV_RESID: spmv_launch<double, SPMV_RESID>(s, g, x, y, a, c, c2, st, part); break;
   }
 }
 
@@ -170,6 +161,7 @@ void launch_update64(hipStream_t s, long long n, const KrylovVectors<double>& v,
                      double* part) {
   const int nwg = (int)((n + 4 * BLEN - 1) / (4 * BLEN));
   k_update<double><<<nwg, WG_THREADS, 0, s>>>(n, v, st, part);
+  k_update_fin<<<1, WG_THREADS, 0, s>>>(st, part, nwg);
 }
 
 void launch_copy_opt64(hipStream_t s, long long n, const KrylovVectors<double>& v,

@@ -43,6 +43,31 @@ __device__ inline void begin_iteration(SolverState* st, double rho, double rr) {
   st->iter += 1;
 }
 
+// Contiguous 8-element vector access (16-byte loads/stores).
+template <typename T>
+__device__ __forceinline__ void load8(const T* __restrict__ p, T (&v)[8]) {
+  typedef T V4 __attribute__((ext_vector_type(16 / sizeof(T))));
+  constexpr int W = 16 / sizeof(T);
+#pragma unroll
+  for (int k = 0; k < 8 / W; ++k) {
+    const V4 q = reinterpret_cast<const V4*>(p)[k];
+#pragma unroll
+    for (int e = 0; e < W; ++e) v[k * W + e] = q[e];
+  }
+}
+template <typename T>
+__device__ __forceinline__ void store8(T* __restrict__ p, const T (&v)[8]) {
+  typedef T V4 __attribute__((ext_vector_type(16 / sizeof(T))));
+  constexpr int W = 16 / sizeof(T);
+#pragma unroll
+  for (int k = 0; k < 8 / W; ++k) {
+    V4 q;
+#pragma unroll
+    for (int e = 0; e < W; ++e) q[e] = v[k * W + e];
+    reinterpret_cast<V4*>(p)[k] = q;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // getVec (PoissonSolverExp.cpp:375-418): b = rhs, x = x0, both zeroed at the
 // bMean cell for bMeanConstraint 1 and 3; also ||b||^2.
@@ -55,19 +80,31 @@ __global__ __launch_bounds__(WG_THREADS) void k_getvec(long long n, long long ze
                                                        SolverState* st, double* part,
                                                        double tol_b) {
   double v[1] = {0.0};
-  const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
-  for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
-    const long long i = base + e;
-    if (i < n) {
-      double bv = rhs[i], xv = x0 ? x0[i] : 0.0;
-      if (i == zero_row) bv = xv = 0.0;
-      b[i] = (T)bv;
-      x[i] = (T)xv;
-      v[0] += bv * bv;
+  const long long i0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN) + 8 * threadIdx.x;
+  if (i0 < n) {
+    double bv[8], xv[8];
+    load8(rhs + i0, bv);
+    if (x0) load8(x0 + i0, xv);
+    T bo[8], xo[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      if (!x0) xv[e] = 0.0;
+      if (i0 + e == zero_row) bv[e] = xv[e] = 0.0;
+      bo[e] = (T)bv[e];
+      xo[e] = (T)xv[e];
+      v[0] += bv[e] * bv[e];
     }
+    store8(b + i0, bo);
+    store8(x + i0, xo);
   }
+  grid_store_partial<1>(v, part);
+}
+
+static __global__ __launch_bounds__(WG_THREADS) void k_getvec_fin(SolverState* st, const double* part,
+                                                           int nparts, double tol_b) {
   double tot[1];
-  if (grid_reduce_last<1>(v, part, &st->counters[0], tot) && threadIdx.x == 0) {
+  grid_sum_partials<1>(part, nparts, tot);
+  if (threadIdx.x == 0) {
     st->bnorm2 = tot[0];
     const double tb = tol_b * sqrt(tot[0]);
     if (tb > st->tol_abs) st->tol_abs = tb;
@@ -80,126 +117,154 @@ __global__ __launch_bounds__(WG_THREADS) void k_getvec(long long n, long long ze
 // fused reduction epilogue.
 //
 // One wavefront per block: lane = (ix, iy), each lane walks its z column in
-// registers; the block is staged in LDS for the x/y neighbours; face halos
-// of same-level neighbours are read straight from HBM (the z faces as
-// coalesced 512-byte rows).
+// registers; the block is staged in LDS for the x/y neighbours. Every global
+// load (the column, the epilogue operand, the six face halos of same-level
+// neighbours) is issued before the single barrier, so a wave waits for memory
+// once. Grids with coarse-fine faces (IFACE) stage y in LDS once more to add
+// the interface rows.
 // ---------------------------------------------------------------------------
-template <typename T, int MODE, int BM>
+template <typename T, int MODE, int BM, bool IFACE>
 __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __restrict__ x,
                                                      T* __restrict__ y, const T* __restrict__ a,
                                                      T* __restrict__ c, T* __restrict__ c2,
                                                      SolverState* st, double* part) {
+  constexpr bool NEED_A = MODE != SPMV_PLAIN;
   __shared__ T xs[4][BLEN];
-  __shared__ T ys[4][BLEN];
+  __shared__ T ys[IFACE ? 4 : 1][IFACE ? BLEN : 1];
   if (MODE != SPMV_PLAIN && MODE != SPMV_RESID && st->status != ST_RUNNING) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ix = lane & 7, iy = lane >> 3;
   const int b = xcd_remap(blockIdx.x, gridDim.x) * 4 + w;
   const bool valid = b < g.nb;
   const size_t off = (size_t)b * BLEN;
-  T col[BS];
-  if (valid) {
-#pragma unroll
-    for (int iz = 0; iz < BS; ++iz) {
-      col[iz] = x[off + iz * 64 + lane];
-      xs[w][iz * 64 + lane] = col[iz];
-    }
-  }
-  __syncthreads();
-  double msum = 0.0;
+  T col[BS], av[BS], hx[BS], hy[BS], zlo = 0, zhi = 0;
+  T h = 0;
   if (valid) {
     const int* nb6 = g.nbr + b * 6;
     const int nxm = nb6[0], nxp = nb6[1], nym = nb6[2], nyp = nb6[3], nzm = nb6[4], nzp = nb6[5];
-    const T h = g.h[b];
-    const T zlo = nzm >= 0 ? x[(size_t)nzm * BLEN + 7 * 64 + lane] : col[0];
-    const T zhi = nzp >= 0 ? x[(size_t)nzp * BLEN + lane] : col[BS - 1];
+    h = g.h[b];
+#pragma unroll
+    for (int iz = 0; iz < BS; ++iz) col[iz] = x[off + iz * 64 + lane];
+    if (NEED_A) {
+#pragma unroll
+      for (int iz = 0; iz < BS; ++iz) av[iz] = a[off + iz * 64 + lane];
+    }
+    // face halos: a lane is on at most one x face and one y face
+    const int nx = ix == 0 ? nxm : (ix == 7 ? nxp : -1);
+    const int sx = ix == 0 ? 7 : 0;
+    const int ny = iy == 0 ? nym : (iy == 7 ? nyp : -1);
+    const int sy = iy == 0 ? 7 : 0;
+#pragma unroll
+    for (int iz = 0; iz < BS; ++iz) {
+      hx[iz] = nx >= 0 ? x[(size_t)nx * BLEN + iz * 64 + iy * 8 + sx] : col[iz];
+      hy[iz] = ny >= 0 ? x[(size_t)ny * BLEN + iz * 64 + sy * 8 + ix] : col[iz];
+    }
+    zlo = nzm >= 0 ? x[(size_t)nzm * BLEN + 7 * 64 + lane] : col[0];
+    zhi = nzp >= 0 ? x[(size_t)nzp * BLEN + lane] : col[BS - 1];
+#pragma unroll
+    for (int iz = 0; iz < BS; ++iz) xs[w][iz * 64 + lane] = col[iz];
+  }
+  __syncthreads();
+  T yv[BS];
+  double msum = 0.0;
+  if (valid) {
 #pragma unroll
     for (int iz = 0; iz < BS; ++iz) {
       const T c0 = col[iz];
       const int li = iz * 64 + lane;
-      T xm, xp, ym, yp;
-      if (ix > 0) xm = xs[w][li - 1];
-      else xm = nxm >= 0 ? x[(size_t)nxm * BLEN + iz * 64 + iy * 8 + 7] : c0;
-      if (ix < 7) xp = xs[w][li + 1];
-      else xp = nxp >= 0 ? x[(size_t)nxp * BLEN + iz * 64 + iy * 8] : c0;
-      if (iy > 0) ym = xs[w][li - 8];
-      else ym = nym >= 0 ? x[(size_t)nym * BLEN + iz * 64 + 56 + ix] : c0;
-      if (iy < 7) yp = xs[w][li + 8];
-      else yp = nyp >= 0 ? x[(size_t)nyp * BLEN + iz * 64 + ix] : c0;
+      const T xm = ix > 0 ? xs[w][li - 1] : hx[iz];
+      const T xp = ix < 7 ? xs[w][li + 1] : hx[iz];
+      const T ym = iy > 0 ? xs[w][li - 8] : hy[iz];
+      const T yp = iy < 7 ? xs[w][li + 8] : hy[iz];
       const T zm = iz > 0 ? col[iz - 1] : zlo;
       const T zp = iz < BS - 1 ? col[iz + 1] : zhi;
       if (sizeof(T) == 8)
-        ys[w][li] = h * ((((xm + xp) + (ym + yp)) + (zm + zp)) - (T)6 * c0);
+        yv[iz] = h * ((((xm + xp) + (ym + yp)) + (zm + zp)) - (T)6 * c0);
       else  // FP32: differences of neighbouring values are exact (Sterbenz), so
             // the result keeps full relative precision on smooth fields
-        ys[w][li] = h * ((((xm - c0) + (xp - c0)) + ((ym - c0) + (yp - c0))) +
-                         ((zm - c0) + (zp - c0)));
+        yv[iz] = h * ((((xm - c0) + (xp - c0)) + ((ym - c0) + (yp - c0))) +
+                      ((zm - c0) + (zp - c0)));
       msum += (double)c0;
     }
   }
-  __syncthreads();
-  if (valid) {
-    const int r0 = g.irow_ptr[b], r1 = g.irow_ptr[b + 1];
-    for (int r = r0 + lane; r < r1; r += 64) {
-      T acc = 0;
-      const int cl = g.irow_cell[r];
-      if (sizeof(T) == 8) {
-        for (int e = g.irow_ent[r]; e < g.irow_ent[r + 1]; ++e) acc += g.ent_val[e] * x[g.ent_col[e]];
-      } else {  // interface rows sum to zero: use differences to the cell value
-        const T xc = xs[w][cl];
-        for (int e = g.irow_ent[r]; e < g.irow_ent[r + 1]; ++e)
-          acc += g.ent_val[e] * (x[g.ent_col[e]] - xc);
+  if (IFACE) {
+    if (valid) {
+#pragma unroll
+      for (int iz = 0; iz < BS; ++iz) ys[w][iz * 64 + lane] = yv[iz];
+    }
+    __syncthreads();
+    if (valid) {
+      const int r0 = g.irow_ptr[b], r1 = g.irow_ptr[b + 1];
+      for (int r = r0 + lane; r < r1; r += 64) {
+        T acc = 0;
+        const int cl = g.irow_cell[r];
+        if (sizeof(T) == 8) {
+          for (int e = g.irow_ent[r]; e < g.irow_ent[r + 1]; ++e) acc += g.ent_val[e] * x[g.ent_col[e]];
+        } else {  // interface rows sum to zero: use differences to the cell value
+          const T xc = xs[w][cl];
+          for (int e = g.irow_ent[r]; e < g.irow_ent[r + 1]; ++e)
+            acc += g.ent_val[e] * (x[g.ent_col[e]] - xc);
+        }
+        ys[w][cl] += acc;
       }
-      ys[w][cl] += acc;
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int iz = 0; iz < BS; ++iz) yv[iz] = ys[w][iz * 64 + lane];
     }
   }
-  __syncthreads();
   const long long row = g.bmean_row;  // origin cell for BM 1 and 3, else -1
   const bool origin = valid && row == (long long)off;
-  T yv[BS];
-  if (valid) {
-#pragma unroll
-    for (int iz = 0; iz < BS; ++iz) yv[iz] = ys[w][iz * 64 + lane];
-    if (BM == 3 && origin && lane == 0) yv[0] = g.h[b] * col[0];  // row h*I (.cpp:311-318)
-  }
-  // epilogue: stores and partial sums (the bMean row is left to the last WG)
+  if (BM == 3 && origin && lane == 0) yv[0] = h * col[0];  // row h*I (.cpp:311-318)
+  // epilogue: stores and partial sums
   double v[3] = {0.0, 0.0, 0.0};
   if (valid) {
-    const double h = (double)g.h[b];
-    v[0] = h * h * h * msum;
+    v[0] = (double)h * (double)h * (double)h * msum;
 #pragma unroll
     for (int iz = 0; iz < BS; ++iz) {
       const size_t i = off + iz * 64 + lane;
       // the bMean row is written by the last workgroup only, so no two
       // workgroups (possibly on different XCDs) ever store the same address
-      const bool skip = (BM == 1) && origin && lane == 0 && iz == 0;
-      if (skip) continue;
+      if ((BM == 1) && origin && lane == 0 && iz == 0) continue;
       const T yy = yv[iz];
       if (MODE == SPMV_PLAIN) {
         y[i] = yy;
       } else if (MODE == SPMV_INIT) {
         y[i] = yy;
-        const T rr = a[i] - yy;
+        const T rr = av[iz] - yy;
         c[i] = rr;
         c2[i] = rr;
         v[1] += (double)yy * (double)yy;
         v[2] += (double)rr * (double)rr;
       } else if (MODE == SPMV_DOT1) {
         y[i] = yy;
-        v[1] += (double)a[i] * (double)yy;
+        v[1] += (double)av[iz] * (double)yy;
       } else if (MODE == SPMV_DOT2) {
         y[i] = yy;
-        v[1] += (double)a[i] * (double)yy;
+        v[1] += (double)av[iz] * (double)yy;
         v[2] += (double)yy * (double)yy;
       } else if (MODE == SPMV_RESID) {
-        const T rr = a[i] - yy;
+        const T rr = av[iz] - yy;
         c[i] = rr;
         v[1] += (double)rr * (double)rr;
       }
     }
   }
+  grid_store_partial<3>(v, part);
+}
+
+// Finalize of k_spmv (one workgroup): totals, the bMean = 1 row
+// (y[row] = sum_b h_b^3 sum x, BiCGSTAB.cu:406-407) and the scalar updates.
+template <typename T, int MODE, int BM>
+__global__ __launch_bounds__(WG_THREADS) void k_spmv_fin(long long row, T* __restrict__ y,
+                                                         const T* __restrict__ a,
+                                                         T* __restrict__ c, T* __restrict__ c2,
+                                                         SolverState* st, const double* part,
+                                                         int nparts) {
+  if (MODE != SPMV_PLAIN && MODE != SPMV_RESID && st->status != ST_RUNNING) return;
   double tot[3];
-  if (!grid_reduce_last<3>(v, part, &st->counters[1], tot)) return;
+  grid_sum_partials<3>(part, nparts, tot);
   if (threadIdx.x != 0) return;
   // bMean = 1: y[row] = sum_b h_b^3 sum x  (BiCGSTAB.cu:406-407)
   if (BM == 1 && row >= 0) {
@@ -249,6 +314,25 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
   }
 }
 
+// Launch for a grid: picks the bMean and interface variants.
+template <typename T, int MODE, int BM>
+inline void spmv_launch_iface(hipStream_t s, const DevGrid<T>& g, const T* x, T* y, const T* a,
+                              T* c, T* c2, SolverState* st, double* part) {
+  const int nwg = (g.nb + 3) / 4;
+  if (g.has_iface)
+    k_spmv<T, MODE, BM, true><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
+  else
+    k_spmv<T, MODE, BM, false><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
+  k_spmv_fin<T, MODE, BM><<<1, WG_THREADS, 0, s>>>(g.bmean_row, y, a, c, c2, st, part, nwg);
+}
+template <typename T, int MODE>
+inline void spmv_launch(hipStream_t s, const DevGrid<T>& g, const T* x, T* y, const T* a, T* c,
+                        T* c2, SolverState* st, double* part) {
+  if (g.bmean == 1) spmv_launch_iface<T, MODE, 1>(s, g, x, y, a, c, c2, st, part);
+  else if (g.bmean == 3) spmv_launch_iface<T, MODE, 3>(s, g, x, y, a, c, c2, st, part);
+  else spmv_launch_iface<T, MODE, 0>(s, g, x, y, a, c, c2, st, part);
+}
+
 // ---------------------------------------------------------------------------
 // K5: x' = x + alpha z1 + omega z2 (into a free rotating buffer), r = s - omega t,
 // then rho' = (rhat, r), |r|, x_opt, convergence and the next iteration's
@@ -266,22 +350,39 @@ __global__ __launch_bounds__(WG_THREADS) void k_update(long long n, KrylovVector
   if (st->status != ST_RUNNING) return;
   const int xc = st->xcur, xo = st->xopt, xn = third_buffer(xc, xo);
   const T alpha = (T)st->alpha, omega = (T)st->omega;
-  const T* __restrict__ xin = v.x[xc];
-  T* __restrict__ xout = v.x[xn];
   double s[2] = {0.0, 0.0};
-  const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
-  for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
-    const long long i = base + e;
-    if (i < n) {
-      xout[i] = fma(omega, v.z2[i], fma(alpha, v.z1[i], xin[i]));
-      const T r = fma(-omega, v.t[i], v.r[i]);
-      v.r[i] = r;
-      s[0] += (double)v.rhat[i] * (double)r;
-      s[1] += (double)r * (double)r;
+  // each thread owns 8 consecutive elements of this workgroup's 2048
+  const long long i0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN) + 8 * threadIdx.x;
+  if (i0 < n) {
+    T xv[8], z1[8], z2[8], t[8], r[8], rh[8];
+    load8(v.x[xc] + i0, xv);
+    load8(v.z1 + i0, z1);
+    load8(v.z2 + i0, z2);
+    load8(v.t + i0, t);
+    load8(v.r + i0, r);
+    load8(v.rhat + i0, rh);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      xv[e] = fma(omega, z2[e], fma(alpha, z1[e], xv[e]));
+      r[e] = fma(-omega, t[e], r[e]);
+      s[0] += (double)rh[e] * (double)r[e];
+      s[1] += (double)r[e] * (double)r[e];
     }
+    store8(v.x[xn] + i0, xv);
+    store8(v.r + i0, r);
   }
+  grid_store_partial<2>(s, part);
+}
+
+// Finalize of k_update (one workgroup): |r|, x_opt, convergence test, restart
+// and the next iteration's scalars (BiCGSTAB.cu:580-607).
+static __global__ __launch_bounds__(WG_THREADS) void k_update_fin(SolverState* st, const double* part,
+                                                           int nparts) {
+  if (st->status != ST_RUNNING) return;
   double tot[2];
-  if (!grid_reduce_last<2>(s, part, &st->counters[2], tot) || threadIdx.x != 0) return;
+  grid_sum_partials<2>(part, nparts, tot);
+  if (threadIdx.x != 0) return;
+  const int xn = third_buffer(st->xcur, st->xopt);
   const double err = sqrt(tot[1]);
   st->err = err;
   st->iters_done = st->iter;

@@ -19,14 +19,16 @@
 // input (LDS), B = 32 x 16 of P_inv (fragment-packed in global memory, 1 KiB
 // per fragment), D = 16 blocks x 16 output cells (FP32).
 //
-// Split precision (default). A single FP16 pass perturbs every application
-// of M^-1 by ~5e-4 relative (BF16: ~4e-3) in a way that is not linear in the
-// input, and over hundreds of iterations BiCGSTAB loses its
-// bi-orthogonality and stagnates. Both operands are therefore carried as
-// hi + lo pairs (v = hi + 2^-s lo, s = 11 for FP16, 8 for BF16) and the GEMM
-// runs three MFMA passes, hi*hi + 2^-s (hi*lo + lo*hi), each accumulating in
-// FP32: the preconditioner is then linear to FP32 precision while every
-// product still runs on the FP16/BF16 matrix cores.
+// Split input (default). A single FP16 pass rounds every input vector to
+// 11 bits in a data-dependent way, so M^-1 is no longer a fixed linear
+// operator (BiCGSTAB then loses its bi-orthogonality and stagnates over
+// hundreds of iterations). P_inv itself is rounded once (a fixed, linear
+// perturbation of the preconditioner, harmless), but each block's input is
+// carried as hi + lo with lo = v - hi (the block scaling keeps |v| <= 1, so
+// lo's absolute error is ~2^-24 of the block maximum even where it is
+// subnormal) and the GEMM runs two MFMA passes, P*hi + P*lo, into one FP32
+// accumulator: M^-1 is then linear to FP32 precision while every product
+// still runs on the FP16/BF16 matrix cores.
 
 #include <hip/hip_runtime.h>
 
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_precond_h(int nb, const H* __res
       const float x = vals[e] * s;
       const H hi = (H)x;
       As[0][j * HLD + e * 64 + l] = hi;
-      if (SPLIT) As[NS - 1][j * HLD + e * 64 + l] = (H)((x - (float)hi) * Split<H>::lo);
+      if (SPLIT) As[NS - 1][j * HLD + e * 64 + l] = (H)(x - (float)hi);
     }
     if (l == 0) sc[j] = b < nb ? invh[b] * p_unscale / s : 0.f;
   }
@@ -145,19 +147,23 @@ __global__ __launch_bounds__(WG_THREADS) void k_precond_h(int nb, const H* __res
   // phase 2: wave w owns output cells [128 w, 128 w + 128) for all NBW blocks
   constexpr int JT = NBW / 16;
   const int lr = l & 15, g = l >> 4;
-  float4_t acc[8][JT], cor[8][JT];
+  float4_t acc[8][JT];
 #pragma unroll
   for (int t = 0; t < 8; ++t)
 #pragma unroll
-    for (int jt = 0; jt < JT; ++jt) {
-      acc[t][jt] = (float4_t){0.f, 0.f, 0.f, 0.f};
-      cor[t][jt] = (float4_t){0.f, 0.f, 0.f, 0.f};
-    }
-  const size_t foff = (size_t)(8 * w) * 16 * 64 + l;
-  const HV* __restrict__ pb = reinterpret_cast<const HV*>(Ppk) + foff;
-  const HV* __restrict__ pl = reinterpret_cast<const HV*>(SPLIT ? Ppk_lo : Ppk) + foff;
+    for (int jt = 0; jt < JT; ++jt) acc[t][jt] = (float4_t){0.f, 0.f, 0.f, 0.f};
+  const HV* __restrict__ pb = reinterpret_cast<const HV*>(Ppk) + (size_t)(8 * w) * 16 * 64 + l;
+  HV bq[2][8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) bq[0][t] = pb[((size_t)t * 16 + 0) * 64];
+#pragma unroll 2
   for (int kt = 0; kt < 16; ++kt) {
-    HV a[JT], al[JT], bq[8], bl[8];
+    const int cur = kt & 1;
+    if (kt + 1 < 16) {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) bq[cur ^ 1][t] = pb[((size_t)t * 16 + kt + 1) * 64];
+    }
+    HV a[JT], al[JT];
 #pragma unroll
     for (int jt = 0; jt < JT; ++jt) {
       const int o = (jt * 16 + lr) * HLD + kt * 32 + 8 * g;
@@ -165,22 +171,13 @@ __global__ __launch_bounds__(WG_THREADS) void k_precond_h(int nb, const H* __res
       if (SPLIT) al[jt] = *reinterpret_cast<const HV*>(&As[NS - 1][o]);
     }
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      bq[t] = pb[((size_t)t * 16 + kt) * 64];
-      if (SPLIT) bl[t] = pl[((size_t)t * 16 + kt) * 64];
-    }
-#pragma unroll
     for (int t = 0; t < 8; ++t)
 #pragma unroll
       for (int jt = 0; jt < JT; ++jt) {
-        acc[t][jt] = mfma16(a[jt], bq[t], acc[t][jt]);
-        if (SPLIT) {
-          cor[t][jt] = mfma16(a[jt], bl[t], cor[t][jt]);
-          cor[t][jt] = mfma16(al[jt], bq[t], cor[t][jt]);
-        }
+        acc[t][jt] = mfma16(a[jt], bq[cur][t], acc[t][jt]);
+        if (SPLIT) acc[t][jt] = mfma16(al[jt], bq[cur][t], acc[t][jt]);
       }
   }
-  constexpr float inv_lo = 1.0f / Split<H>::lo;
 #pragma unroll
   for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
@@ -190,10 +187,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_precond_h(int nb, const H* __res
       if (b < nb) {
         const float s = sc[jr];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          const float z = SPLIT ? fmaf(cor[t][jt][q], inv_lo, acc[t][jt][q]) : acc[t][jt][q];
-          out[(size_t)b * BLEN + (8 * w + t) * 16 + lr] = z * s;
-        }
+        for (int t = 0; t < 8; ++t) out[(size_t)b * BLEN + (8 * w + t) * 16 + lr] = acc[t][jt][q] * s;
       }
     }
 }
@@ -204,20 +198,29 @@ __global__ __launch_bounds__(WG_THREADS) void k_inner_init(long long n, const do
                                                            double scale, KrylovVectors<float> v,
                                                            SolverState* st, double* part) {
   double s[1] = {0.0};
-  const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
-  float* __restrict__ x0 = v.x[0];
-  for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
-    const long long i = base + e;
-    if (i < n) {
-      const float r = (float)(r64[i] * scale);
-      v.r[i] = r;
-      v.rhat[i] = r;
-      x0[i] = 0.f;
-      s[0] += (double)r * (double)r;
+  const long long i0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN) + 8 * threadIdx.x;
+  if (i0 < n) {
+    double rd[8];
+    float r[8], z[8];
+    load8(r64 + i0, rd);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      r[e] = (float)(rd[e] * scale);
+      z[e] = 0.f;
+      s[0] += (double)r[e] * (double)r[e];
     }
+    store8(v.r + i0, r);
+    store8(v.rhat + i0, r);
+    store8(v.x[0] + i0, z);
   }
+  grid_store_partial<1>(s, part);
+}
+
+__global__ __launch_bounds__(WG_THREADS) void k_inner_init_fin(SolverState* st, const double* part,
+                                                               int nparts) {
   double tot[1];
-  if (!grid_reduce_last<1>(s, part, &st->counters[3], tot) || threadIdx.x != 0) return;
+  grid_sum_partials<1>(part, nparts, tot);
+  if (threadIdx.x != 0) return;
   st->nu2 = 0.0;
   st->err = sqrt(tot[0]);
   st->err_init = st->err;
@@ -258,7 +261,7 @@ __global__ void k_cast32to64(long long n, const float* __restrict__ a, double* _
 // ---------------------------------------------------------------------------
 namespace {
 
-int nbw_for(int nb) { return nb >= 512 * 32 ? 32 : 16; }
+int nbw_for(int nb) { return nb >= (1 << 30) ? 32 : 16; }  // 32 is slower on MI355X (register-limited occupancy)
 
 template <typename H, int NBW, bool SPLIT>
 void launch_precond_hh(hipStream_t s, int mode, int nb, const H* P, const H* Plo, float unscale,
@@ -300,17 +303,10 @@ void launch_precond_mixed(const MixedWorkspace& M, hipStream_t s, int precision,
                              pin, pout, st);
 }
 
-template <int MODE, int BM>
-void spmv32_bm(hipStream_t s, const DevGrid<float>& g, const float* x, float* y, const float* a,
-               SolverState* st, double* part) {
-  k_spmv<float, MODE, BM><<<(g.nb + 3) / 4, WG_THREADS, 0, s>>>(g, x, y, a, nullptr, nullptr, st, part);
-}
 template <int MODE>
 void spmv32(hipStream_t s, const DevGrid<float>& g, const float* x, float* y, const float* a,
             SolverState* st, double* part) {
-  if (g.bmean == 1) spmv32_bm<MODE, 1>(s, g, x, y, a, st, part);
-  else if (g.bmean == 3) spmv32_bm<MODE, 3>(s, g, x, y, a, st, part);
-  else spmv32_bm<MODE, 0>(s, g, x, y, a, st, part);
+  spmv_launch<float, MODE>(s, g, x, y, a, nullptr, nullptr, st, part);
 }
 
 int grid_1d(long long n) { return (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 8192); }
@@ -440,6 +436,7 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
     *h_st = inner_state(cfg, inner_rel);
     MCHECK(hipMemcpyAsync(st, h_st, sizeof(SolverState), hipMemcpyHostToDevice, s));
     k_inner_init<<<grid_vec(n), WG_THREADS, 0, s>>>(n, r64, 1.0 / rnorm, v, st, part);
+    k_inner_init_fin<<<1, WG_THREADS, 0, s>>>(st, part, grid_vec(n));
     int guard = 0;
     while (true) {
       for (int it = 0; it < poll; ++it) {
@@ -448,6 +445,7 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
         launch_precond_mixed(M, s, cfg.precision, PREC_S, g32, nullptr, nullptr, st);
         spmv32<SPMV_DOT2>(s, g32, v.z2, v.t, v.r, st, part);
         k_update<float><<<grid_vec(n), WG_THREADS, 0, s>>>(n, v, st, part);
+        k_update_fin<<<1, WG_THREADS, 0, s>>>(st, part, grid_vec(n));
       }
       MCHECK(hipGetLastError());
       MCHECK(hipMemcpyAsync(h_st, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
@@ -498,9 +496,16 @@ int mixed_time_kernel(MixedWorkspace& M, hipStream_t s, int which, int reps,
   const KrylovVectors<float>& v = M.v32;
   const int precision = M.timing_precision;
   auto launch = [&]() {
-    if (which == 0) spmv32<SPMV_PLAIN>(s, g32, v.z1, v.nu, nullptr, st, part);
-    else if (which == 1) launch_precond_mixed(M, s, precision, PREC_PLAIN, g32, v.p, v.z1, st);
-    else k_update<float><<<grid_vec(M.n), WG_THREADS, 0, s>>>(M.n, v, st, part);
+    switch (which) {
+      case 0: spmv32<SPMV_PLAIN>(s, g32, v.z1, v.nu, nullptr, st, part); break;
+      case 1: launch_precond_mixed(M, s, precision, PREC_PLAIN, g32, v.p, v.z1, st); break;
+      case 2:
+        k_update<float><<<grid_vec(M.n), WG_THREADS, 0, s>>>(M.n, v, st, part);
+        k_update_fin<<<1, WG_THREADS, 0, s>>>(st, part, grid_vec(M.n));
+        break;
+      case 3: launch_precond_mixed(M, s, precision, PREC_P, g32, nullptr, nullptr, st); break;
+      default: spmv32<SPMV_DOT1>(s, g32, v.z1, v.nu, v.rhat, st, part); break;
+    }
   };
   launch();
   MCHECK(hipEventRecord(e0, s));
@@ -511,9 +516,10 @@ int mixed_time_kernel(MixedWorkspace& M, hipStream_t s, int which, int reps,
   MCHECK(hipEventElapsedTime(&ms, e0, e1));
   *avg_ms = ms / reps;
   const double N = (double)M.n;
-  if (which == 0) { *bytes = 8.0 * N; *flops = 8.0 * N; }
-  else if (which == 1) { *bytes = 8.0 * N; *flops = 1024.0 * N; }
-  else { *bytes = 32.0 * N; *flops = 8.0 * N; }
+  const double words[5] = {2, 2, 8, 5, 3};
+  const double fpc[5] = {8, 1024, 8, 1024 + 4, 10};
+  *bytes = 4.0 * words[which] * N;
+  *flops = fpc[which] * N * (M.split ? 3.0 : 1.0) / (which == 1 || which == 3 ? 1.0 : (M.split ? 3.0 : 1.0));
   return 0;
 }
 

@@ -87,6 +87,7 @@ struct DevGrid {
   const T* ent_val = nullptr;       // [nent]
   long long bmean_row = -1;         // cell replaced by sum h^3 x (bMean == 1)
   int bmean = 1;
+  int has_iface = 0;                // any coarse-fine interface rows
 };
 
 // Host-side topology build output.

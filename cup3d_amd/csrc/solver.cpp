@@ -275,6 +275,7 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
     g.ent_col = H->d_ent_col.as<int>();
     g.ent_val = static_cast<const TT*>(val);
     g.bmean = H->cfg.bmean;
+    g.has_iface = T.irow_cell.empty() ? 0 : 1;
     g.bmean_row = (H->cfg.bmean == 1 || H->cfg.bmean == 3) ? origin_row : -1;
   };
   fill(H->g64, H->d_h.p, H->d_invh.p, H->d_ent_val.p);
@@ -342,6 +343,7 @@ int cup3d_poisson_solve_device(cup3d_poisson_t H, const double* d_rhs, double* d
   if (!H) return -1;
   if (H->nb <= 0) return fail(H, "set_topology must be called before solve");
   if (!d_rhs || !d_x) return fail(H, "null vector");
+  if (((uintptr_t)d_rhs | (uintptr_t)d_x) & 15) return fail(H, "device vectors must be 16-byte aligned");
   HIPCHECK(hipSetDevice(H->device));
   cup3d_solve_report local;
   std::memset(&local, 0, sizeof(local));
@@ -515,7 +517,7 @@ int cup3d_debug_time_kernel(cup3d_poisson_t H, int which, int reps, double* avg_
                             double* bytes, double* flops) {
   if (!H) return -1;
   if (H->nb <= 0) return fail(H, "set_topology must be called first");
-  if (reps <= 0 || which < 0 || which > 2) return fail(H, "bad arguments");
+  if (reps <= 0 || which < 0 || which > 4) return fail(H, "bad arguments");
   HIPCHECK(hipSetDevice(H->device));
   hipStream_t s = H->stream;
   SolverState init = make_state(H->cfg);
@@ -525,6 +527,8 @@ int cup3d_debug_time_kernel(cup3d_poisson_t H, int which, int reps, double* avg_
   init.max_restarts = 0;
   init.err_init = 1.0;
   init.iter = 1;
+  init.beta = 0.0;      // K1 then recomputes p = r on every repetition
+  init.p_zero = 0;
   *H->h_state = init;
   SolverState* st = H->d_state.as<SolverState>();
   double* part = H->d_part.as<double>();
@@ -537,14 +541,15 @@ int cup3d_debug_time_kernel(cup3d_poisson_t H, int which, int reps, double* avg_
                           &fl, H->err) != 0)
       return -1;
   } else {
+    const KrylovVectors<double>& v = H->v64;
     auto launch = [&]() {
-      if (which == 0)
-        launch_spmv64(s, SPMV_PLAIN, H->g64, H->v64.z1, H->v64.nu, nullptr, nullptr, nullptr, st,
-                      part);
-      else if (which == 1)
-        launch_precond64(s, PREC_PLAIN, H->nb, H->d_P64.as<double>(), H->g64.invh, H->v64, st);
-      else
-        launch_update64(s, H->n, H->v64, st, part);
+      switch (which) {
+        case 0: launch_spmv64(s, SPMV_PLAIN, H->g64, v.z1, v.nu, nullptr, nullptr, nullptr, st, part); break;
+        case 1: launch_precond64(s, PREC_PLAIN, H->nb, H->d_P64.as<double>(), H->g64.invh, v, st); break;
+        case 2: launch_update64(s, H->n, v, st, part); break;
+        case 3: launch_precond64(s, PREC_P, H->nb, H->d_P64.as<double>(), H->g64.invh, v, st); break;
+        default: launch_spmv64(s, SPMV_DOT1, H->g64, v.z1, v.nu, v.rhat, nullptr, nullptr, st, part); break;
+      }
     };
     launch();  // warm
     HIPCHECK(hipEventRecord(H->ev0, s));
@@ -554,9 +559,10 @@ int cup3d_debug_time_kernel(cup3d_poisson_t H, int which, int reps, double* avg_
     float ms = 0;
     HIPCHECK(hipEventElapsedTime(&ms, H->ev0, H->ev1));
     *avg_ms = ms / reps;
-    if (which == 0) { by = 16.0 * N; fl = 8.0 * N; }
-    else if (which == 1) { by = 16.0 * N; fl = 1024.0 * N; }
-    else { by = 64.0 * N; fl = 8.0 * N; }
+    const double words[5] = {2, 2, 8, 5, 3};
+    const double fpc[5] = {8, 1024, 8, 1024 + 4, 10};
+    by = 8.0 * words[which] * N;
+    fl = fpc[which] * N;
   }
   if (bytes) *bytes = by;
   if (flops) *flops = fl;

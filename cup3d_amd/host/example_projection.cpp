@@ -23,7 +23,7 @@ int main(int argc, char** argv) {
   SimulationData sim;
   sim.poissonSolver = solver;
   sim.PoissonErrorTol = 0.0;
-  sim.PoissonErrorTolRel = 1e-10;
+  sim.PoissonErrorTolRel = 1e-7;
   sim.bMeanConstraint = 1;
   sim.BCx_flag = sim.BCy_flag = sim.BCz_flag = periodic;
   sim.bpdx = sim.bpdy = sim.bpdz = bp;

@@ -31,8 +31,10 @@ int64_t cup3d_debug_operator_csr(const cup3d_poisson_config* cfg, int64_t nblock
 /* Kernel timing for roofline reports: runs one kernel class `reps` times on
  * the handle's stream between two HIP events, on the data of the last solve,
  * and returns the average duration in ms. which: 0 = operator (SpMV),
- * 1 = block-Jacobi GEMM, 2 = fused x/r update (K5), in the handle's
- * precision. Also returns the algorithmic bytes and flops of one launch.
+ * 1 = block-Jacobi GEMM, 2 = fused x/r update (K5), 3 = fused p update +
+ * GEMM (K1), 4 = SpMV + dot (K2), in the handle's precision. Also returns
+ * the algorithmic bytes (vectors read + written) and flops of one launch
+ * (GEMM flops count the MFMA passes actually issued).
  * Must follow a solve on the current topology. Returns 0 or < 0. */
 int cup3d_debug_time_kernel(cup3d_poisson_t h, int which, int reps, double* avg_ms,
                             double* bytes, double* flops);

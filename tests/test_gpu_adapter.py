@@ -13,8 +13,12 @@ from cup3d_amd._lib import lib_path
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("solver,err", [("hip_iterative", 1e-8), ("cuda_iterative", 1e-8),
-                                         ("hip_iterative_mixed", 1e-7)])
+# A relative residual of 1e-7 bounds the relative error by kappa * 1e-7, with
+# kappa = 12 / (4 sin^2(pi h)) ~ 310 for the 32^3 periodic Laplacian: 3e-5 in
+# L2. The FP64 recurrence lands well inside that; the refined mixed solve
+# leaves a rougher residual and gets closer to the bound (max norm).
+@pytest.mark.parametrize("solver,err", [("hip_iterative", 1e-5), ("cuda_iterative", 1e-5),
+                                         ("hip_iterative_mixed", 1e-4)])
 def test_adapter_solves_fourier_mode(gpu, solver, err):
     exe = lib_path("example_projection")
     out = subprocess.run([exe, solver, "4"], capture_output=True, text=True, timeout=120)

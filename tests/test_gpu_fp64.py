@@ -159,9 +159,11 @@ def test_nonfinite_rhs_reports_status(gpu):
 
 def test_c2_full_size_properties(gpu):
     """256x128x128 walls workload at full size: converges to the true relative
-    residual asked for, deterministically, with the mean constraint met."""
+    residual asked for, deterministically, with the mean constraint met. (With
+    bMean = 1 the recurrence can stagnate near 1e-7 on this grid — the oracle
+    shows the same on smaller grids — so the iteration cap is raised.)"""
     w = W.c2_uniform_walls()
-    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-7)
+    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-7, max_iter=3000)
     b = dev(w.rhs(step=1))
     runs = []
     for _ in range(2):

@@ -27,11 +27,12 @@ def compatible_rhs(blocks, h0, seed):
 
 
 @pytest.mark.parametrize("prec,passes,tol", [("mixed_fp16", 1, 2e-3), ("mixed_bf16", 1, 1.2e-2),
-                                             ("mixed_fp16", 3, 2e-6), ("mixed_bf16", 3, 1e-4)])
+                                             ("mixed_fp16", 3, 1e-3), ("mixed_bf16", 3, 8e-3)])
 @pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "amr3_walls", "amr2_periodic_offcentre"])
 def test_half_preconditioner_close_to_fp64(gpu, name, prec, passes, tol):
-    """One FP16 pass: ~5e-4 relative (BF16 ~4e-3); split hi/lo, three passes:
-    FP32-level. Per block, relative L2."""
+    """Distance to the FP64 preconditioner, per block, relative L2: set by
+    the one-time rounding of P_inv (FP16 2^-12, BF16 2^-9 per entry) and, for
+    one pass, by the rounding of the input."""
     blocks, bpd, h0, bc = CASES[name]
     S = PoissonSolver(bpd, h0, bc, precision=prec, gemm_passes=passes).set_topology(blocks)
     O = pyoracle.Oracle(blocks, bpd, h0, bc, 1)
@@ -94,3 +95,23 @@ def test_mixed_deterministic(gpu):
         S.solve_device(b, x)
         out.append(x.clone())
     assert torch.equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("prec", ["mixed_fp16", "mixed_bf16"])
+def test_split_preconditioner_is_linear(gpu, prec):
+    """M(u + v) = M(u) + M(v) to the precision of the hi + lo split of the
+    input; a single half-precision pass is visibly nonlinear (that is what
+    stalls BiCGSTAB)."""
+    blocks, bpd, h0, bc = CASES["amr3_walls"]
+    n = len(blocks) * 512
+    u = dev(random_field(len(blocks), 5))
+    v = dev(random_field(len(blocks), 6) * 0.37)
+    res = {}
+    for passes in (3, 1):
+        S = PoissonSolver(bpd, h0, bc, precision=prec, gemm_passes=passes).set_topology(blocks)
+        z = lambda p: S.apply_preconditioner(p, torch.zeros(n, dtype=torch.float64, device=gpu))
+        zuv, zu, zv = z(u + v), z(u), z(v)
+        res[passes] = float(torch.linalg.norm(zuv - zu - zv) / torch.linalg.norm(zuv))
+    # hi + lo keeps 22 significant bits in FP16, 16 in BF16
+    assert res[3] < {"mixed_fp16": 3e-6, "mixed_bf16": 2e-5}[prec], res
+    assert res[1] > 10 * res[3], res
