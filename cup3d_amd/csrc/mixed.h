@@ -15,9 +15,7 @@ namespace cup3d {
 struct MixedWorkspace {
   void* d_P16 = nullptr;   // FP16 P_inv, scaled, fragment-packed
   void* d_Pbf = nullptr;   // BF16 P_inv, fragment-packed
-  void* d_P16lo = nullptr; // low parts of the split operands
-  void* d_Pbflo = nullptr;
-  bool split = true;       // hi/lo three-pass GEMM (false: one pass)
+  bool split = true;       // hi/lo split input, two MFMA passes (false: one pass)
   double p16_scale = 1.0;  // P16 = P * p16_scale
   void* d_vec32 = nullptr; // FP32 Krylov vectors
   void* d_res64 = nullptr; // FP64 refinement residual

@@ -70,29 +70,32 @@ __device__ __forceinline__ int acc_row(int g, int q) { return 4 * g + q; }
 
 constexpr int HLD = BLEN + 8;  // LDS row stride in halfs (1040 bytes)
 constexpr float P16_SCALE = 16384.0f;
+constexpr int GW = 8;          // waves per GEMM workgroup
+constexpr int GT = GW * 64;    // threads per GEMM workgroup
 
-template <typename H>
-struct Split;
-template <>
-struct Split<_Float16> {
-  static constexpr float lo = 2048.0f;  // 2^11
-};
-template <>
-struct Split<__bf16> {
-  static constexpr float lo = 256.0f;  // 2^8
-};
-
+// One workgroup applies M^-1 to NBW consecutive blocks.
+//  phase 1: wave w owns blocks [w NBW/8, (w+1) NBW/8); every load of the
+//           fused vector update is issued before the first one is used
+//           (float4, contiguous), then the per-block power-of-two scale and
+//           the hi/lo cast into LDS.
+//  phase 2: wave w owns output cells [64 w, 64 w + 64) of all NBW blocks:
+//           16 k-steps of 32 cells, each 4 (cell tiles) x NBW/16 (block
+//           tiles) x (1 or 2 passes) MFMAs; the P_inv fragments of the next
+//           k-step are prefetched from L2 while the current ones are used.
+//  phase 3: FP32 epilogue, scaled by h_b^-1 and the two scales.
 template <int MODE, typename H, int NBW, bool SPLIT>
-__global__ __launch_bounds__(WG_THREADS) void k_precond_h(int nb, const H* __restrict__ Ppk,
-                                                          const H* __restrict__ Ppk_lo,
-                                                          float p_unscale,
-                                                          const float* __restrict__ invh,
-                                                          KrylovVectors<float> v,
-                                                          const float* __restrict__ pin,
-                                                          float* __restrict__ pout,
-                                                          SolverState* st) {
+__global__ __launch_bounds__(GT) void k_precond_h(int nb, const H* __restrict__ Ppk,
+                                                  float p_unscale,
+                                                  const float* __restrict__ invh,
+                                                  KrylovVectors<float> v,
+                                                  const float* __restrict__ pin,
+                                                  float* __restrict__ pout, SolverState* st) {
   typedef typename HalfVec<H>::type HV;
+  typedef H H4 __attribute__((ext_vector_type(4)));
   constexpr int NS = SPLIT ? 2 : 1;
+  constexpr int NPB = NBW / GW;  // blocks per wave in phase 1
+  constexpr int F4 = 2 * NPB;    // float4 per lane per vector (a block is 2 x 64 float4)
+  static_assert(NBW % GW == 0 && NBW % 16 == 0, "block group must fill the waves and tiles");
   __shared__ __attribute__((aligned(16))) H As[NS][NBW * HLD];
   __shared__ float sc[NBW];
   if (MODE != PREC_PLAIN && st->status != ST_RUNNING) return;
@@ -104,64 +107,105 @@ __global__ __launch_bounds__(WG_THREADS) void k_precond_h(int nb, const H* __res
   const int restart = MODE == PREC_P ? st->restart_pending : 0;
   float* __restrict__ out = MODE == PREC_PLAIN ? pout : (MODE == PREC_S ? v.z2 : v.z1);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  // phase 1: vector update, per-block scaling, cast into LDS (one wave per block)
-  for (int j = w; j < NBW; j += 4) {
-    const int b = j0 + j;
-    float vals[BS];
-    float amax = 0.f;
+
+  // phase 1. Element u of a lane: block jw + u/2, cells 256 (u%2) + 4 l .. +3.
+  const int jw = w * NPB;
+  float4 ra[F4], rb[F4], rc[F4];
 #pragma unroll
-    for (int e = 0; e < BS; ++e) {
-      float val = 0.f;
-      if (b < nb) {
-        const size_t i = (size_t)b * BLEN + e * 64 + l;
-        if (MODE == PREC_P) {
-          const float r = v.r[i];
-          val = pzero ? r : fmaf(beta, fmaf(-omega, v.nu[i], v.p[i]), r);
-          v.p[i] = val;
-          if (restart) v.rhat[i] = r;
-        } else if (MODE == PREC_S) {
-          val = fmaf(-alpha, v.nu[i], v.r[i]);
-          v.r[i] = val;
-        } else {
-          val = pin[i];
-        }
+  for (int u = 0; u < F4; ++u) {
+    const int b = j0 + jw + u / 2;
+    const size_t i4 = ((size_t)b * BLEN + 256 * (u & 1) + 4 * l) / 4;
+    if (b < nb) {
+      if (MODE == PREC_P) {
+        ra[u] = reinterpret_cast<const float4*>(v.r)[i4];
+        rb[u] = reinterpret_cast<const float4*>(v.nu)[i4];
+        rc[u] = reinterpret_cast<const float4*>(v.p)[i4];
+      } else if (MODE == PREC_S) {
+        ra[u] = reinterpret_cast<const float4*>(v.r)[i4];
+        rb[u] = reinterpret_cast<const float4*>(v.nu)[i4];
+      } else {
+        ra[u] = reinterpret_cast<const float4*>(pin)[i4];
       }
-      vals[e] = val;
-      amax = fmaxf(amax, fabsf(val));
+    } else {
+      ra[u] = rb[u] = rc[u] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
+  }
+  float amax[NPB];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
+  for (int jj = 0; jj < NPB; ++jj) amax[jj] = 0.f;
+#pragma unroll
+  for (int u = 0; u < F4; ++u) {
+    const int b = j0 + jw + u / 2;
+    const size_t i4 = ((size_t)b * BLEN + 256 * (u & 1) + 4 * l) / 4;
+    float4 val = ra[u];
+    if (MODE == PREC_P) {
+      if (!pzero) {
+        val.x = fmaf(beta, fmaf(-omega, rb[u].x, rc[u].x), ra[u].x);
+        val.y = fmaf(beta, fmaf(-omega, rb[u].y, rc[u].y), ra[u].y);
+        val.z = fmaf(beta, fmaf(-omega, rb[u].z, rc[u].z), ra[u].z);
+        val.w = fmaf(beta, fmaf(-omega, rb[u].w, rc[u].w), ra[u].w);
+      }
+      if (b < nb) {
+        reinterpret_cast<float4*>(v.p)[i4] = val;
+        if (restart) reinterpret_cast<float4*>(v.rhat)[i4] = ra[u];
+      }
+    } else if (MODE == PREC_S) {
+      val.x = fmaf(-alpha, rb[u].x, ra[u].x);
+      val.y = fmaf(-alpha, rb[u].y, ra[u].y);
+      val.z = fmaf(-alpha, rb[u].z, ra[u].z);
+      val.w = fmaf(-alpha, rb[u].w, ra[u].w);
+      if (b < nb) reinterpret_cast<float4*>(v.r)[i4] = val;
+    }
+    ra[u] = val;
+    amax[u / 2] = fmaxf(amax[u / 2], fmaxf(fmaxf(fabsf(val.x), fabsf(val.y)),
+                                           fmaxf(fabsf(val.z), fabsf(val.w))));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int jj = 0; jj < NPB; ++jj) amax[jj] = fmaxf(amax[jj], __shfl_xor(amax[jj], o, 64));
+#pragma unroll
+  for (int u = 0; u < F4; ++u) {
+    const int jj = u / 2;
     int ex = 0;
-    if (amax > 0.f) frexpf(amax, &ex);
-    const float s = amax > 0.f ? ldexpf(1.0f, -ex) : 1.0f;
+    if (amax[jj] > 0.f) frexpf(amax[jj], &ex);
+    const float s = amax[jj] > 0.f ? ldexpf(1.0f, -ex) : 1.0f;
+    const float x[4] = {ra[u].x * s, ra[u].y * s, ra[u].z * s, ra[u].w * s};
+    H4 hi, lo;
 #pragma unroll
-    for (int e = 0; e < BS; ++e) {
-      const float x = vals[e] * s;
-      const H hi = (H)x;
-      As[0][j * HLD + e * 64 + l] = hi;
-      if (SPLIT) As[NS - 1][j * HLD + e * 64 + l] = (H)(x - (float)hi);
+    for (int c = 0; c < 4; ++c) {
+      hi[c] = (H)x[c];
+      lo[c] = (H)(x[c] - (float)hi[c]);
     }
-    if (l == 0) sc[j] = b < nb ? invh[b] * p_unscale / s : 0.f;
+    const int o = (jw + jj) * HLD + 256 * (u & 1) + 4 * l;
+    *reinterpret_cast<H4*>(&As[0][o]) = hi;
+    if (SPLIT) *reinterpret_cast<H4*>(&As[NS - 1][o]) = lo;
+    if ((u & 1) == 0 && l == 0) {
+      const int b = j0 + jw + jj;
+      sc[jw + jj] = b < nb ? invh[b] * p_unscale / s : 0.f;
+    }
   }
   __syncthreads();
-  // phase 2: wave w owns output cells [128 w, 128 w + 128) for all NBW blocks
+
+  // phase 2
   constexpr int JT = NBW / 16;
   const int lr = l & 15, g = l >> 4;
-  float4_t acc[8][JT];
+  float4_t acc[4][JT];
 #pragma unroll
-  for (int t = 0; t < 8; ++t)
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
     for (int jt = 0; jt < JT; ++jt) acc[t][jt] = (float4_t){0.f, 0.f, 0.f, 0.f};
-  const HV* __restrict__ pb = reinterpret_cast<const HV*>(Ppk) + (size_t)(8 * w) * 16 * 64 + l;
-  HV bq[2][8];
+  // fragment (it, kt) of the packed P_inv: 64 lanes x 8 values at ((it 16 + kt) 64 + l) 8
+  const HV* __restrict__ pb = reinterpret_cast<const HV*>(Ppk) + (size_t)(4 * w) * 16 * 64 + l;
+  HV bq[2][4];
 #pragma unroll
-  for (int t = 0; t < 8; ++t) bq[0][t] = pb[((size_t)t * 16 + 0) * 64];
+  for (int t = 0; t < 4; ++t) bq[0][t] = pb[(size_t)t * 16 * 64];
 #pragma unroll 2
   for (int kt = 0; kt < 16; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < 16) {
 #pragma unroll
-      for (int t = 0; t < 8; ++t) bq[cur ^ 1][t] = pb[((size_t)t * 16 + kt + 1) * 64];
+      for (int t = 0; t < 4; ++t) bq[cur ^ 1][t] = pb[((size_t)t * 16 + kt + 1) * 64];
     }
     HV a[JT], al[JT];
 #pragma unroll
@@ -171,13 +215,15 @@ __global__ __launch_bounds__(WG_THREADS) void k_precond_h(int nb, const H* __res
       if (SPLIT) al[jt] = *reinterpret_cast<const HV*>(&As[NS - 1][o]);
     }
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int jt = 0; jt < JT; ++jt) {
         acc[t][jt] = mfma16(a[jt], bq[cur][t], acc[t][jt]);
         if (SPLIT) acc[t][jt] = mfma16(al[jt], bq[cur][t], acc[t][jt]);
       }
   }
+
+  // phase 3
 #pragma unroll
   for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
@@ -187,7 +233,8 @@ __global__ __launch_bounds__(WG_THREADS) void k_precond_h(int nb, const H* __res
       if (b < nb) {
         const float s = sc[jr];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) out[(size_t)b * BLEN + (8 * w + t) * 16 + lr] = acc[t][jt][q] * s;
+        for (int t = 0; t < 4; ++t)
+          out[(size_t)b * BLEN + (4 * w + t) * 16 + lr] = acc[t][jt][q] * s;
       }
     }
 }
@@ -261,46 +308,44 @@ __global__ void k_cast32to64(long long n, const float* __restrict__ a, double* _
 // ---------------------------------------------------------------------------
 namespace {
 
-int nbw_for(int nb) { return nb >= (1 << 30) ? 32 : 16; }  // 32 is slower on MI355X (register-limited occupancy)
+#ifndef CUP3D_GEMM_NBW
+#define CUP3D_GEMM_NBW 32
+#endif
+// Blocks per GEMM workgroup: P_inv (512 KiB in FP16) is streamed from L2 once
+// per workgroup, so wider groups cut that traffic; 32 gives one workgroup per
+// CU at 8192 blocks.
+constexpr int GEMM_NBW = CUP3D_GEMM_NBW;
 
-template <typename H, int NBW, bool SPLIT>
-void launch_precond_hh(hipStream_t s, int mode, int nb, const H* P, const H* Plo, float unscale,
+template <typename H, bool SPLIT>
+void launch_precond_hh(hipStream_t s, int mode, int nb, const H* P, float unscale,
                        const float* invh, const KrylovVectors<float>& v, const float* pin,
                        float* pout, SolverState* st) {
-  const int nwg = (nb + NBW - 1) / NBW;
+  const int nwg = (nb + GEMM_NBW - 1) / GEMM_NBW;
   if (mode == PREC_P)
-    k_precond_h<PREC_P, H, NBW, SPLIT><<<nwg, WG_THREADS, 0, s>>>(nb, P, Plo, unscale, invh, v, pin, pout, st);
+    k_precond_h<PREC_P, H, GEMM_NBW, SPLIT><<<nwg, GT, 0, s>>>(nb, P, unscale, invh, v, pin, pout, st);
   else if (mode == PREC_S)
-    k_precond_h<PREC_S, H, NBW, SPLIT><<<nwg, WG_THREADS, 0, s>>>(nb, P, Plo, unscale, invh, v, pin, pout, st);
+    k_precond_h<PREC_S, H, GEMM_NBW, SPLIT><<<nwg, GT, 0, s>>>(nb, P, unscale, invh, v, pin, pout, st);
   else
-    k_precond_h<PREC_PLAIN, H, NBW, SPLIT><<<nwg, WG_THREADS, 0, s>>>(nb, P, Plo, unscale, invh, v, pin, pout, st);
+    k_precond_h<PREC_PLAIN, H, GEMM_NBW, SPLIT><<<nwg, GT, 0, s>>>(nb, P, unscale, invh, v, pin, pout, st);
 }
 
 template <typename H>
-void launch_precond_h(hipStream_t s, int mode, int nb, const void* Ppk, const void* Plo,
-                      float unscale, bool split, const float* invh,
-                      const KrylovVectors<float>& v, const float* pin, float* pout,
-                      SolverState* st) {
+void launch_precond_h(hipStream_t s, int mode, int nb, const void* Ppk, float unscale,
+                      bool split, const float* invh, const KrylovVectors<float>& v,
+                      const float* pin, float* pout, SolverState* st) {
   const H* P = static_cast<const H*>(Ppk);
-  const H* L = static_cast<const H*>(Plo);
-  const bool wide = nbw_for(nb) == 32;
-  if (split) {
-    if (wide) launch_precond_hh<H, 32, true>(s, mode, nb, P, L, unscale, invh, v, pin, pout, st);
-    else launch_precond_hh<H, 16, true>(s, mode, nb, P, L, unscale, invh, v, pin, pout, st);
-  } else {
-    if (wide) launch_precond_hh<H, 32, false>(s, mode, nb, P, L, unscale, invh, v, pin, pout, st);
-    else launch_precond_hh<H, 16, false>(s, mode, nb, P, L, unscale, invh, v, pin, pout, st);
-  }
+  if (split) launch_precond_hh<H, true>(s, mode, nb, P, unscale, invh, v, pin, pout, st);
+  else launch_precond_hh<H, false>(s, mode, nb, P, unscale, invh, v, pin, pout, st);
 }
 
 void launch_precond_mixed(const MixedWorkspace& M, hipStream_t s, int precision, int mode,
                           const DevGrid<float>& g, const float* pin, float* pout, SolverState* st) {
   if (precision == CUP3D_PREC_MIXED_FP16)
-    launch_precond_h<_Float16>(s, mode, g.nb, M.d_P16, M.d_P16lo, 1.0f / P16_SCALE, M.split,
-                               g.invh, M.v32, pin, pout, st);
+    launch_precond_h<_Float16>(s, mode, g.nb, M.d_P16, 1.0f / P16_SCALE, M.split, g.invh, M.v32,
+                               pin, pout, st);
   else
-    launch_precond_h<__bf16>(s, mode, g.nb, M.d_Pbf, M.d_Pbflo, 1.0f, M.split, g.invh, M.v32,
-                             pin, pout, st);
+    launch_precond_h<__bf16>(s, mode, g.nb, M.d_Pbf, 1.0f, M.split, g.invh, M.v32, pin, pout,
+                             st);
 }
 
 template <int MODE>
@@ -341,8 +386,8 @@ SolverState inner_state(const cup3d_poisson_config& c, double tol_rel) {
 
 int mixed_init_pinv(MixedWorkspace& M, const std::vector<double>& P, std::string& err) {
   // fragment (it, kt): lane l, element e = P[32 kt + 8 (l/16) + e][16 it + l%16]
-  std::vector<_Float16> p16((size_t)BLEN * BLEN), p16lo((size_t)BLEN * BLEN);
-  std::vector<__bf16> pbf((size_t)BLEN * BLEN), pbflo((size_t)BLEN * BLEN);
+  std::vector<_Float16> p16((size_t)BLEN * BLEN);
+  std::vector<__bf16> pbf((size_t)BLEN * BLEN);
   for (int it = 0; it < 32; ++it)
     for (int kt = 0; kt < 16; ++kt)
       for (int l = 0; l < 64; ++l)
@@ -351,19 +396,13 @@ int mixed_init_pinv(MixedWorkspace& M, const std::vector<double>& P, std::string
           const size_t o = (((size_t)it * 16 + kt) * 64 + l) * 8 + e;
           const float vs = (float)(v * P16_SCALE);
           p16[o] = (_Float16)vs;
-          p16lo[o] = (_Float16)((vs - (float)p16[o]) * Split<_Float16>::lo);
           pbf[o] = (__bf16)(float)v;
-          pbflo[o] = (__bf16)(((float)v - (float)pbf[o]) * Split<__bf16>::lo);
         }
   M.p16_scale = P16_SCALE;
   MCHECK(hipMalloc(&M.d_P16, p16.size() * sizeof(_Float16)));
   MCHECK(hipMalloc(&M.d_Pbf, pbf.size() * sizeof(__bf16)));
   MCHECK(hipMemcpy(M.d_P16, p16.data(), p16.size() * sizeof(_Float16), hipMemcpyHostToDevice));
   MCHECK(hipMemcpy(M.d_Pbf, pbf.data(), pbf.size() * sizeof(__bf16), hipMemcpyHostToDevice));
-  MCHECK(hipMalloc(&M.d_P16lo, p16lo.size() * sizeof(_Float16)));
-  MCHECK(hipMalloc(&M.d_Pbflo, pbflo.size() * sizeof(__bf16)));
-  MCHECK(hipMemcpy(M.d_P16lo, p16lo.data(), p16lo.size() * sizeof(_Float16), hipMemcpyHostToDevice));
-  MCHECK(hipMemcpy(M.d_Pbflo, pbflo.data(), pbflo.size() * sizeof(__bf16), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -516,19 +555,19 @@ int mixed_time_kernel(MixedWorkspace& M, hipStream_t s, int which, int reps,
   MCHECK(hipEventElapsedTime(&ms, e0, e1));
   *avg_ms = ms / reps;
   const double N = (double)M.n;
+  // algorithmic FP32 words per cell and flops per cell; a GEMM pass is
+  // 2 x 512 flops per cell, two passes with the split input
   const double words[5] = {2, 2, 8, 5, 3};
-  const double fpc[5] = {8, 1024, 8, 1024 + 4, 10};
+  const double passes = M.split ? 2.0 : 1.0;
+  const double fpc[5] = {8, 1024 * passes, 8, 1024 * passes + 4, 10};
   *bytes = 4.0 * words[which] * N;
-  *flops = fpc[which] * N * (M.split ? 3.0 : 1.0) / (which == 1 || which == 3 ? 1.0 : (M.split ? 3.0 : 1.0));
+  *flops = fpc[which] * N;
   return 0;
 }
 
 void mixed_free(MixedWorkspace& M) {
   if (M.d_P16) (void)hipFree(M.d_P16);
   if (M.d_Pbf) (void)hipFree(M.d_Pbf);
-  if (M.d_P16lo) (void)hipFree(M.d_P16lo);
-  if (M.d_Pbflo) (void)hipFree(M.d_Pbflo);
-  M.d_P16lo = M.d_Pbflo = nullptr;
   if (M.d_vec32) (void)hipFree(M.d_vec32);
   if (M.d_res64) (void)hipFree(M.d_res64);
   M.d_P16 = M.d_Pbf = M.d_vec32 = M.d_res64 = nullptr;
