@@ -37,6 +37,16 @@ F16_MFMA_PEAK = 2500.0  # TFLOP/s, dense FP16/BF16 matrix
 KERNELS = {0: "spmv", 1: "precond_gemm", 2: "update_K5", 3: "precond_update_K1",
            4: "spmv_dot_K2"}
 CALLS_PER_ITER = {2: 1, 3: 2, 4: 2}  # K1 and K3 share the GEMM kernel, K2/K4 the SpMV
+COPY_BYTES = 256 << 20  # calibration copy for the FETCH_SIZE / WRITE_SIZE counters
+# HBM bytes per launch from two rocprofv3 counter passes (FETCH_SIZE x 2.000,
+# WRITE_SIZE x 1.000, calibrated on the 256 MiB copy of --kernels-only), kept
+# in profiles/mixed_fp16_uniform_256x128x128_traffic.md: (workload, form,
+# kernel) -> bytes.
+TRAFFIC = {
+    ("uniform_256x128x128_walls_blobs256_sphere", "mixed_fp16", "precond_update_K1"): 88.25e6,
+    ("uniform_256x128x128_walls_blobs256_sphere", "mixed_fp16", "spmv_dot_K2"): 53.10e6,
+    ("uniform_256x128x128_walls_blobs256_sphere", "mixed_fp16", "update_K5"): 134.3e6,
+}
 
 
 def parse():
@@ -45,16 +55,20 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--form", default="mixed_fp16", choices=["fp64", "mixed_fp16", "mixed_bf16"])
-    p.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    p.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"])
     p.add_argument("--bmean", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-iters", type=int, default=10)
+    p.add_argument("--kernels-only", action="store_true",
+                   help="launch the timed kernels a few times and a 256 MiB copy (for "
+                        "hardware-counter runs), print their names, and exit")
     return p.parse_args()
 
 
 def make_workload(name):
     from cup3d_amd import workloads as W
-    return {"c1": W.c1_periodic_32, "c2": W.c2_uniform_walls, "c3": W.c3_amr_sphere}[name]()
+    return {"c1": W.c1_periodic_32, "c2": W.c2_uniform_walls, "c3": W.c3_amr_sphere,
+            "c5": W.c5_amr_school}[name]()
 
 
 def time_form(w, form, bmean, rhs, p_prev, steps, warmup):
@@ -128,6 +142,17 @@ def main():
 
     dev = torch.device("cuda:0")
     w = make_workload(args.workload)
+    if args.kernels_only:
+        S = solver_for(w, precision=args.form, bmean=args.bmean)
+        k = kernel_times(S, [2, 3, 4], reps=5)
+        a = torch.empty(COPY_BYTES // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"workload": w.name, "form": args.form, "copy_bytes": COPY_BYTES,
+                          "kernels": {KERNELS[i]: v["bytes"] for i, v in k.items()}}))
+        return
     rhs_prev = torch.from_numpy(w.rhs(step=0).reshape(-1)).to(dev)
     rhs = torch.from_numpy(w.rhs(step=1).reshape(-1)).to(dev)
     # previous-step pressure and an FP64 reference solution (untimed setup)
@@ -154,7 +179,8 @@ def main():
     dom = max(kern, key=lambda k: kern[k]["avg_us"] * CALLS_PER_ITER[k])
     kd = kern[dom]
     roof = {"bound": "hbm", "achieved": round(kd["GB/s"], 1), "peak": HBM_PEAK, "unit": "GB/s",
-            "frac": round(kd["GB/s"] / HBM_PEAK, 4), "traffic": None,
+            "frac": round(kd["GB/s"] / HBM_PEAK, 4),
+            "traffic": TRAFFIC.get((w.name, args.form, KERNELS[dom])),
             "kernel": KERNELS[dom], "avg_us": round(kd["avg_us"], 2),
             "alg_bytes_per_launch": kd["bytes"]}
     k64 = kernel_times(S64, [1, 2, 3, 4])

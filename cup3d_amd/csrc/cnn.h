@@ -1,6 +1,7 @@
 // Learned warm start: a 3-layer 3-D CNN (2 -> 16 -> 16 -> 1 channels, 3^3
 // kernels, GELU) applied per 8^3 block with a one-cell halo, trained and run
-// by hand-written HIP kernels (cnn.hip).
+// by hand-written HIP kernels (cnn.hip). It predicts the pressure increment
+// of a time step from the residual of the previous pressure.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,18 +24,18 @@ constexpr int CNN_KT1 = 7, CNN_KT2 = 14, CNN_KT3 = 14;
 constexpr int CNN_PK = (CNN_KT1 + CNN_KT2 + CNN_KT3) * 64 * 8;
 
 struct CnnSample {
-  float* rhs_h = nullptr;  // rhs / h_b        [n]
-  float* prev = nullptr;   // p_prev           [n]
-  float* target = nullptr; // (p - p_prev) / s [n]
-  float s = 1.f;           // RMS(p_prev)
+  float* res_h = nullptr;   // (b - A p_prev) / h_b  [n]
+  float* prev = nullptr;    // p_prev               [n]
+  float* target = nullptr;  // (p - p_prev) / s_r   [n]
+  float s_r = 1.f;          // RMS((b - A p_prev) / h_b)
+  float s_p = 1.f;          // RMS(p_prev)
 };
 
 struct CnnState {
   float* d_w = nullptr;      // FP32 master weights [CNN_NW]
   float* d_adam = nullptr;   // m, v [2 * CNN_NW]
   void* d_wpk = nullptr;     // FP16 packed weights for inference
-  double* d_scal = nullptr;  // device scalars: [0] = s of the last predict
-  unsigned* d_counter = nullptr;
+  double* d_scal = nullptr;  // device scalars: [0] = s_r, [1] = s_p of the last predict
   double* d_part = nullptr;
   float* d_scratch = nullptr;  // training activations / gradients
   long long scratch_cap = 0;
@@ -55,9 +56,11 @@ int cnn_init(CnnState& C, std::string& err);
 int cnn_set_topology(CnnState& C, int nb, const int* d_nbr, const double* d_h, std::string& err);
 int cnn_set_weights(CnnState& C, const float* w, int nw, std::string& err);
 int cnn_get_weights(CnnState& C, float* w, int nw, std::string& err);
-int cnn_predict(CnnState& C, hipStream_t s, const double* d_rhs, const double* d_prev,
+// d_out = p_prev + s_r * net(res / (h s_r), p_prev / s_p), res = b - A p_prev.
+int cnn_predict(CnnState& C, hipStream_t s, const double* d_res, const double* d_prev,
                 double* d_out, std::string& err);
-int cnn_add_sample(CnnState& C, hipStream_t s, const double* d_rhs, const double* d_prev,
+// Training triple: the residual of p_prev, p_prev, and the solution p.
+int cnn_add_sample(CnnState& C, hipStream_t s, const double* d_res, const double* d_prev,
                    const double* d_p, std::string& err);
 void cnn_clear_samples(CnnState& C);
 int cnn_train(CnnState& C, hipStream_t s, int steps, int batch, float lr, uint64_t seed,
@@ -65,12 +68,6 @@ int cnn_train(CnnState& C, hipStream_t s, int steps, int batch, float lr, uint64
 int cnn_eval(CnnState& C, hipStream_t s, int k, double* loss, std::string& err);
 // Full-batch FP32 gradient of the loss over every block of sample k.
 int cnn_gradient(CnnState& C, hipStream_t s, int k, float* grad, double* loss, std::string& err);
-// Replace the guess in d_x by the net's prediction when its FP64 residual is
-// smaller; reports the guess used and its residual norm.
-int cnn_warm_start(CnnState& C, hipStream_t s, const DevGrid<double>& g,
-                   const KrylovVectors<double>& v, SolverState* st, SolverState* h_st,
-                   double* part, const double* d_rhs, double* d_x, int* used, double* resid,
-                   std::string& err);
 void cnn_free(CnnState& C);
 
 }  // namespace cup3d

@@ -1,16 +1,19 @@
 // Learned warm start: network, inference and training kernels.
 //
-// Network (per 8^3 block, one-cell halo):
-//   input  2 x 10^3      ch0 = rhs / (h_b s), ch1 = p_prev / s, s = RMS(p_prev)
+// Network (per 8^3 block, one-cell halo), in the increment form of the
+// pressure solve (PressureProjection.cpp:368-385 solves for p - p_old with
+// rhs b - A p_old):
+//   input  2 x 10^3      ch0 = res / (h_b s_r), ch1 = p_prev / s_p,
+//                        res = b - A p_prev, s_r = RMS(res / h_b), s_p = RMS(p_prev)
 //   conv 3^3 valid  2 -> 16, GELU
 //   conv 3^3 same  16 -> 16, GELU      (zero padding inside the block)
 //   conv 3^3 same  16 -> 1
-//   p_net = p_prev + s * out
-// rhs / h_b has the units of the 7-point stencil of p (b = h^3 lap p), so the
-// two channels are commensurate on every AMR level. Halo: a face halo cell
-// takes the adjacent cell of the same-level neighbour block; every other halo
-// cell (wall, coarse/fine face, edge, corner) repeats the nearest cell of the
-// block.
+//   p_net = p_prev + s_r * out
+// res / h_b has the units of the 7-point stencil applied to the increment
+// (A = h^3 lap), so the input is commensurate on every AMR level. Halo: a face
+// halo cell takes the adjacent cell of the same-level neighbour block; every
+// other halo cell (wall, coarse/fine face, edge, corner) repeats the nearest
+// cell of the block.
 //
 // Inference (k_cnn_infer): one workgroup per block, activations in LDS as
 // FP16 channel-last [10][10][10][16] with a zero ring, each layer an implicit
@@ -67,27 +70,30 @@ __device__ __forceinline__ float halo_value(const T* __restrict__ f, const int* 
 }
 
 // ---------------------------------------------------------------------------
-// s = RMS(p_prev) (deterministic two-pass reduction)
+// scal[slot] = RMS(f / h_b) (h = nullptr: RMS(f)); deterministic two-pass sum
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(WG_THREADS) void k_cnn_rms(long long n, const double* __restrict__ f,
-                                                        double* scal, unsigned* counter,
+                                                        const double* __restrict__ h,
                                                         double* part) {
   double v[1] = {0.0};
   const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
   for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
     const long long i = base + e;
-    if (i < n) v[0] += f[i] * f[i];
+    if (i < n) {
+      const double x = h ? f[i] / h[i / BLEN] : f[i];
+      v[0] += x * x;
+    }
   }
   grid_store_partial<1>(v, part);
 }
 
-__global__ __launch_bounds__(WG_THREADS) void k_cnn_rms_fin(long long n, double* scal,
+__global__ __launch_bounds__(WG_THREADS) void k_cnn_rms_fin(long long n, double* scal, int slot,
                                                             const double* part, int nparts) {
   double tot[1];
   grid_sum_partials<1>(part, nparts, tot);
   if (threadIdx.x == 0) {
     const double s = sqrt(tot[0] / (double)n);
-    scal[0] = s > 0 ? s : 1.0;
+    scal[slot] = s > 0 ? s : 1.0;
   }
 }
 
@@ -167,7 +173,7 @@ __device__ __forceinline__ void conv_layer(const _Float16* __restrict__ in,
 
 __global__ __launch_bounds__(WG_THREADS) void k_cnn_infer(int nb, const int* __restrict__ nbr,
                                                           const double* __restrict__ h,
-                                                          const double* __restrict__ rhs,
+                                                          const double* __restrict__ res,
                                                           const double* __restrict__ prev,
                                                           const double* __restrict__ scal,
                                                           const _Float16* __restrict__ wpk,
@@ -178,16 +184,17 @@ __global__ __launch_bounds__(WG_THREADS) void k_cnn_infer(int nb, const int* __r
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   if (b >= nb) return;
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
-  const float s = (float)scal[0], inv_s = 1.0f / s;
-  const float inv_hs = (float)(1.0 / (h[b] * scal[0]));
+  const double s_r = scal[0];
+  const float inv_sp = (float)(1.0 / scal[1]);
+  const float inv_hs = (float)(1.0 / (h[b] * s_r));
   // input: 8-channel cells (2 real), and a zero-ringed act1 buffer
   for (int p = tid; p < PADC; p += WG_THREADS) {
     const int pz = p / 100, py = (p / 10) % 10, px = p % 10;
     half8_t v;
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] = (_Float16)0.f;
-    v[0] = (_Float16)(halo_value(rhs, nbr, b, pz, py, px) * inv_hs);
-    v[1] = (_Float16)(halo_value(prev, nbr, b, pz, py, px) * inv_s);
+    v[0] = (_Float16)(halo_value(res, nbr, b, pz, py, px) * inv_hs);
+    v[1] = (_Float16)(halo_value(prev, nbr, b, pz, py, px) * inv_sp);
     *reinterpret_cast<half8_t*>(&bufB[p * 8]) = v;
   }
   for (int i = tid; i < PADC * CNN_C / 8; i += WG_THREADS) {
@@ -217,7 +224,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_cnn_infer(int nb, const int* __r
   const size_t off = (size_t)b * BLEN;
   conv_layer<CNN_C, CNN_KT3, false>(bufB, wf + (CNN_KT1 + CNN_KT2) * 64, wv, l,
                                     [&](int m, int n, float v) {
-                                      if (n == 0) out[off + m] = prev[off + m] + (double)s * (double)(v + b3);
+                                      if (n == 0) out[off + m] = prev[off + m] + s_r * (double)(v + b3);
                                     });
 }
 
@@ -225,11 +232,11 @@ __global__ __launch_bounds__(WG_THREADS) void k_cnn_infer(int nb, const int* __r
 // Training (FP32)
 // ---------------------------------------------------------------------------
 struct SampleRef {
-  const float* rhs_h;
+  const float* res_h;
   const float* prev;
   const float* target;
-  float s;
-  float pad;
+  float s_r;
+  float s_p;
 };
 
 // per-item scratch layout (floats)
@@ -265,11 +272,11 @@ __global__ __launch_bounds__(WG_THREADS) void k_cnn_train_item(
   float* DZ2 = sc + TS_DZ2;
   float* DZ1 = sc + TS_DZ1;
   for (int j = tid; j < CNN_NW; j += WG_THREADS) W[j] = wg[j];
-  const float inv_s = 1.0f / S.s;
+  const float inv_sr = 1.0f / S.s_r, inv_sp = 1.0f / S.s_p;
   for (int p = tid; p < PADC; p += WG_THREADS) {
     const int pz = p / 100, py = (p / 10) % 10, px = p % 10;
-    X[p] = halo_value(S.rhs_h, nbr, b, pz, py, px) * inv_s;
-    X[PADC + p] = halo_value(S.prev, nbr, b, pz, py, px) * inv_s;
+    X[p] = halo_value(S.res_h, nbr, b, pz, py, px) * inv_sr;
+    X[PADC + p] = halo_value(S.prev, nbr, b, pz, py, px) * inv_sp;
     const bool ring = pz == 0 || pz == 9 || py == 0 || py == 9 || px == 0 || px == 9;
     if (ring)
       for (int c = 0; c < CNN_C; ++c) {
@@ -426,8 +433,8 @@ __global__ void k_cnn_adam(int nitems, const float* __restrict__ gbuf, float* __
   w[j] -= lr * (m / bc1) / (sqrtf(v / bc2) + 1e-8f);
 }
 
-// Training-set entry: rhs / h_b, p_prev and (p - p_prev) / s in FP32.
-__global__ void k_cnn_sample(long long n, const double* __restrict__ rhs,
+// Training-set entry: res / h_b, p_prev and (p - p_prev) / s_r in FP32.
+__global__ void k_cnn_sample(long long n, const double* __restrict__ res,
                              const double* __restrict__ prev, const double* __restrict__ p,
                              const double* __restrict__ h, const double* __restrict__ scal,
                              float* __restrict__ rh, float* __restrict__ pv,
@@ -435,7 +442,7 @@ __global__ void k_cnn_sample(long long n, const double* __restrict__ rhs,
   const double inv_s = 1.0 / scal[0];
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
-    rh[i] = (float)(rhs[i] / h[i / BLEN]);
+    rh[i] = (float)(res[i] / h[i / BLEN]);
     pv[i] = (float)prev[i];
     tg[i] = (float)((p[i] - prev[i]) * inv_s);
   }
@@ -460,8 +467,6 @@ int cnn_init(CnnState& C, std::string& err) {
   CCHECK(hipMalloc(&C.d_adam, 2 * CNN_NW * sizeof(float)));
   CCHECK(hipMalloc(&C.d_wpk, CNN_PK * sizeof(_Float16)));
   CCHECK(hipMalloc(&C.d_scal, 8 * sizeof(double)));
-  CCHECK(hipMalloc(&C.d_counter, 16 * sizeof(unsigned)));
-  CCHECK(hipMemset(C.d_counter, 0, 16 * sizeof(unsigned)));
   CCHECK(hipMemset(C.d_w, 0, CNN_NW * sizeof(float)));
   CCHECK(hipMemset(C.d_adam, 0, 2 * CNN_NW * sizeof(float)));
   return 0;
@@ -512,52 +517,57 @@ int cnn_get_weights(CnnState& C, float* w, int nw, std::string& err) {
   return 0;
 }
 
-static int rms(CnnState& C, hipStream_t s, const double* f, std::string& err) {
+// scal[0] = s_r = RMS(res / h_b), scal[1] = s_p = RMS(p_prev)
+static int scales(CnnState& C, hipStream_t s, const double* res, const double* prev,
+                  std::string& err) {
   const int nwg = (int)((C.n + 4 * BLEN - 1) / (4 * BLEN));
-  k_cnn_rms<<<nwg, WG_THREADS, 0, s>>>(C.n, f, C.d_scal, C.d_counter, C.d_part);
-  k_cnn_rms_fin<<<1, WG_THREADS, 0, s>>>(C.n, C.d_scal, C.d_part, nwg);
+  k_cnn_rms<<<nwg, WG_THREADS, 0, s>>>(C.n, res, C.d_h, C.d_part);
+  k_cnn_rms_fin<<<1, WG_THREADS, 0, s>>>(C.n, C.d_scal, 0, C.d_part, nwg);
+  k_cnn_rms<<<nwg, WG_THREADS, 0, s>>>(C.n, prev, nullptr, C.d_part);
+  k_cnn_rms_fin<<<1, WG_THREADS, 0, s>>>(C.n, C.d_scal, 1, C.d_part, nwg);
   CCHECK(hipGetLastError());
   return 0;
 }
 
-int cnn_predict(CnnState& C, hipStream_t s, const double* d_rhs, const double* d_prev,
+int cnn_predict(CnnState& C, hipStream_t s, const double* d_res, const double* d_prev,
                 double* d_out, std::string& err) {
   if (!C.have_weights) {
     err = "cnn_predict: no weights (set or train them first)";
     return -1;
   }
-  if (rms(C, s, d_prev, err)) return -1;
-  k_cnn_infer<<<C.nb, WG_THREADS, 0, s>>>(C.nb, C.d_nbr, C.d_h, d_rhs, d_prev, C.d_scal,
+  if (scales(C, s, d_res, d_prev, err)) return -1;
+  k_cnn_infer<<<C.nb, WG_THREADS, 0, s>>>(C.nb, C.d_nbr, C.d_h, d_res, d_prev, C.d_scal,
                                           static_cast<const _Float16*>(C.d_wpk), C.d_w, d_out);
   CCHECK(hipGetLastError());
   CCHECK(hipStreamSynchronize(s));
   return 0;
 }
 
-int cnn_add_sample(CnnState& C, hipStream_t s, const double* d_rhs, const double* d_prev,
+int cnn_add_sample(CnnState& C, hipStream_t s, const double* d_res, const double* d_prev,
                    const double* d_p, std::string& err) {
   if (C.nb <= 0) {
     err = "cnn_add_sample: no topology";
     return -1;
   }
   CnnSample S;
-  CCHECK(hipMalloc(&S.rhs_h, 3 * C.n * sizeof(float)));
-  S.prev = S.rhs_h + C.n;
+  CCHECK(hipMalloc(&S.res_h, 3 * C.n * sizeof(float)));
+  S.prev = S.res_h + C.n;
   S.target = S.prev + C.n;
-  if (rms(C, s, d_prev, err)) return -1;
-  k_cnn_sample<<<4096, 256, 0, s>>>(C.n, d_rhs, d_prev, d_p, C.d_h, C.d_scal, S.rhs_h, S.prev,
+  if (scales(C, s, d_res, d_prev, err)) return -1;
+  k_cnn_sample<<<4096, 256, 0, s>>>(C.n, d_res, d_prev, d_p, C.d_h, C.d_scal, S.res_h, S.prev,
                                     S.target);
   CCHECK(hipGetLastError());
-  double sc = 1.0;
-  CCHECK(hipMemcpyAsync(&sc, C.d_scal, sizeof(double), hipMemcpyDeviceToHost, s));
+  double sc[2] = {1.0, 1.0};
+  CCHECK(hipMemcpyAsync(sc, C.d_scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   CCHECK(hipStreamSynchronize(s));
-  S.s = (float)sc;
+  S.s_r = (float)sc[0];
+  S.s_p = (float)sc[1];
   C.samples.push_back(S);
   return 0;
 }
 
 void cnn_clear_samples(CnnState& C) {
-  for (auto& S : C.samples) (void)hipFree(S.rhs_h);
+  for (auto& S : C.samples) (void)hipFree(S.res_h);
   C.samples.clear();
 }
 
@@ -582,7 +592,8 @@ static int upload_samples(CnnState& C, std::vector<SampleRef>& refs, SampleRef**
                           std::string& err) {
   refs.resize(C.samples.size());
   for (size_t k = 0; k < C.samples.size(); ++k)
-    refs[k] = {C.samples[k].rhs_h, C.samples[k].prev, C.samples[k].target, C.samples[k].s, 0.f};
+    refs[k] = {C.samples[k].res_h, C.samples[k].prev, C.samples[k].target, C.samples[k].s_r,
+               C.samples[k].s_p};
   CCHECK(hipMalloc(d_refs, refs.size() * sizeof(SampleRef)));
   CCHECK(hipMemcpy(*d_refs, refs.data(), refs.size() * sizeof(SampleRef), hipMemcpyHostToDevice));
   return 0;
@@ -741,41 +752,12 @@ int cnn_gradient(CnnState& C, hipStream_t s, int k, float* grad, double* loss, s
   return 0;
 }
 
-int cnn_warm_start(CnnState& C, hipStream_t s, const DevGrid<double>& g,
-                   const KrylovVectors<double>& v, SolverState* st, SolverState* h_st,
-                   double* part, const double* d_rhs, double* d_x, int* used, double* resid,
-                   std::string& err) {
-  *used = 0;  // CUP3D_GUESS_GIVEN
-  // residual norms of both guesses with the solver's operator (FP64)
-  auto resid_of = [&](const double* x, double* out) -> int {
-    launch_spmv64(s, SPMV_RESID, g, x, nullptr, d_rhs, v.nu, nullptr, st, part);
-    CCHECK(hipMemcpyAsync(h_st, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
-    CCHECK(hipStreamSynchronize(s));
-    *out = std::sqrt(h_st->true_rr);
-    return 0;
-  };
-  double r_prev = 0, r_net = 0;
-  if (resid_of(d_x, &r_prev)) return -1;
-  *resid = r_prev;
-  if (!C.have_weights) return 0;
-  if (cnn_predict(C, s, d_rhs, d_x, C.d_pred, err)) return -1;
-  if (resid_of(C.d_pred, &r_net)) return -1;
-  if (std::isfinite(r_net) && r_net < r_prev) {
-    CCHECK(hipMemcpyAsync(d_x, C.d_pred, C.n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    CCHECK(hipStreamSynchronize(s));
-    *used = 2;  // CUP3D_GUESS_CNN
-    *resid = r_net;
-  }
-  return 0;
-}
-
 void cnn_free(CnnState& C) {
   cnn_clear_samples(C);
   hipFree(C.d_w);
   hipFree(C.d_adam);
   hipFree(C.d_wpk);
   hipFree(C.d_scal);
-  hipFree(C.d_counter);
   if (C.d_part) hipFree(C.d_part);
   if (C.d_pred) hipFree(C.d_pred);
   if (C.d_scratch) hipFree(C.d_scratch);

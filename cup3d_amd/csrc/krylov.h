@@ -194,18 +194,39 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
     }
     __syncthreads();
     if (valid) {
-      const int r0 = g.irow_ptr[b], r1 = g.irow_ptr[b + 1];
-      for (int r = r0 + lane; r < r1; r += 64) {
+      // interface rows, one per lane, 64 rows per chunk; U entries of a row
+      // are loaded (index and weight, then the x values) before any is used
+      constexpr int U = sizeof(T) == 8 ? 8 : 16;
+      const int c0 = g.ich_ptr[b], c1 = g.ich_ptr[b + 1];
+      for (int ch = c0; ch < c1; ++ch) {
+        const int meta = g.ell_meta[ch * 64 + lane];
+        const int K = g.ell_k[ch];
+        const size_t eo = (size_t)g.ell_off[ch] + lane;
+        const int cl = meta & 0xffff, len = meta >> 16;
+        const T xc = xs[w][cl];
         T acc = 0;
-        const int cl = g.irow_cell[r];
-        if (sizeof(T) == 8) {
-          for (int e = g.irow_ent[r]; e < g.irow_ent[r + 1]; ++e) acc += g.ent_val[e] * x[g.ent_col[e]];
-        } else {  // interface rows sum to zero: use differences to the cell value
-          const T xc = xs[w][cl];
-          for (int e = g.irow_ent[r]; e < g.irow_ent[r + 1]; ++e)
-            acc += g.ent_val[e] * (x[g.ent_col[e]] - xc);
+        for (int k0 = 0; k0 < K; k0 += U) {
+          int ci[U];
+          T cv[U], xv[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const bool on = k0 + u < len;
+            ci[u] = on ? g.ell_col[eo + (size_t)(k0 + u) * 64] : 0;
+            cv[u] = on ? g.ell_val[eo + (size_t)(k0 + u) * 64] : (T)0;
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) xv[u] = k0 + u < len ? x[ci[u]] : xc;
+          if (sizeof(T) == 8) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+              if (k0 + u < len) acc += cv[u] * xv[u];
+          } else {  // interface rows sum to zero: use differences to the cell value
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+              if (k0 + u < len) acc += cv[u] * (xv[u] - xc);
+          }
         }
-        ys[w][cl] += acc;
+        if (len > 0) ys[w][cl] += acc;
       }
     }
     __syncthreads();

@@ -80,11 +80,16 @@ struct DevGrid {
   const T* h = nullptr;             // [nb] cell size
   const T* invh = nullptr;          // [nb]
   const double* h3 = nullptr;       // [nb] h^3 (always FP64: mean constraint)
-  const int* irow_ptr = nullptr;    // [nb+1] interface rows per block
-  const int* irow_cell = nullptr;   // [nrows] local cell 0..511
-  const int* irow_ent = nullptr;    // [nrows+1] entry range
-  const int* ent_col = nullptr;     // [nent] global cell index
-  const T* ent_val = nullptr;       // [nent]
+  // coarse-fine interface rows, sliced ELL: the rows of block b form chunks
+  // [ich_ptr[b], ich_ptr[b+1]) of 64 rows (one per lane); chunk c holds
+  // ell_meta[64 c + lane] = cell | len << 16 (len 0: no row) and its entries
+  // slot-major at ell_off[c] + 64 k + lane, k < len <= ell_k[c]
+  const int* ich_ptr = nullptr;     // [nb+1]
+  const int* ell_meta = nullptr;    // [nchunks * 64]
+  const int* ell_off = nullptr;     // [nchunks]
+  const int* ell_k = nullptr;       // [nchunks]
+  const int* ell_col = nullptr;     // global cell index
+  const T* ell_val = nullptr;
   long long bmean_row = -1;         // cell replaced by sum h^3 x (bMean == 1)
   int bmean = 1;
   int has_iface = 0;                // any coarse-fine interface rows
@@ -96,8 +101,10 @@ struct HostTopology {
   std::vector<int> nbr;             // nb*6
   std::vector<int8_t> kind;         // nb*6
   std::vector<double> h, invh, h3;
-  std::vector<int> irow_ptr, irow_cell, irow_ent, ent_col;
+  std::vector<int> irow_ptr, irow_cell, irow_ent, ent_col;  // CSR form of the interface rows
   std::vector<double> ent_val;
+  std::vector<int> ich_ptr, ell_meta, ell_off, ell_k, ell_col;  // device (sliced ELL) form
+  std::vector<double> ell_val;
   long long bmean_row = -1;
   int origin_block = -1;
   long long n_coarse_faces = 0, n_fine_faces = 0;

@@ -74,8 +74,8 @@ struct cup3d_poisson {
   std::vector<int> blocks;  // nb x 4
   int nb = 0;
   long long n = 0;
-  DevBuf d_nbr, d_h, d_invh, d_h3, d_hf, d_invhf, d_irow_ptr, d_irow_cell, d_irow_ent, d_ent_col,
-      d_ent_val, d_ent_valf, d_blocks;
+  DevBuf d_nbr, d_h, d_invh, d_h3, d_hf, d_invhf, d_ich_ptr, d_ell_meta, d_ell_off, d_ell_k,
+      d_ell_col, d_ell_val, d_ell_valf, d_blocks;
   DevGrid<double> g64;
   DevGrid<float> g32;
 
@@ -244,19 +244,20 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
   H->nb = (int)nblocks;
   H->n = (long long)nblocks * BLEN;
   std::vector<float> hf(T.h.begin(), T.h.end()), invhf(T.invh.begin(), T.invh.end()),
-      valf(T.ent_val.begin(), T.ent_val.end());
+      valf(T.ell_val.begin(), T.ell_val.end());
   if (upload(H, H->d_nbr, T.nbr.data(), T.nbr.size() * sizeof(int)) ||
       upload(H, H->d_h, T.h.data(), T.h.size() * sizeof(double)) ||
       upload(H, H->d_invh, T.invh.data(), T.invh.size() * sizeof(double)) ||
       upload(H, H->d_h3, T.h3.data(), T.h3.size() * sizeof(double)) ||
       upload(H, H->d_hf, hf.data(), hf.size() * sizeof(float)) ||
       upload(H, H->d_invhf, invhf.data(), invhf.size() * sizeof(float)) ||
-      upload(H, H->d_irow_ptr, T.irow_ptr.data(), T.irow_ptr.size() * sizeof(int)) ||
-      upload(H, H->d_irow_cell, T.irow_cell.data(), T.irow_cell.size() * sizeof(int)) ||
-      upload(H, H->d_irow_ent, T.irow_ent.data(), T.irow_ent.size() * sizeof(int)) ||
-      upload(H, H->d_ent_col, T.ent_col.data(), T.ent_col.size() * sizeof(int)) ||
-      upload(H, H->d_ent_val, T.ent_val.data(), T.ent_val.size() * sizeof(double)) ||
-      upload(H, H->d_ent_valf, valf.data(), valf.size() * sizeof(float)) ||
+      upload(H, H->d_ich_ptr, T.ich_ptr.data(), T.ich_ptr.size() * sizeof(int)) ||
+      upload(H, H->d_ell_meta, T.ell_meta.data(), T.ell_meta.size() * sizeof(int)) ||
+      upload(H, H->d_ell_off, T.ell_off.data(), T.ell_off.size() * sizeof(int)) ||
+      upload(H, H->d_ell_k, T.ell_k.data(), T.ell_k.size() * sizeof(int)) ||
+      upload(H, H->d_ell_col, T.ell_col.data(), T.ell_col.size() * sizeof(int)) ||
+      upload(H, H->d_ell_val, T.ell_val.data(), T.ell_val.size() * sizeof(double)) ||
+      upload(H, H->d_ell_valf, valf.data(), valf.size() * sizeof(float)) ||
       upload(H, H->d_blocks, H->blocks.data(), H->blocks.size() * sizeof(int)))
     return -1;
   const long long origin_row = T.origin_block >= 0 ? (long long)T.origin_block * BLEN : -1;
@@ -269,17 +270,18 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
     g.h = static_cast<const TT*>(h);
     g.invh = static_cast<const TT*>(invh);
     g.h3 = H->d_h3.as<double>();
-    g.irow_ptr = H->d_irow_ptr.as<int>();
-    g.irow_cell = H->d_irow_cell.as<int>();
-    g.irow_ent = H->d_irow_ent.as<int>();
-    g.ent_col = H->d_ent_col.as<int>();
-    g.ent_val = static_cast<const TT*>(val);
+    g.ich_ptr = H->d_ich_ptr.as<int>();
+    g.ell_meta = H->d_ell_meta.as<int>();
+    g.ell_off = H->d_ell_off.as<int>();
+    g.ell_k = H->d_ell_k.as<int>();
+    g.ell_col = H->d_ell_col.as<int>();
+    g.ell_val = static_cast<const TT*>(val);
     g.bmean = H->cfg.bmean;
     g.has_iface = T.irow_cell.empty() ? 0 : 1;
     g.bmean_row = (H->cfg.bmean == 1 || H->cfg.bmean == 3) ? origin_row : -1;
   };
-  fill(H->g64, H->d_h.p, H->d_invh.p, H->d_ent_val.p);
-  fill(H->g32, H->d_hf.p, H->d_invhf.p, H->d_ent_valf.p);
+  fill(H->g64, H->d_h.p, H->d_invh.p, H->d_ell_val.p);
+  fill(H->g32, H->d_hf.p, H->d_invhf.p, H->d_ell_valf.p);
   // FP64 Krylov vectors: x[3], r, rhat, p, nu, t, z1, z2, b
   const size_t n = (size_t)H->n;
   if (H->d_vec64.alloc(11 * n * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(vectors) failed");
@@ -338,6 +340,62 @@ static int solve_fp64(cup3d_poisson_t H, const double* d_rhs, double* d_x, const
   return S.status;
 }
 
+// Increment form of a step: b and x0 as getVec makes them (v64.b, v64.x[0]),
+// and r0 = b - A x0 in v64.nu; returns ||r0|| (FP64) or -1 on error.
+static double increment_residual(cup3d_poisson_t H, const double* d_rhs, const double* d_prev) {
+  hipStream_t s = H->stream;
+  SolverState* st = H->d_state.as<SolverState>();
+  double* part = H->d_part.as<double>();
+  *H->h_state = make_state(H->cfg);
+  if (hipMemcpyAsync(st, H->h_state, sizeof(SolverState), hipMemcpyHostToDevice, s) != hipSuccess)
+    return fail(H, "hipMemcpyAsync(state) failed");
+  const long long zero_row = (H->cfg.bmean == 1 || H->cfg.bmean == 3) && H->topo.origin_block >= 0
+                                 ? (long long)H->topo.origin_block * BLEN
+                                 : -1;
+  launch_getvec64(s, H->n, zero_row, d_rhs, d_prev, H->v64.b, H->v64.x[0], st, part, 0.0);
+  launch_spmv64(s, SPMV_RESID, H->g64, H->v64.x[0], nullptr, H->v64.b, H->v64.nu, nullptr, st,
+                part);
+  if (hipMemcpyAsync(H->h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail(H, "increment residual failed");
+  return std::sqrt(H->h_state->true_rr);
+}
+
+// ||b - A x|| for b in v64.b (FP64), the residual into v64.t.
+static double resid_norm(cup3d_poisson_t H, const double* d_x) {
+  hipStream_t s = H->stream;
+  SolverState* st = H->d_state.as<SolverState>();
+  launch_spmv64(s, SPMV_RESID, H->g64, d_x, nullptr, H->v64.b, H->v64.t, nullptr, st,
+                H->d_part.as<double>());
+  if (hipMemcpyAsync(H->h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail(H, "residual failed");
+  return std::sqrt(H->h_state->true_rr);
+}
+
+// Learned warm start: the net's guess replaces d_x only when its FP64
+// residual is smaller than that of the given guess (fallback otherwise).
+static int cnn_warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* used,
+                          double* resid) {
+  *used = CUP3D_GUESS_GIVEN;
+  const double r_prev = increment_residual(H, d_rhs, d_x);
+  if (r_prev < 0) return -1;
+  *resid = r_prev;
+  if (!H->cnn.have_weights) return 0;
+  double* pred = H->cnn.d_pred;
+  if (cnn_predict(H->cnn, H->stream, H->v64.nu, H->v64.x[0], pred, H->err) != 0) return -1;
+  const double r_net = resid_norm(H, pred);
+  if (r_net < 0 && !std::isnan(r_net)) return -1;
+  if (std::isfinite(r_net) && r_net < r_prev) {
+    HIPCHECK(hipMemcpyAsync(d_x, pred, (size_t)H->n * sizeof(double), hipMemcpyDeviceToDevice,
+                            H->stream));
+    HIPCHECK(hipStreamSynchronize(H->stream));
+    *used = CUP3D_GUESS_CNN;
+    *resid = r_net;
+  }
+  return 0;
+}
+
 int cup3d_poisson_solve_device(cup3d_poisson_t H, const double* d_rhs, double* d_x,
                                cup3d_solve_report* rep) {
   if (!H) return -1;
@@ -355,10 +413,7 @@ int cup3d_poisson_solve_device(cup3d_poisson_t H, const double* d_rhs, double* d
   int st;
   if (H->cfg.guess == CUP3D_GUESS_CNN) {
     // learned warm start with fallback to the given guess
-    if (cnn_warm_start(H->cnn, H->stream, H->g64, H->v64, H->d_state.as<SolverState>(),
-                       H->h_state, H->d_part.as<double>(), d_rhs, d_x, &guess_used, &guess_resid,
-                       H->err) != 0)
-      return -1;
+    if (cnn_warm_start(H, d_rhs, d_x, &guess_used, &guess_resid) != 0) return -1;
   }
   if (H->cfg.precision == CUP3D_PREC_FP64)
     st = solve_fp64(H, d_rhs, d_x, x0, &local);
@@ -472,15 +527,20 @@ int cup3d_poisson_cnn_predict(cup3d_poisson_t H, const double* d_rhs, const doub
                               double* d_out) {
   if (!H) return -1;
   if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  if (!d_rhs || !d_prev || !d_out) return fail(H, "null vector");
   HIPCHECK(hipSetDevice(H->device));
-  return cnn_predict(H->cnn, H->stream, d_rhs, d_prev, d_out, H->err);
+  if (increment_residual(H, d_rhs, d_prev) < 0) return -1;
+  return cnn_predict(H->cnn, H->stream, H->v64.nu, H->v64.x[0], d_out, H->err);
 }
 
 int cup3d_poisson_cnn_add_sample(cup3d_poisson_t H, const double* d_rhs, const double* d_prev,
                                  const double* d_p) {
   if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  if (!d_rhs || !d_prev || !d_p) return fail(H, "null vector");
   HIPCHECK(hipSetDevice(H->device));
-  return cnn_add_sample(H->cnn, H->stream, d_rhs, d_prev, d_p, H->err);
+  if (increment_residual(H, d_rhs, d_prev) < 0) return -1;
+  return cnn_add_sample(H->cnn, H->stream, H->v64.nu, H->v64.x[0], d_p, H->err);
 }
 
 int cup3d_poisson_cnn_clear_samples(cup3d_poisson_t H) {

@@ -1,0 +1,129 @@
+"""Learned warm start on a sequence of pressure solves: generate, train, judge.
+
+The sequence is the uniform 256x128x128 walls workload with 64 Gaussian
+blobs (sigma 0.04) moving 0.005 per step (``workloads.c4_uniform_sequence``).
+Each step is solved by the FP64 drop-in to ||b - A x|| / ||b|| <= 1e-7 from
+the previous step's pressure; the first ``--train`` steps give the training
+triples (rhs_t, p_{t-1}, p_t), which the library's own HIP kernels learn
+(FP32 forward/backward, Adam). The net is then judged on the remaining steps
+and on a sequence from a fresh seed by what it saves: the number of
+iterations, for the FP64 and the mixed-precision solve, against starting from
+the previous pressure. The solve always finishes to the same tolerance and
+falls back to the previous pressure whenever the net's guess has the larger
+residual, so a worse net costs one inference and two residuals, never
+accuracy.
+
+  python -m cup3d_amd.learned [--steps 400] [--train 300] [--fresh 40]
+                              [--train-steps 3000] [--batch 256] [--bpd 32 16 16]
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+
+def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=3000, batch=256,
+        lr=1e-3, seed=1, tol=1e-7, log=sys.stderr):
+    import torch
+
+    from . import workloads as W
+    from .poisson import solver_for
+
+    dev = torch.device("cuda:0")
+    w = W.c4_uniform_sequence(tuple(bpd))
+    kw = dict(tol_rel=0.0, tol_rhs=tol, bmean=0, max_iter=5000)
+    ref = solver_for(w, precision="fp64", **kw)
+
+    def sequence(n, s):
+        """FP64 solutions p_0 .. p_n of steps 0..n from previous pressures."""
+        rhs, sols, its = [], [], []
+        p = torch.zeros(w.ncells, dtype=torch.float64, device=dev)
+        for t in range(n + 1):
+            b = torch.from_numpy(w.rhs(step=t, seed=s).reshape(-1)).to(dev)
+            x = p.clone()
+            rep = ref.solve_device(b, x)
+            if rep["status"] != 0:
+                raise RuntimeError(f"step {t}: FP64 solve did not converge: {rep}")
+            rhs.append(b)
+            sols.append(x)
+            its.append(rep["iterations"])
+            p = x
+        return rhs, sols, its
+
+    t0 = time.time()
+    rhs, sols, its = sequence(steps, W.SEED)
+    print(f"sequence of {steps} solves: {time.time() - t0:.1f} s", file=log, flush=True)
+    for t in range(1, train + 1):
+        ref.cnn_add_sample(rhs[t], sols[t - 1], sols[t])
+    t0 = time.time()
+    l0, _ = ref.cnn_train(1, batch=batch, lr=lr, seed=seed)
+    _, l1 = ref.cnn_train(train_steps - 1, batch=batch, lr=lr, seed=seed + 1)
+    train_s = time.time() - t0
+    print(f"training: {train_steps} steps of {batch} blocks, loss {l0:.3e} -> {l1:.3e}, "
+          f"{train_s:.1f} s", file=log, flush=True)
+    weights = ref.cnn_get_weights()
+
+    def judge(rhs_, sols_, ts):
+        out = {}
+        for form in ("fp64", "mixed_fp16"):
+            base = solver_for(w, precision=form, guess="given", **kw)
+            net = solver_for(w, precision=form, guess="cnn", **kw)
+            net.cnn_set_weights(weights)
+            it_b, it_n, used, ratio, err = [], [], 0, [], []
+            for t in ts:
+                xb = sols_[t - 1].clone()
+                rb = base.solve_device(rhs_[t], xb)
+                xn = sols_[t - 1].clone()
+                rn = net.solve_device(rhs_[t], xn)
+                if rb["status"] != 0 or rn["status"] != 0:
+                    raise RuntimeError(f"step {t}: {form} solve did not converge")
+                it_b.append(rb["iterations"])
+                it_n.append(rn["iterations"])
+                used += int(rn["guess_used"] == 2)
+                ratio.append(rn["guess_resid"] / rb["err_init"])
+                err.append(float(torch.linalg.norm(xn - sols_[t]) / torch.linalg.norm(sols_[t])))
+            out[form] = {
+                "iterations_prev_guess": float(np.mean(it_b)),
+                "iterations_net_guess": float(np.mean(it_n)),
+                "iterations_saved_pct": round(100.0 * (1 - np.sum(it_n) / np.sum(it_b)), 2),
+                "net_guess_used": f"{used}/{len(ts)}",
+                "initial_resid_ratio_median": float(np.median(ratio)),
+                "err_vs_fp64_max": float(np.max(err)),
+            }
+        return out
+
+    held = judge(rhs, sols, range(train + 1, steps + 1))
+    frhs, fsols, _ = sequence(fresh, W.FRESH_SEED)
+    fr = judge(frhs, fsols, range(1, fresh + 1))
+    return {
+        "workload": w.name, "cells": w.ncells, "steps": steps, "train_triples": train,
+        "train_steps": train_steps, "batch_blocks": batch, "lr": lr,
+        "loss_first": l0, "loss_last": l1, "train_seconds": round(train_s, 1),
+        "fp64_iterations_mean_train": float(np.mean(its[1:train + 1])),
+        "held_out": held, "fresh_seed": fr,
+    }
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--steps", type=int, default=400)
+    p.add_argument("--train", type=int, default=300)
+    p.add_argument("--fresh", type=int, default=40)
+    p.add_argument("--train-steps", type=int, default=3000)
+    p.add_argument("--batch", type=int, default=256)
+    p.add_argument("--lr", type=float, default=1e-3)
+    p.add_argument("--bpd", type=int, nargs=3, default=[32, 16, 16])
+    a = p.parse_args()
+    if not 0 < a.train < a.steps:
+        raise SystemExit("need 0 < --train < --steps")
+    print(json.dumps(run(a.bpd, a.steps, a.train, a.fresh, a.train_steps, a.batch, a.lr)))
+
+
+if __name__ == "__main__":
+    main()

@@ -146,10 +146,14 @@ int cup3d_poisson_apply_preconditioner(cup3d_poisson_t h, const double* d_p, dou
 int cup3d_cnn_num_weights(void);
 int cup3d_poisson_cnn_set_weights(cup3d_poisson_t h, const float* w, int n_weights);
 int cup3d_poisson_cnn_get_weights(cup3d_poisson_t h, float* w, int n_weights);
-/* d_out = net(d_rhs, d_prev) on all blocks (device pointers, doubles). */
+/* d_out = p_prev + s_r * net(res / (h s_r), p_prev / s_p) on all blocks, with
+ * res = b - A p_prev the residual of the previous pressure for this rhs
+ * (b, p_prev as getVec makes them from d_rhs, d_prev), s_r = RMS(res / h),
+ * s_p = RMS(p_prev). Device pointers, doubles. */
 int cup3d_poisson_cnn_predict(cup3d_poisson_t h, const double* d_rhs, const double* d_prev,
                               double* d_out);
-/* Append one triple (rhs, p_prev, p) (device pointers) to the training set. */
+/* Append one triple (rhs, p_prev, p) (device pointers) to the training set;
+ * stored as (res / h, p_prev, (p - p_prev) / s_r) in FP32. */
 int cup3d_poisson_cnn_add_sample(cup3d_poisson_t h, const double* d_rhs, const double* d_prev,
                                  const double* d_p);
 int cup3d_poisson_cnn_clear_samples(cup3d_poisson_t h);

@@ -1,15 +1,17 @@
 """FP64 CPU oracle of the learned warm start — TEST INFRASTRUCTURE ONLY.
 
-The network has no counterpart in the reference solver (it is new in this
-build): a 3-layer 3-D CNN applied per 8^3 block with a one-cell halo,
-  input  2 x 10^3  (rhs / (h_b s), p_prev / s)
+The network has no counterpart in the reference solver: a 3-layer 3-D CNN
+applied per 8^3 block with a one-cell halo, in the increment form of the
+pressure solve (PressureProjection.cpp:368-385 solves A (p - p_old) =
+b - A p_old):
+  input  2 x 10^3  (res / (h_b s_r), p_prev / s_p),  res = b - A p_prev
   conv 3^3 valid, 2 -> 16, GELU
   conv 3^3 same (zero padded), 16 -> 16, GELU
   conv 3^3 same (zero padded), 16 -> 1
-  p_net = p_prev + s * out,   s = RMS(p_prev) over the grid.
-rhs / h_b carries the units of the 7-point stencil applied to p
-(b = h^3 lap p, stencil = h^2 lap p), so both channels are commensurate on
-every AMR level.
+  p_net = p_prev + s_r * out,
+  s_r = RMS(res / h_b), s_p = RMS(p_prev) over the grid (1 where zero).
+res / h_b carries the units of the 7-point stencil applied to the increment
+(A = h^3 lap), so the input is commensurate on every AMR level.
 
 Halo rule: a face halo cell takes the adjacent cell of the same-level
 neighbour block when the face is shared with one; every other halo cell
@@ -89,14 +91,18 @@ def halo(field, nbr):
     return out
 
 
-def inputs(rhs, prev, h, nbr):
-    """Network input [nb,2,10,10,10] and the scale s."""
+def rms(f):
+    s = math.sqrt(float(np.mean(np.asarray(f, np.float64) ** 2)))
+    return s if s > 0 else 1.0
+
+
+def inputs(res, prev, h, nbr):
+    """Network input [nb,2,10,10,10] and the scales (s_r, s_p)."""
     prev = np.asarray(prev, np.float64).reshape(-1, 8, 8, 8)
-    rhs = np.asarray(rhs, np.float64).reshape(-1, 8, 8, 8)
-    s = math.sqrt(float(np.mean(prev * prev)))
-    s = s if s > 0 else 1.0
-    x = np.stack([halo(rhs / (np.asarray(h)[:, None, None, None] * s), nbr), halo(prev / s, nbr)], 1)
-    return x, s
+    rh = np.asarray(res, np.float64).reshape(-1, 8, 8, 8) / np.asarray(h)[:, None, None, None]
+    s_r, s_p = rms(rh), rms(prev)
+    x = np.stack([halo(rh / s_r, nbr), halo(prev / s_p, nbr)], 1)
+    return x, s_r, s_p
 
 
 def conv_valid(x, W):
@@ -178,7 +184,8 @@ def adam(w, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
     return w - lr * mh / (np.sqrt(vh) + eps), m, v
 
 
-def predict(w, rhs, prev, h, nbr):
-    x, s = inputs(rhs, prev, h, nbr)
+def predict(w, res, prev, h, nbr):
+    """p_net for res = b - A p_prev."""
+    x, s_r, _ = inputs(res, prev, h, nbr)
     out, _ = forward(w, x)
-    return np.asarray(prev, np.float64).reshape(-1, 8, 8, 8) + s * out
+    return np.asarray(prev, np.float64).reshape(-1, 8, 8, 8) + s_r * out

@@ -87,6 +87,16 @@ def tiny_amr(periodic=False, three_levels=False):
     return blocks, (2, 2, 2), h0, bc
 
 
+def school4_small():
+    """Four levels around 6 seeded ellipsoids on 4x2x2 base blocks (625
+    blocks): every coarse-fine configuration of the school grid at a size the
+    oracle handles in seconds."""
+    from cup3d_amd import workloads as W
+    bpd, h0 = (4, 2, 2), 1.0 / 16
+    blocks = W.amr_school_blocks(bpd, h0, 3, (0.18, 0.09, 0.045), 6, W.SCHOOL_AXES, W.SEED)
+    return blocks, bpd, h0, (WALL,) * 3
+
+
 def global_to_blocks(g, blocks):
     return np.stack([g[8 * k:8 * k + 8, 8 * j:8 * j + 8, 8 * i:8 * i + 8]
                      for (_, i, j, k) in blocks])

@@ -13,7 +13,7 @@ import pytest
 
 from cup3d_amd import poisson as P
 from cup3d_amd._lib import load_library
-from helpers import grid_cases, tiny_amr
+from helpers import school4_small, grid_cases, tiny_amr
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -80,6 +80,7 @@ def debug_csr(blocks, bpd, h0, bc, bmean):
 
 CASES = dict(grid_cases())
 CASES["tiny_amr3_walls"] = tiny_amr(three_levels=True)
+CASES["school4_small"] = school4_small()
 CASES["tiny_amr_periodic"] = tiny_amr(periodic=True)
 
 

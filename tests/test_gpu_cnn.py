@@ -5,6 +5,7 @@ import ctypes
 
 import cnn_oracle as C
 import numpy as np
+import pyoracle
 import pytest
 import torch
 
@@ -48,17 +49,24 @@ def smooth_fields(blocks, h0, seed):
     return rhs, prev, h
 
 
+def residual(blocks, bpd, h0, bc, rhs, prev):
+    """b - A p_prev with the oracle's operator (bMean = 0)."""
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
+    return rhs.reshape(-1) - O.spmv(prev.reshape(-1))
+
+
 @pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "amr2_periodic_offcentre", "amr3_walls"])
 def test_predict_matches_oracle(gpu, name):
     blocks, bpd, h0, bc = grid_cases()[name]
-    S = PoissonSolver(bpd, h0, bc).set_topology(blocks)
+    S = PoissonSolver(bpd, h0, bc, bmean=0).set_topology(blocks)
     w = C.init_weights(5)
     w[C.NW - 1] = 0.05
     S.cnn_set_weights(w.astype(np.float32))
     rhs, prev, h = smooth_fields(blocks, h0, 1)
     out = torch.zeros(len(blocks) * 512, dtype=torch.float64, device=gpu)
     S.cnn_predict(dev(rhs), dev(prev), out)
-    ref = C.predict(w, rhs, prev, h, nbr_table(blocks, bpd, bc))
+    res = residual(blocks, bpd, h0, bc, rhs, prev)
+    ref = C.predict(w, res, prev, h, nbr_table(blocks, bpd, bc))
     inc_g = out.cpu().numpy().reshape(ref.shape) - prev
     inc_o = ref - prev
     # FP16 weights and activations, FP32 accumulation: ~1e-3 of the increment
@@ -68,7 +76,7 @@ def test_predict_matches_oracle(gpu, name):
 
 def test_gradient_matches_oracle(gpu):
     blocks, bpd, h0, bc = grid_cases()["amr2_periodic_offcentre"]
-    S = PoissonSolver(bpd, h0, bc).set_topology(blocks)
+    S = PoissonSolver(bpd, h0, bc, bmean=0).set_topology(blocks)
     w = C.init_weights(7)
     w[C.NW - 1] = 0.02
     S.cnn_set_weights(w.astype(np.float32))
@@ -81,8 +89,9 @@ def test_gradient_matches_oracle(gpu):
     g = np.zeros(C.NW, np.float32)
     loss = ctypes.c_double()
     assert f(S.h, 0, g.ctypes.data, ctypes.byref(loss)) == 0
-    x, s = C.inputs(rhs, prev, h, nbr_table(blocks, bpd, bc))
-    target = (p - prev) / s
+    res = residual(blocks, bpd, h0, bc, rhs, prev)
+    x, s_r, _ = C.inputs(res, prev, h, nbr_table(blocks, bpd, bc))
+    target = (p - prev) / s_r
     lo, go = C.loss_and_grad(w.astype(np.float32).astype(np.float64), x, target)
     assert abs(loss.value - lo) < 1e-4 * lo
     assert np.linalg.norm(g - go) < 1e-3 * np.linalg.norm(go)

@@ -13,12 +13,13 @@ import torch
 
 from cup3d_amd import workloads as W
 from cup3d_amd.poisson import PoissonSolver, solver_for
-from helpers import fft_solution_periodic, grid_cases, random_field, tiny_amr
+from helpers import school4_small, fft_solution_periodic, grid_cases, random_field, tiny_amr
 
 pytestmark = pytest.mark.gpu
 
 CASES = dict(grid_cases())
 CASES["tiny_amr3_walls"] = tiny_amr(three_levels=True)
+CASES["school4_small"] = school4_small()
 CASES["tiny_amr_periodic"] = tiny_amr(periodic=True)
 
 

@@ -8,12 +8,13 @@ import torch
 
 from cup3d_amd import workloads as W
 from cup3d_amd.poisson import PoissonSolver, solver_for
-from helpers import fft_solution_periodic, grid_cases, random_field, tiny_amr
+from helpers import school4_small, fft_solution_periodic, grid_cases, random_field, tiny_amr
 
 pytestmark = pytest.mark.gpu
 
 CASES = dict(grid_cases())
 CASES["tiny_amr3_walls"] = tiny_amr(three_levels=True)
+CASES["school4_small"] = school4_small()
 
 
 def dev(a):
@@ -65,6 +66,23 @@ def test_mixed_solve_reaches_fp64_tolerance(gpu, name, prec):
     xo, info = O.bicgstab(b, tol_abs=1e-10 * np.linalg.norm(bb), tol_rel=0.0)
     err = np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo)
     assert err < 1e-7, err
+
+
+@pytest.mark.parametrize("prec", ["mixed_fp16", "mixed_bf16"])
+def test_mixed_school4_residual_checked_by_oracle(gpu, prec):
+    """Four-level grid, singular walls system (bMean = 0, compatible rhs):
+    the refined solve reports true relative residual <= 1e-9, and the oracle's
+    own operator, applied on the CPU to the returned pressure, confirms it."""
+    blocks, bpd, h0, bc = CASES["school4_small"]
+    b = compatible_rhs(blocks, h0, 4)
+    S = PoissonSolver(bpd, h0, bc, precision=prec, bmean=0, tol_rel=0.0, tol_rhs=1e-9,
+                      max_iter=4000).set_topology(blocks)
+    x = torch.zeros(len(blocks) * 512, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(dev(b), x)
+    assert rep["status"] == 0 and rep["true_rel_resid"] <= 1e-9, rep
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
+    r = b.reshape(-1) - O.spmv(x.cpu().numpy())
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.0001e-9
 
 
 def test_mixed_c1_error_vs_exact(gpu):
