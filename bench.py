@@ -27,6 +27,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -155,12 +157,22 @@ def main():
         return
     rhs_prev = torch.from_numpy(w.rhs(step=0).reshape(-1)).to(dev)
     rhs = torch.from_numpy(w.rhs(step=1).reshape(-1)).to(dev)
-    # previous-step pressure and an FP64 reference solution (untimed setup)
-    R = solver_for(w, tol_rel=0.0, tol_rhs=1e-10, bmean=args.bmean, max_iter=5000)
-    p_prev = torch.zeros(w.ncells, dtype=torch.float64, device=dev)
-    R.solve_device(rhs_prev, p_prev)
-    x_ref = p_prev.clone()
-    R.solve_device(rhs, x_ref)
+    # previous-step pressure and a reference solution to ||b - Ax|| <= 1e-10 ||b||
+    # (untimed setup): the FP64 drop-in, or where its recurrence stagnates
+    # above that, the FP64-refined mixed solve (its convergence test is the
+    # same FP64 true residual)
+    ref_form, ref_res = "fp64", None
+    for form in ("fp64", "mixed_fp16"):
+        R = solver_for(w, tol_rel=0.0, tol_rhs=1e-10, bmean=args.bmean, precision=form,
+                       max_iter=5000, max_refine=40)
+        p_prev = torch.zeros(w.ncells, dtype=torch.float64, device=dev)
+        r0 = R.solve_device(rhs_prev, p_prev)
+        x_ref = p_prev.clone()
+        r1 = R.solve_device(rhs, x_ref)
+        ref_form, ref_res = form, r1["true_rel_resid"]
+        if r0["status"] == 0 and r1["status"] == 0:
+            break
+    hw = torch.from_numpy(np.repeat((w.h0 / 2.0 ** w.blocks[:, 0]) ** 3, 512)).to(dev)
 
     S64, x64, rep64, ms64, ok64 = time_form(w, "fp64", args.bmean, rhs, p_prev, args.steps,
                                             args.warmup)
@@ -173,6 +185,10 @@ def main():
         raise SystemExit(f"solve did not reach the tolerance: {rep}")
 
     def rel(a, b):
+        """Relative L2 distance after removing the volume-weighted means (the
+        caller's post-solve projection, PressureProjection.cpp:404-431)."""
+        a = a - (hw * a).sum() / hw.sum()
+        b = b - (hw * b).sum() / hw.sum()
         return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
 
     kern = kernel_times(S, [2, 3, 4])
@@ -208,6 +224,7 @@ def main():
         "cell_iterations_per_s": round(w.ncells * rep["iterations"] / (ms * 1e-3), 1),
         "true_rel_resid": rep["true_rel_resid"],
         "err_vs_fp64_reference": rel(x, x_ref),
+        "reference": {"form": ref_form, "true_rel_resid": ref_res},
         "fp64": {"solves_per_s": round(1e3 / ms64, 3), "ms_per_step": round(ms64, 3),
                  "iterations": rep64["iterations"], "true_rel_resid": rep64["true_rel_resid"],
                  "err_vs_fp64_reference": rel(x64, x_ref),

@@ -47,6 +47,7 @@ struct CnnState {
   const int* d_nbr = nullptr;
   const double* d_h = nullptr;
   int adam_t = 0;
+  float res_weight = 0.f;  // weight of the local-residual loss term of the last training
   bool have_weights = false;
   std::vector<CnnSample> samples;
 };
@@ -63,11 +64,15 @@ int cnn_predict(CnnState& C, hipStream_t s, const double* d_res, const double* d
 int cnn_add_sample(CnnState& C, hipStream_t s, const double* d_res, const double* d_prev,
                    const double* d_p, std::string& err);
 void cnn_clear_samples(CnnState& C);
-int cnn_train(CnnState& C, hipStream_t s, int steps, int batch, float lr, uint64_t seed,
-              double* loss, std::string& err);
+// Loss per item: 0.5 mean((out - target)^2) + res_weight 0.5 mean(rho^2),
+// rho = lap7(out) - res / (h s_r) with the target closing the stencil
+// outside the block (the discrete increment equation A delta = res).
+int cnn_train(CnnState& C, hipStream_t s, int steps, int batch, float lr, float res_weight,
+              uint64_t seed, double* loss, std::string& err);
 int cnn_eval(CnnState& C, hipStream_t s, int k, double* loss, std::string& err);
 // Full-batch FP32 gradient of the loss over every block of sample k.
-int cnn_gradient(CnnState& C, hipStream_t s, int k, float* grad, double* loss, std::string& err);
+int cnn_gradient(CnnState& C, hipStream_t s, int k, float res_weight, float* grad, double* loss,
+                 std::string& err);
 void cnn_free(CnnState& C);
 
 }  // namespace cup3d

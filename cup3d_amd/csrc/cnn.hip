@@ -255,8 +255,10 @@ template <bool EVAL>
 __global__ __launch_bounds__(WG_THREADS) void k_cnn_train_item(
     int nitems, const int* __restrict__ items, const SampleRef* __restrict__ samples,
     const int* __restrict__ nbr, const float* __restrict__ wg, float* __restrict__ scratch,
-    float* __restrict__ gbuf, float* __restrict__ lossbuf, float inv_ntot) {
+    float* __restrict__ gbuf, float* __restrict__ lossbuf, float inv_ntot, float lam) {
   __shared__ float W[CNN_NW];
+  __shared__ float OP[PADC];   // target halo, then the prediction inside it
+  __shared__ float RHO[BLEN];  // local residual
   const int it = blockIdx.x;
   if (it >= nitems) return;
   const int tid = threadIdx.x;
@@ -277,6 +279,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_cnn_train_item(
     const int pz = p / 100, py = (p / 10) % 10, px = p % 10;
     X[p] = halo_value(S.res_h, nbr, b, pz, py, px) * inv_sr;
     X[PADC + p] = halo_value(S.prev, nbr, b, pz, py, px) * inv_sp;
+    OP[p] = halo_value(S.target, nbr, b, pz, py, px);
     const bool ring = pz == 0 || pz == 9 || py == 0 || py == 9 || px == 0 || px == 9;
     if (ring)
       for (int c = 0; c < CNN_C; ++c) {
@@ -331,10 +334,33 @@ __global__ __launch_bounds__(WG_THREADS) void k_cnn_train_item(
     const float d = a - T[m];
     lsum += d * d;
     D[m] = d * inv_ntot;
+    OP[padded(m)] = a;
   }
-  lred[tid] = lsum;
   __syncthreads();
-  if (tid == 0) {
+  if (lam != 0.f) {
+    // rho = lap7(out) - q, the stencil closed by the target halo outside
+    // the block; its gradient adds lam * (in-block L^T rho) / N to dL/dout
+    for (int m = tid; m < BLEN; m += WG_THREADS) {
+      const int P = padded(m);
+      const float r = ((OP[P - 1] + OP[P + 1]) + (OP[P - 10] + OP[P + 10]) +
+                       (OP[P - 100] + OP[P + 100])) - 6.f * OP[P] - X[P];
+      RHO[m] = r;
+      lsum += lam * r * r;
+    }
+    __syncthreads();
+    for (int m = tid; m < BLEN; m += WG_THREADS) {
+      const int z = m >> 6, y = (m >> 3) & 7, x = m & 7;
+      float gsum = -6.f * RHO[m];
+      if (x > 0) gsum += RHO[m - 1];
+      if (x < 7) gsum += RHO[m + 1];
+      if (y > 0) gsum += RHO[m - 8];
+      if (y < 7) gsum += RHO[m + 8];
+      if (z > 0) gsum += RHO[m - 64];
+      if (z < 7) gsum += RHO[m + 64];
+      D[m] += lam * gsum * inv_ntot;
+    }
+  }
+  lred[tid] = lsum;  if (tid == 0) {
     float s = 0.f;
     for (int i = 0; i < WG_THREADS; ++i) s += lred[i];
     lossbuf[it] = s;
@@ -599,16 +625,17 @@ static int upload_samples(CnnState& C, std::vector<SampleRef>& refs, SampleRef**
   return 0;
 }
 
-int cnn_train(CnnState& C, hipStream_t s, int steps, int batch, float lr, uint64_t seed,
-              double* loss, std::string& err) {
+int cnn_train(CnnState& C, hipStream_t s, int steps, int batch, float lr, float res_weight,
+              uint64_t seed, double* loss, std::string& err) {
   if (C.samples.empty()) {
     err = "cnn_train: empty training set";
     return -1;
   }
-  if (steps <= 0 || batch <= 0 || batch > 65536) {
-    err = "cnn_train: bad steps/batch";
+  if (steps <= 0 || batch <= 0 || batch > 65536 || !(res_weight >= 0.f)) {
+    err = "cnn_train: bad steps/batch/res_weight";
     return -1;
   }
+  C.res_weight = res_weight;
   if (!C.have_weights) {  // He-style initialisation from the seed
     std::mt19937_64 R(seed ^ 0x5eedULL);
     std::vector<float> w(CNN_NW, 0.f);
@@ -646,7 +673,8 @@ int cnn_train(CnnState& C, hipStream_t s, int steps, int batch, float lr, uint64
     CCHECK(hipMemcpyAsync(C.d_items, items.data(), items.size() * sizeof(int),
                           hipMemcpyHostToDevice, s));
     k_cnn_train_item<false><<<batch, WG_THREADS, 0, s>>>(batch, C.d_items, d_refs, C.d_nbr, C.d_w,
-                                                         scratch, gbuf, lbuf, inv_ntot);
+                                                         scratch, gbuf, lbuf, inv_ntot,
+                                                         C.res_weight);
     C.adam_t += 1;
     const float bc1 = 1.0f - std::pow(0.9f, (float)C.adam_t);
     const float bc2 = 1.0f - std::pow(0.999f, (float)C.adam_t);
@@ -692,7 +720,7 @@ int cnn_eval(CnnState& C, hipStream_t s, int k, double* loss, std::string& err) 
     }
     CCHECK(hipMemcpyAsync(C.d_items, items.data(), 2 * m * sizeof(int), hipMemcpyHostToDevice, s));
     k_cnn_train_item<true><<<m, WG_THREADS, 0, s>>>(m, C.d_items, d_refs, C.d_nbr, C.d_w,
-                                                    scratch, gbuf, lbuf, 0.f);
+                                                    scratch, gbuf, lbuf, 0.f, C.res_weight);
     CCHECK(hipMemcpyAsync(lh.data(), lbuf, m * sizeof(float), hipMemcpyDeviceToHost, s));
     CCHECK(hipStreamSynchronize(s));
     for (int i = 0; i < m; ++i) L += lh[i];
@@ -710,7 +738,8 @@ __global__ void k_cnn_sum_grads(int nitems, const float* __restrict__ gbuf, floa
   g[j] = a;
 }
 
-int cnn_gradient(CnnState& C, hipStream_t s, int k, float* grad, double* loss, std::string& err) {
+int cnn_gradient(CnnState& C, hipStream_t s, int k, float res_weight, float* grad, double* loss,
+                 std::string& err) {
   if (k < 0 || k >= (int)C.samples.size()) {
     err = "cnn_gradient: no such sample";
     return -1;
@@ -739,7 +768,8 @@ int cnn_gradient(CnnState& C, hipStream_t s, int k, float* grad, double* loss, s
   }
   CCHECK(hipMemcpyAsync(C.d_items, items.data(), 2 * m * sizeof(int), hipMemcpyHostToDevice, s));
   k_cnn_train_item<false><<<m, WG_THREADS, 0, s>>>(m, C.d_items, d_refs, C.d_nbr, C.d_w, scratch,
-                                                   gbuf, lbuf, 1.0f / ((float)m * BLEN));
+                                                   gbuf, lbuf, 1.0f / ((float)m * BLEN),
+                                                   res_weight);
   k_cnn_sum_grads<<<(CNN_NW + 255) / 256, 256, 0, s>>>(m, gbuf, gsum);
   std::vector<float> lh(m);
   CCHECK(hipMemcpyAsync(grad, gsum, CNN_NW * sizeof(float), hipMemcpyDeviceToHost, s));

@@ -43,6 +43,29 @@ __device__ inline void begin_iteration(SolverState* st, double rho, double rr) {
   st->iter += 1;
 }
 
+// Finalize kernels (one workgroup) work on a copy of the state in LDS: the
+// copy is loaded by the first threads together with the partial sums (one
+// memory round trip for both), thread 0 runs the scalar recurrences on it,
+// and the copy is written back in parallel.
+constexpr int STATE_WORDS = sizeof(SolverState) / 4;
+static_assert(sizeof(SolverState) % 4 == 0 && STATE_WORDS <= WG_THREADS, "state copy");
+
+template <int K>
+__device__ __forceinline__ void fin_load(const SolverState* st, SolverState& S,
+                                         const double* part, int nparts, double (&tot)[K]) {
+  if (threadIdx.x < STATE_WORDS)
+    reinterpret_cast<unsigned*>(&S)[threadIdx.x] =
+        reinterpret_cast<const unsigned*>(st)[threadIdx.x];
+  grid_sum_partials<K>(part, nparts, tot);  // ends with a barrier
+}
+
+__device__ __forceinline__ void fin_store(SolverState* st, const SolverState& S) {
+  __syncthreads();
+  if (threadIdx.x < STATE_WORDS)
+    reinterpret_cast<unsigned*>(st)[threadIdx.x] =
+        reinterpret_cast<const unsigned*>(&S)[threadIdx.x];
+}
+
 // Contiguous 8-element vector access (16-byte loads/stores).
 template <typename T>
 __device__ __forceinline__ void load8(const T* __restrict__ p, T (&v)[8]) {
@@ -102,13 +125,60 @@ __global__ __launch_bounds__(WG_THREADS) void k_getvec(long long n, long long ze
 
 static __global__ __launch_bounds__(WG_THREADS) void k_getvec_fin(SolverState* st, const double* part,
                                                            int nparts, double tol_b) {
+  __shared__ SolverState S;
   double tot[1];
-  grid_sum_partials<1>(part, nparts, tot);
+  fin_load<1>(st, S, part, nparts, tot);
   if (threadIdx.x == 0) {
-    st->bnorm2 = tot[0];
+    S.bnorm2 = tot[0];
     const double tb = tol_b * sqrt(tot[0]);
-    if (tb > st->tol_abs) st->tol_abs = tb;
+    if (tb > S.tol_abs) S.tol_abs = tb;
   }
+  fin_store(st, S);
+}
+
+// ---------------------------------------------------------------------------
+// Coarse-fine interface rows (the merged rows of PoissonSolverExp.cpp:
+// 167-217 for every cell that touches a coarse-fine face), one thread per
+// row over the whole grid: corr[64 c + lane] = sum_k val_k x[col_k] (FP64),
+// or sum_k val_k (x[col_k] - x_cell) in FP32, where the rows sum to zero and
+// differences keep the precision. Entries are taken 16 at a time, all loads
+// of a batch issued before the first use (most rows fit in one batch).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(WG_THREADS) void k_iface_rows(DevGrid<T> g, const T* __restrict__ x,
+                                                           const SolverState* st) {
+  constexpr int U = 16;
+  if (st && st->status != ST_RUNNING) return;
+  const int ch = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (ch >= g.nch) return;
+  const int meta = g.ell_meta[ch * 64 + lane];
+  const int K = g.ell_k[ch];  // longest row of the chunk (same in all lanes)
+  const int len = meta >> 16;
+  const size_t eo = (size_t)g.ell_off[ch] + lane;
+  const T xc = x[(size_t)g.ich_blk[ch] * BLEN + (meta & 0xffff)];
+  T acc = 0;
+  for (int k0 = 0; k0 < K; k0 += U) {
+    int ci[U];
+    T cv[U], xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool on = k0 + u < len;
+      ci[u] = on ? g.ell_col[eo + (size_t)(k0 + u) * 64] : 0;
+      cv[u] = on ? g.ell_val[eo + (size_t)(k0 + u) * 64] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) xv[u] = k0 + u < len ? x[ci[u]] : xc;
+    if (sizeof(T) == 8) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (k0 + u < len) acc += cv[u] * xv[u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (k0 + u < len) acc += cv[u] * (xv[u] - xc);
+    }
+  }
+  if (len > 0) g.corr[ch * 64 + lane] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -194,40 +264,21 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
     }
     __syncthreads();
     if (valid) {
-      // interface rows, one per lane, 64 rows per chunk; U entries of a row
-      // are loaded (index and weight, then the x values) before any is used
-      constexpr int U = sizeof(T) == 8 ? 8 : 16;
+      // interface-row sums of k_iface_rows (<= 296 rows = 5 chunks per block);
+      // every load is issued before the first add
+      constexpr int MAXCH = 5;
       const int c0 = g.ich_ptr[b], c1 = g.ich_ptr[b + 1];
-      for (int ch = c0; ch < c1; ++ch) {
-        const int meta = g.ell_meta[ch * 64 + lane];
-        const int K = g.ell_k[ch];
-        const size_t eo = (size_t)g.ell_off[ch] + lane;
-        const int cl = meta & 0xffff, len = meta >> 16;
-        const T xc = xs[w][cl];
-        T acc = 0;
-        for (int k0 = 0; k0 < K; k0 += U) {
-          int ci[U];
-          T cv[U], xv[U];
+      int meta[MAXCH];
+      T cs[MAXCH];
 #pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const bool on = k0 + u < len;
-            ci[u] = on ? g.ell_col[eo + (size_t)(k0 + u) * 64] : 0;
-            cv[u] = on ? g.ell_val[eo + (size_t)(k0 + u) * 64] : (T)0;
-          }
-#pragma unroll
-          for (int u = 0; u < U; ++u) xv[u] = k0 + u < len ? x[ci[u]] : xc;
-          if (sizeof(T) == 8) {
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-              if (k0 + u < len) acc += cv[u] * xv[u];
-          } else {  // interface rows sum to zero: use differences to the cell value
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-              if (k0 + u < len) acc += cv[u] * (xv[u] - xc);
-          }
-        }
-        if (len > 0) ys[w][cl] += acc;
+      for (int j = 0; j < MAXCH; ++j) {
+        const bool on = c0 + j < c1;
+        meta[j] = on ? g.ell_meta[(c0 + j) * 64 + lane] : 0;
+        cs[j] = on ? g.corr[(c0 + j) * 64 + lane] : (T)0;
       }
+#pragma unroll
+      for (int j = 0; j < MAXCH; ++j)
+        if ((meta[j] >> 16) > 0) ys[w][meta[j] & 0xffff] += cs[j];
     }
     __syncthreads();
     if (valid) {
@@ -283,56 +334,59 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv_fin(long long row, T* __res
                                                          T* __restrict__ c, T* __restrict__ c2,
                                                          SolverState* st, const double* part,
                                                          int nparts) {
-  if (MODE != SPMV_PLAIN && MODE != SPMV_RESID && st->status != ST_RUNNING) return;
+  __shared__ SolverState S;
   double tot[3];
-  grid_sum_partials<3>(part, nparts, tot);
-  if (threadIdx.x != 0) return;
-  // bMean = 1: y[row] = sum_b h_b^3 sum x  (BiCGSTAB.cu:406-407)
-  if (BM == 1 && row >= 0) {
-    const T yr = (T)tot[0];
-    if (MODE == SPMV_PLAIN) {
-      y[row] = yr;
-    } else if (MODE == SPMV_INIT) {
-      y[row] = yr;
-      const T rr = a[row] - yr;
-      c[row] = rr;
-      c2[row] = rr;
-      tot[1] += (double)yr * (double)yr;
-      tot[2] += (double)rr * (double)rr;
+  fin_load<3>(st, S, part, nparts, tot);
+  if (MODE != SPMV_PLAIN && MODE != SPMV_RESID && S.status != ST_RUNNING) return;
+  if (threadIdx.x == 0) {
+    // bMean = 1: y[row] = sum_b h_b^3 sum x  (BiCGSTAB.cu:406-407)
+    if (BM == 1 && row >= 0) {
+      const T yr = (T)tot[0];
+      if (MODE == SPMV_PLAIN) {
+        y[row] = yr;
+      } else if (MODE == SPMV_INIT) {
+        y[row] = yr;
+        const T rr = a[row] - yr;
+        c[row] = rr;
+        c2[row] = rr;
+        tot[1] += (double)yr * (double)yr;
+        tot[2] += (double)rr * (double)rr;
+      } else if (MODE == SPMV_DOT1) {
+        y[row] = yr;
+        tot[1] += (double)a[row] * (double)yr;
+      } else if (MODE == SPMV_DOT2) {
+        y[row] = yr;
+        tot[1] += (double)a[row] * (double)yr;
+        tot[2] += (double)yr * (double)yr;
+      } else if (MODE == SPMV_RESID) {
+        const T rr = a[row] - yr;
+        c[row] = rr;
+        tot[1] += (double)rr * (double)rr;
+      }
+    }
+    S.mean_sum = tot[0];
+    if (MODE == SPMV_INIT) {
+      // initial residual (BiCGSTAB.cu:435-471)
+      S.nu2 = tot[1];
+      S.err = sqrt(tot[2]);
+      S.err_init = S.err;
+      S.err_opt = S.err;
+      S.rhrh = tot[2];
+      begin_iteration(&S, tot[2], tot[2]);
     } else if (MODE == SPMV_DOT1) {
-      y[row] = yr;
-      tot[1] += (double)a[row] * (double)yr;
+      S.dot_a = tot[1];
+      S.alpha = S.rho_curr / (tot[1] + S.eps);  // set_alpha (:276-279)
+      S.p_zero = 0;
+      S.restart_pending = 0;
     } else if (MODE == SPMV_DOT2) {
-      y[row] = yr;
-      tot[1] += (double)a[row] * (double)yr;
-      tot[2] += (double)yr * (double)yr;
+      S.dot_ts = tot[1];
+      S.dot_tt = tot[2];
+      S.omega = tot[1] / (tot[2] + S.eps);  // set_omega (:281-284)
     } else if (MODE == SPMV_RESID) {
-      const T rr = a[row] - yr;
-      c[row] = rr;
-      tot[1] += (double)rr * (double)rr;
+      S.true_rr = tot[1];
     }
   }
-  st->mean_sum = tot[0];
-  if (MODE == SPMV_INIT) {
-    // initial residual (BiCGSTAB.cu:435-471)
-    st->nu2 = tot[1];
-    st->err = sqrt(tot[2]);
-    st->err_init = st->err;
-    st->err_opt = st->err;
-    st->rhrh = tot[2];
-    begin_iteration(st, tot[2], tot[2]);
-  } else if (MODE == SPMV_DOT1) {
-    st->dot_a = tot[1];
-    st->alpha = st->rho_curr / (tot[1] + st->eps);  // set_alpha (:276-279)
-    st->p_zero = 0;
-    st->restart_pending = 0;
-  } else if (MODE == SPMV_DOT2) {
-    st->dot_ts = tot[1];
-    st->dot_tt = tot[2];
-    st->omega = tot[1] / (tot[2] + st->eps);  // set_omega (:281-284)
-  } else if (MODE == SPMV_RESID) {
-    st->true_rr = tot[1];
-  }
+  fin_store(st, S);
 }
 
 // Launch for a grid: picks the bMean and interface variants.
@@ -340,10 +394,13 @@ template <typename T, int MODE, int BM>
 inline void spmv_launch_iface(hipStream_t s, const DevGrid<T>& g, const T* x, T* y, const T* a,
                               T* c, T* c2, SolverState* st, double* part) {
   const int nwg = (g.nb + 3) / 4;
-  if (g.has_iface)
+  if (g.has_iface) {
+    const bool gated = MODE != SPMV_PLAIN && MODE != SPMV_RESID;
+    k_iface_rows<T><<<(g.nch + 3) / 4, WG_THREADS, 0, s>>>(g, x, gated ? st : nullptr);
     k_spmv<T, MODE, BM, true><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
-  else
+  } else {
     k_spmv<T, MODE, BM, false><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
+  }
   k_spmv_fin<T, MODE, BM><<<1, WG_THREADS, 0, s>>>(g.bmean_row, y, a, c, c2, st, part, nwg);
 }
 template <typename T, int MODE>
@@ -397,31 +454,38 @@ __global__ __launch_bounds__(WG_THREADS) void k_update(long long n, KrylovVector
 
 // Finalize of k_update (one workgroup): |r|, x_opt, convergence test, restart
 // and the next iteration's scalars (BiCGSTAB.cu:580-607).
-static __global__ __launch_bounds__(WG_THREADS) void k_update_fin(SolverState* st, const double* part,
-                                                           int nparts) {
-  if (st->status != ST_RUNNING) return;
-  double tot[2];
-  grid_sum_partials<2>(part, nparts, tot);
-  if (threadIdx.x != 0) return;
-  const int xn = third_buffer(st->xcur, st->xopt);
-  const double err = sqrt(tot[1]);
-  st->err = err;
-  st->iters_done = st->iter;
-  st->xcur = xn;
+// End of an iteration (BiCGSTAB.cu:584-607): the residual norm, x_opt,
+// convergence, then the start of the next iteration.
+__device__ inline void update_scalars(SolverState& S, double rho, double rr) {
+  const int xn = third_buffer(S.xcur, S.xopt);
+  const double err = sqrt(rr);
+  S.err = err;
+  S.iters_done = S.iter;
+  S.xcur = xn;
   if (!(err == err) || err > 1.7e308) {
-    st->status = ST_NONFINITE;
+    S.status = ST_NONFINITE;
     return;
   }
-  if (err < st->err_opt) {
-    st->err_opt = err;
-    st->xopt = xn;
-    if (err <= st->tol_abs || err / st->err_init <= st->tol_rel) {
-      st->status = ST_CONVERGED;
+  if (err < S.err_opt) {
+    S.err_opt = err;
+    S.xopt = xn;
+    if (err <= S.tol_abs || err / S.err_init <= S.tol_rel) {
+      S.status = ST_CONVERGED;
       return;
     }
   }
-  st->rho_prev = st->rho_curr;  // set_rho (:607)
-  begin_iteration(st, tot[0], tot[1]);
+  S.rho_prev = S.rho_curr;  // set_rho (:607)
+  begin_iteration(&S, rho, rr);
+}
+
+static __global__ __launch_bounds__(WG_THREADS) void k_update_fin(SolverState* st, const double* part,
+                                                           int nparts) {
+  __shared__ SolverState S;
+  double tot[2];
+  fin_load<2>(st, S, part, nparts, tot);
+  if (S.status != ST_RUNNING) return;
+  if (threadIdx.x == 0) update_scalars(S, tot[0], tot[1]);
+  fin_store(st, S);
 }
 
 // Copy the best iterate out: x_user = X[xopt] (PoissonSolverExp.cpp:452-472

@@ -265,15 +265,18 @@ __global__ __launch_bounds__(WG_THREADS) void k_inner_init(long long n, const do
 
 __global__ __launch_bounds__(WG_THREADS) void k_inner_init_fin(SolverState* st, const double* part,
                                                                int nparts) {
+  __shared__ SolverState S;
   double tot[1];
-  grid_sum_partials<1>(part, nparts, tot);
-  if (threadIdx.x != 0) return;
-  st->nu2 = 0.0;
-  st->err = sqrt(tot[0]);
-  st->err_init = st->err;
-  st->err_opt = st->err;
-  st->rhrh = tot[0];
-  begin_iteration(st, tot[0], tot[0]);
+  fin_load<1>(st, S, part, nparts, tot);
+  if (threadIdx.x == 0) {
+    S.nu2 = 0.0;
+    S.err = sqrt(tot[0]);
+    S.err_init = S.err;
+    S.err_opt = S.err;
+    S.rhrh = tot[0];
+    begin_iteration(&S, tot[0], tot[0]);
+  }
+  fin_store(st, S);
 }
 
 // x64 += scale * (double) x32[xopt]   (refinement update, FP64)

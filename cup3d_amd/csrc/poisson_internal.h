@@ -85,6 +85,9 @@ struct DevGrid {
   // ell_meta[64 c + lane] = cell | len << 16 (len 0: no row) and its entries
   // slot-major at ell_off[c] + 64 k + lane, k < len <= ell_k[c]
   const int* ich_ptr = nullptr;     // [nb+1]
+  const int* ich_blk = nullptr;     // [nchunks] block of the chunk
+  int nch = 0;                      // nchunks
+  T* corr = nullptr;                // [nchunks * 64] interface-row sums of the current SpMV
   const int* ell_meta = nullptr;    // [nchunks * 64]
   const int* ell_off = nullptr;     // [nchunks]
   const int* ell_k = nullptr;       // [nchunks]
@@ -103,7 +106,7 @@ struct HostTopology {
   std::vector<double> h, invh, h3;
   std::vector<int> irow_ptr, irow_cell, irow_ent, ent_col;  // CSR form of the interface rows
   std::vector<double> ent_val;
-  std::vector<int> ich_ptr, ell_meta, ell_off, ell_k, ell_col;  // device (sliced ELL) form
+  std::vector<int> ich_ptr, ich_blk, ell_meta, ell_off, ell_k, ell_col;  // device (sliced ELL) form
   std::vector<double> ell_val;
   long long bmean_row = -1;
   int origin_block = -1;

@@ -74,8 +74,8 @@ struct cup3d_poisson {
   std::vector<int> blocks;  // nb x 4
   int nb = 0;
   long long n = 0;
-  DevBuf d_nbr, d_h, d_invh, d_h3, d_hf, d_invhf, d_ich_ptr, d_ell_meta, d_ell_off, d_ell_k,
-      d_ell_col, d_ell_val, d_ell_valf, d_blocks;
+  DevBuf d_nbr, d_h, d_invh, d_h3, d_hf, d_invhf, d_ich_ptr, d_ich_blk, d_ell_meta, d_ell_off,
+      d_ell_k, d_ell_col, d_ell_val, d_ell_valf, d_corr64, d_corr32, d_blocks;
   DevGrid<double> g64;
   DevGrid<float> g32;
 
@@ -252,6 +252,7 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
       upload(H, H->d_hf, hf.data(), hf.size() * sizeof(float)) ||
       upload(H, H->d_invhf, invhf.data(), invhf.size() * sizeof(float)) ||
       upload(H, H->d_ich_ptr, T.ich_ptr.data(), T.ich_ptr.size() * sizeof(int)) ||
+      upload(H, H->d_ich_blk, T.ich_blk.data(), T.ich_blk.size() * sizeof(int)) ||
       upload(H, H->d_ell_meta, T.ell_meta.data(), T.ell_meta.size() * sizeof(int)) ||
       upload(H, H->d_ell_off, T.ell_off.data(), T.ell_off.size() * sizeof(int)) ||
       upload(H, H->d_ell_k, T.ell_k.data(), T.ell_k.size() * sizeof(int)) ||
@@ -261,7 +262,11 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
       upload(H, H->d_blocks, H->blocks.data(), H->blocks.size() * sizeof(int)))
     return -1;
   const long long origin_row = T.origin_block >= 0 ? (long long)T.origin_block * BLEN : -1;
-  auto fill = [&](auto& g, const void* h, const void* invh, const void* val) {
+  const size_t nslots = T.ich_blk.size() * 64;
+  if (H->d_corr64.alloc(nslots * sizeof(double)) != hipSuccess ||
+      H->d_corr32.alloc(nslots * sizeof(float)) != hipSuccess)
+    return fail(H, "hipMalloc(interface sums) failed");
+  auto fill = [&](auto& g, const void* h, const void* invh, const void* val, void* corr) {
     using G = std::remove_reference_t<decltype(g)>;
     using TT = std::remove_const_t<std::remove_pointer_t<decltype(G::h)>>;
     g.nb = H->nb;
@@ -271,6 +276,9 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
     g.invh = static_cast<const TT*>(invh);
     g.h3 = H->d_h3.as<double>();
     g.ich_ptr = H->d_ich_ptr.as<int>();
+    g.ich_blk = H->d_ich_blk.as<int>();
+    g.nch = (int)T.ich_blk.size();
+    g.corr = static_cast<TT*>(corr);
     g.ell_meta = H->d_ell_meta.as<int>();
     g.ell_off = H->d_ell_off.as<int>();
     g.ell_k = H->d_ell_k.as<int>();
@@ -280,8 +288,8 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
     g.has_iface = T.irow_cell.empty() ? 0 : 1;
     g.bmean_row = (H->cfg.bmean == 1 || H->cfg.bmean == 3) ? origin_row : -1;
   };
-  fill(H->g64, H->d_h.p, H->d_invh.p, H->d_ell_val.p);
-  fill(H->g32, H->d_hf.p, H->d_invhf.p, H->d_ell_valf.p);
+  fill(H->g64, H->d_h.p, H->d_invh.p, H->d_ell_val.p, H->d_corr64.p);
+  fill(H->g32, H->d_hf.p, H->d_invhf.p, H->d_ell_valf.p, H->d_corr32.p);
   // FP64 Krylov vectors: x[3], r, rhat, p, nu, t, z1, z2, b
   const size_t n = (size_t)H->n;
   if (H->d_vec64.alloc(11 * n * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(vectors) failed");
@@ -554,11 +562,11 @@ int cup3d_poisson_cnn_num_samples(cup3d_poisson_t H) {
   return H ? (int)H->cnn.samples.size() : -1;
 }
 
-int cup3d_poisson_cnn_train(cup3d_poisson_t H, int steps, int batch, float lr, uint64_t seed,
-                            double* loss) {
+int cup3d_poisson_cnn_train(cup3d_poisson_t H, int steps, int batch, float lr, float res_weight,
+                            uint64_t seed, double* loss) {
   if (!H) return -1;
   HIPCHECK(hipSetDevice(H->device));
-  return cnn_train(H->cnn, H->stream, steps, batch, lr, seed, loss, H->err);
+  return cnn_train(H->cnn, H->stream, steps, batch, lr, res_weight, seed, loss, H->err);
 }
 
 int cup3d_poisson_cnn_eval(cup3d_poisson_t H, int k, double* loss) {
@@ -567,10 +575,11 @@ int cup3d_poisson_cnn_eval(cup3d_poisson_t H, int k, double* loss) {
   return cnn_eval(H->cnn, H->stream, k, loss, H->err);
 }
 
-int cup3d_debug_cnn_gradient(cup3d_poisson_t H, int k, float* grad, double* loss) {
+int cup3d_debug_cnn_gradient(cup3d_poisson_t H, int k, float res_weight, float* grad,
+                             double* loss) {
   if (!H || !grad || !loss) return -1;
   HIPCHECK(hipSetDevice(H->device));
-  return cnn_gradient(H->cnn, H->stream, k, grad, loss, H->err);
+  return cnn_gradient(H->cnn, H->stream, k, res_weight, grad, loss, H->err);
 }
 
 int cup3d_debug_time_kernel(cup3d_poisson_t H, int which, int reps, double* avg_ms,

@@ -89,6 +89,7 @@ void taylor_patch(int k1, int k2, double s1, double s2, double W[5][5]) {
 static void build_interface_ell(HostTopology& T) {
   const int nb = T.nb;
   T.ich_ptr.assign(nb + 1, 0);
+  T.ich_blk.clear();
   T.ell_meta.clear();
   T.ell_off.clear();
   T.ell_k.clear();
@@ -110,6 +111,7 @@ static void build_interface_ell(HostTopology& T) {
       const size_t off = T.ell_col.size();
       T.ell_off.push_back((int)off);
       T.ell_k.push_back(K);
+      T.ich_blk.push_back(b);
       T.ell_col.resize(off + (size_t)K * 64, 0);
       T.ell_val.resize(off + (size_t)K * 64, 0.0);
       for (size_t j = 0; j < 64; ++j) {
@@ -127,6 +129,13 @@ static void build_interface_ell(HostTopology& T) {
     }
     T.ich_ptr[b + 1] = (int)T.ell_k.size();
   }
+}
+
+// The SpMV adds at most 5 chunks (296 surface cells) per block.
+static bool interface_chunks_fit(const HostTopology& T) {
+  for (int b = 0; b < T.nb; ++b)
+    if (T.ich_ptr[b + 1] - T.ich_ptr[b] > 5) return false;
+  return true;
 }
 
 bool build_topology(const TopologyInput& in, HostTopology& T, std::string& err) {
@@ -318,6 +327,10 @@ bool build_topology(const TopologyInput& in, HostTopology& T, std::string& err) 
     T.irow_ptr[b + 1] = (int)T.irow_cell.size();
   }
   build_interface_ell(T);
+  if (!interface_chunks_fit(T)) {
+    err = "more than 296 interface rows in one block";
+    return false;
+  }
   if (T.ell_col.size() > 0x7fffffffULL) {
     err = "grid too large for 32-bit interface table offsets";
     return false;

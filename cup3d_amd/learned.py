@@ -14,7 +14,8 @@ residual, so a worse net costs one inference and two residuals, never
 accuracy.
 
   python -m cup3d_amd.learned [--steps 400] [--train 300] [--fresh 40]
-                              [--train-steps 3000] [--batch 256] [--bpd 32 16 16]
+                              [--train-steps 3000] [--batch 256] [--res-weight 1]
+                              [--bpd 32 16 16]
 
 Prints one JSON line.
 """
@@ -29,7 +30,7 @@ import numpy as np
 
 
 def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=3000, batch=256,
-        lr=1e-3, seed=1, tol=1e-7, log=sys.stderr):
+        lr=1e-3, res_weight=1.0, seed=1, tol=1e-7, log=sys.stderr):
     import torch
 
     from . import workloads as W
@@ -62,8 +63,9 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=3000, batc
     for t in range(1, train + 1):
         ref.cnn_add_sample(rhs[t], sols[t - 1], sols[t])
     t0 = time.time()
-    l0, _ = ref.cnn_train(1, batch=batch, lr=lr, seed=seed)
-    _, l1 = ref.cnn_train(train_steps - 1, batch=batch, lr=lr, seed=seed + 1)
+    l0, _ = ref.cnn_train(1, batch=batch, lr=lr, seed=seed, res_weight=res_weight)
+    _, l1 = ref.cnn_train(train_steps - 1, batch=batch, lr=lr, seed=seed + 1,
+                          res_weight=res_weight)
     train_s = time.time() - t0
     print(f"training: {train_steps} steps of {batch} blocks, loss {l0:.3e} -> {l1:.3e}, "
           f"{train_s:.1f} s", file=log, flush=True)
@@ -103,7 +105,7 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=3000, batc
     fr = judge(frhs, fsols, range(1, fresh + 1))
     return {
         "workload": w.name, "cells": w.ncells, "steps": steps, "train_triples": train,
-        "train_steps": train_steps, "batch_blocks": batch, "lr": lr,
+        "train_steps": train_steps, "batch_blocks": batch, "lr": lr, "res_weight": res_weight,
         "loss_first": l0, "loss_last": l1, "train_seconds": round(train_s, 1),
         "fp64_iterations_mean_train": float(np.mean(its[1:train + 1])),
         "held_out": held, "fresh_seed": fr,
@@ -118,11 +120,14 @@ def main():
     p.add_argument("--train-steps", type=int, default=3000)
     p.add_argument("--batch", type=int, default=256)
     p.add_argument("--lr", type=float, default=1e-3)
+    p.add_argument("--res-weight", type=float, default=1.0,
+                   help="weight of the local-residual term of the training loss")
     p.add_argument("--bpd", type=int, nargs=3, default=[32, 16, 16])
     a = p.parse_args()
     if not 0 < a.train < a.steps:
         raise SystemExit("need 0 < --train < --steps")
-    print(json.dumps(run(a.bpd, a.steps, a.train, a.fresh, a.train_steps, a.batch, a.lr)))
+    print(json.dumps(run(a.bpd, a.steps, a.train, a.fresh, a.train_steps, a.batch, a.lr,
+                         a.res_weight)))
 
 
 if __name__ == "__main__":

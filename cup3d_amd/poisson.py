@@ -79,7 +79,8 @@ def lib():
     L.cup3d_poisson_cnn_clear_samples.argtypes = [_vp]
     L.cup3d_poisson_cnn_num_samples.argtypes = [_vp]
     L.cup3d_poisson_cnn_train.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
-                                          ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+                                          ctypes.c_float, ctypes.c_uint64,
+                                          ctypes.POINTER(ctypes.c_double)]
     L.cup3d_poisson_cnn_eval.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.cup3d_poisson_num_interface_rows.argtypes = [_vp]
     L.cup3d_poisson_num_interface_rows.restype = ctypes.c_int64
@@ -236,10 +237,10 @@ class PoissonSolver:
     def cnn_num_samples(self):
         return self.L.cup3d_poisson_cnn_num_samples(self.h)
 
-    def cnn_train(self, steps, batch=256, lr=1e-3, seed=0):
+    def cnn_train(self, steps, batch=256, lr=1e-3, seed=0, res_weight=0.0):
         loss = (ctypes.c_double * 2)()
-        if self.L.cup3d_poisson_cnn_train(self.h, int(steps), int(batch), float(lr), int(seed),
-                                          loss) != 0:
+        if self.L.cup3d_poisson_cnn_train(self.h, int(steps), int(batch), float(lr),
+                                          float(res_weight), int(seed), loss) != 0:
             raise self._err("cnn_train")
         return loss[0], loss[1]
 

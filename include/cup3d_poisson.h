@@ -159,12 +159,15 @@ int cup3d_poisson_cnn_add_sample(cup3d_poisson_t h, const double* d_rhs, const d
 int cup3d_poisson_cnn_clear_samples(cup3d_poisson_t h);
 int cup3d_poisson_cnn_num_samples(cup3d_poisson_t h);
 /* `steps` Adam steps on minibatches of `batch` random (sample, block) items
- * drawn with `seed`; loss = 0.5 mean (net - target)^2 in normalised units.
- * loss[0] = first minibatch loss, loss[1] = last. */
-int cup3d_poisson_cnn_train(cup3d_poisson_t h, int steps, int batch, float lr, uint64_t seed,
-                            double* loss);
+ * drawn with `seed`. Loss in normalised units:
+ *   0.5 mean (net - target)^2 + res_weight * 0.5 mean rho^2,
+ * rho = 7-point Laplacian of the net's increment minus res / (h s_r) (the
+ * local residual of the increment equation; the target closes the stencil
+ * outside the block). loss[0] = first minibatch loss, loss[1] = last. */
+int cup3d_poisson_cnn_train(cup3d_poisson_t h, int steps, int batch, float lr, float res_weight,
+                            uint64_t seed, double* loss);
 /* Mean loss of the current weights over every block of training sample k
- * (FP32 forward), for validation. */
+ * (FP32 forward, res_weight of the last training), for validation. */
 int cup3d_poisson_cnn_eval(cup3d_poisson_t h, int k, double* loss);
 
 /* Introspection for tests and reports. */

@@ -42,7 +42,8 @@ int cup3d_debug_time_kernel(cup3d_poisson_t h, int which, int reps, double* avg_
 /* Full-batch FP32 gradient (n_weights floats) and loss of the learned warm
  * start over every block of training sample k (grids up to 4096 blocks), for
  * checking the backward kernels against an FP64 reference. */
-int cup3d_debug_cnn_gradient(cup3d_poisson_t h, int k, float* grad, double* loss);
+int cup3d_debug_cnn_gradient(cup3d_poisson_t h, int k, float res_weight, float* grad,
+                             double* loss);
 
 #ifdef __cplusplus
 }

@@ -132,13 +132,43 @@ def forward(w, x):
     return out[:, 0], (x, z1, h1, z2, h2)
 
 
-def loss_and_grad(w, x, target):
-    """Mean squared error 0.5 * mean((out - target)^2) and its gradient."""
+def local_residual(out, q, target_halo):
+    """rho = (7-point Laplacian of out) - q in every cell of every block, the
+    stencil closed outside the block by the target's halo [nb,10,10,10]. With
+    out = delta / s_r and q = res / (h s_r), rho = 0 is A delta = res."""
+    o = np.array(target_halo, np.float64)
+    o[:, 1:9, 1:9, 1:9] = out
+    lap = (o[:, :8, 1:9, 1:9] + o[:, 2:, 1:9, 1:9] + o[:, 1:9, :8, 1:9] + o[:, 1:9, 2:, 1:9] +
+           o[:, 1:9, 1:9, :8] + o[:, 1:9, 1:9, 2:] - 6.0 * out)
+    return lap - q
+
+
+def lap_transpose_in_block(rho):
+    """Adjoint of the in-block part of the Laplacian: sum of rho over the
+    in-block neighbours of each cell minus 6 rho."""
+    g = -6.0 * rho
+    g[:, 1:] += rho[:, :-1]
+    g[:, :-1] += rho[:, 1:]
+    g[:, :, 1:] += rho[:, :, :-1]
+    g[:, :, :-1] += rho[:, :, 1:]
+    g[:, :, :, 1:] += rho[:, :, :, :-1]
+    g[:, :, :, :-1] += rho[:, :, :, 1:]
+    return g
+
+
+def loss_and_grad(w, x, target, lam=0.0, target_halo=None):
+    """Loss 0.5 mean((out - target)^2) + lam 0.5 mean(rho^2), rho the local
+    residual of the prediction (local_residual; needs target_halo when
+    lam != 0), and its gradient."""
     W1, b1, W2, b2, W3, b3 = unpack(w)
     out, (x, z1, h1, z2, h2) = forward(w, x)
     n = out.size
     d = (out - target) / n
     loss = 0.5 * float(np.sum((out - target) ** 2)) / n
+    if lam != 0.0:
+        rho = local_residual(out, x[:, 0, 1:9, 1:9, 1:9], target_halo)
+        loss += lam * 0.5 * float(np.sum(rho ** 2)) / n
+        d = d + lam * lap_transpose_in_block(rho) / n
     dout = d[:, None]
 
     def wgrad(inp_padded, dz_):

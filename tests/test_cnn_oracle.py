@@ -46,6 +46,38 @@ def test_gradient_matches_finite_differences():
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
 
 
+def test_residual_loss_gradient_matches_finite_differences():
+    rng = np.random.default_rng(5)
+    w = C.init_weights(4) + 0.01 * rng.standard_normal(C.NW)
+    x = rng.standard_normal((2, 2, 10, 10, 10))
+    target = rng.standard_normal((2, 8, 8, 8))
+    th = rng.standard_normal((2, 10, 10, 10))
+    th[:, 1:9, 1:9, 1:9] = target
+    _, g = C.loss_and_grad(w, x, target, 0.7, th)
+    for i in list(rng.choice(C.NW, 16, replace=False)) + [8240, 8100]:
+        e = np.zeros(C.NW)
+        e[i] = 1e-5
+        lp, _ = C.loss_and_grad(w + e, x, target, 0.7, th)
+        lm, _ = C.loss_and_grad(w - e, x, target, 0.7, th)
+        fd = (lp - lm) / 2e-5
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
+
+
+def test_local_residual_vanishes_on_exact_increment():
+    """A delta solved exactly on a periodic 8^3 box (one block, halo from its
+    own periodic copy) has rho = 0: the loss term is the discrete equation."""
+    rng = np.random.default_rng(6)
+    d = rng.standard_normal((8, 8, 8))
+    d -= d.mean()
+    # q = 7-point Laplacian of d with periodic wrap
+    lap = sum(np.roll(d, s, a) for a in range(3) for s in (1, -1)) - 6 * d
+    th = np.pad(d, 1, mode="wrap")[None]
+    rho = C.local_residual(d[None], lap[None], th)
+    assert np.abs(rho).max() < 1e-12
+    rho2 = C.local_residual(d[None] + 1e-3, lap[None], th)
+    assert np.abs(rho2).max() > 1e-4   # a shift inside the block shows at its faces
+
+
 def test_delta_input_gives_kernel():
     """Closed form: with GELU replaced by identity (linear layer 1 only), a
     delta input returns the flipped kernel — checked on layer 1 directly."""

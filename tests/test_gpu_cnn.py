@@ -74,7 +74,8 @@ def test_predict_matches_oracle(gpu, name):
     assert rel < 1e-2, rel
 
 
-def test_gradient_matches_oracle(gpu):
+@pytest.mark.parametrize("lam", [0.0, 0.6])
+def test_gradient_matches_oracle(gpu, lam):
     blocks, bpd, h0, bc = grid_cases()["amr2_periodic_offcentre"]
     S = PoissonSolver(bpd, h0, bc, bmean=0).set_topology(blocks)
     w = C.init_weights(7)
@@ -85,14 +86,16 @@ def test_gradient_matches_oracle(gpu):
     S.cnn_add_sample(dev(rhs), dev(prev), dev(p))
     L = lib()
     f = L.cup3d_debug_cnn_gradient
-    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
+                  ctypes.POINTER(ctypes.c_double)]
     g = np.zeros(C.NW, np.float32)
     loss = ctypes.c_double()
-    assert f(S.h, 0, g.ctypes.data, ctypes.byref(loss)) == 0
+    assert f(S.h, 0, lam, g.ctypes.data, ctypes.byref(loss)) == 0
     res = residual(blocks, bpd, h0, bc, rhs, prev)
     x, s_r, _ = C.inputs(res, prev, h, nbr_table(blocks, bpd, bc))
     target = (p - prev) / s_r
-    lo, go = C.loss_and_grad(w.astype(np.float32).astype(np.float64), x, target)
+    th = C.halo(target, nbr_table(blocks, bpd, bc))
+    lo, go = C.loss_and_grad(w.astype(np.float32).astype(np.float64), x, target, lam, th)
     assert abs(loss.value - lo) < 1e-4 * lo
     assert np.linalg.norm(g - go) < 1e-3 * np.linalg.norm(go)
 
