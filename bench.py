@@ -110,6 +110,32 @@ def kernel_times(S, which_list, reps=20):
     return out
 
 
+def copy_ceiling(dev, nbytes=1 << 30, reps=10):
+    """Device-to-device copy rate (read + write bytes / s), the practical HBM
+    ceiling next to the 8 TB/s datasheet peak."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    return round(2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+
+
+def host_solve_ms(S, rhs, p_prev):
+    """One solve through the host entry point (H2D of rhs and guess, D2H of the
+    pressure over PCIe included)."""
+    r = rhs.cpu().numpy()
+    x = p_prev.cpu().numpy().copy()
+    t0 = time.perf_counter()
+    rep = S.solve_host(r, x)
+    return round((time.perf_counter() - t0) * 1e3, 3), rep["status"]
+
+
 def cpu_baseline(w, bmean, rhs, x0, gpu_iters, iters):
     """Reference CPU algorithm (pipelined BiCGSTAB + per-block CG), restated in
     oracle/oracle.cpp and built with the reference's release flags; timed for
@@ -191,6 +217,7 @@ def main():
         b = b - (hw * b).sum() / hw.sum()
         return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
 
+    host_ms, host_status = host_solve_ms(S, rhs, p_prev)
     kern = kernel_times(S, [2, 3, 4])
     dom = max(kern, key=lambda k: kern[k]["avg_us"] * CALLS_PER_ITER[k])
     kd = kern[dom]
@@ -232,6 +259,9 @@ def main():
                  "gemm_mfma_frac": round(g64["TFLOP/s"] / FP64_MFMA_PEAK, 4),
                  "kernels_us": {KERNELS[k]: round(v["avg_us"], 2) for k, v in k64.items()}},
         "speedup_vs_fp64": round(ms64 / ms, 3),
+        "host_to_host": {"ms_per_solve": host_ms, "status": host_status,
+                         "note": "cup3d_poisson_solve from host arrays, PCIe copies included"},
+        "hbm_copy_GB/s": copy_ceiling(dev),
         "kernels": {KERNELS[k]: {kk: round(vv, 3) for kk, vv in v.items()}
                     for k, v in kern.items()},
         "roofline": roof,

@@ -97,6 +97,31 @@ __global__ __launch_bounds__(WG_THREADS) void k_cnn_rms_fin(long long n, double*
   }
 }
 
+// scal[2] = volume-weighted mean of p - p_prev: sum h^3 (p - p_prev) / sum h^3
+__global__ __launch_bounds__(WG_THREADS) void k_cnn_dmean(long long n, const double* __restrict__ p,
+                                                          const double* __restrict__ prev,
+                                                          const double* __restrict__ h,
+                                                          double* part) {
+  double v[2] = {0.0, 0.0};
+  const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
+  for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
+    const long long i = base + e;
+    if (i < n) {
+      const double hb = h[i / BLEN], h3 = hb * hb * hb;
+      v[0] += h3 * (p[i] - prev[i]);
+      v[1] += h3;
+    }
+  }
+  grid_store_partial<2>(v, part);
+}
+
+__global__ __launch_bounds__(WG_THREADS) void k_cnn_dmean_fin(double* scal, const double* part,
+                                                              int nparts) {
+  double tot[2];
+  grid_sum_partials<2>(part, nparts, tot);
+  if (threadIdx.x == 0) scal[2] = tot[1] > 0 ? tot[0] / tot[1] : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // FP16 weight packing for the implicit-GEMM inference.
 // Fragment kt, lane l (n = l % 16 output channel, g = l / 16), element e:
@@ -360,7 +385,9 @@ __global__ __launch_bounds__(WG_THREADS) void k_cnn_train_item(
       D[m] += lam * gsum * inv_ntot;
     }
   }
-  lred[tid] = lsum;  if (tid == 0) {
+  lred[tid] = lsum;
+  __syncthreads();  // lred and D complete before the sum and the backward pass
+  if (tid == 0) {
     float s = 0.f;
     for (int i = 0; i < WG_THREADS; ++i) s += lred[i];
     lossbuf[it] = s;
@@ -459,18 +486,21 @@ __global__ void k_cnn_adam(int nitems, const float* __restrict__ gbuf, float* __
   w[j] -= lr * (m / bc1) / (sqrtf(v / bc2) + 1e-8f);
 }
 
-// Training-set entry: res / h_b, p_prev and (p - p_prev) / s_r in FP32.
+// Training-set entry: res / h_b, p_prev and (p - p_prev - m) / s_r in FP32,
+// m the volume-weighted mean of p - p_prev (scal[2]): a constant shift of the
+// pressure is invisible to the operator and to the inputs, so it is not a
+// target.
 __global__ void k_cnn_sample(long long n, const double* __restrict__ res,
                              const double* __restrict__ prev, const double* __restrict__ p,
                              const double* __restrict__ h, const double* __restrict__ scal,
                              float* __restrict__ rh, float* __restrict__ pv,
                              float* __restrict__ tg) {
-  const double inv_s = 1.0 / scal[0];
+  const double inv_s = 1.0 / scal[0], m = scal[2];
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     rh[i] = (float)(res[i] / h[i / BLEN]);
     pv[i] = (float)prev[i];
-    tg[i] = (float)((p[i] - prev[i]) * inv_s);
+    tg[i] = (float)((p[i] - prev[i] - m) * inv_s);
   }
 }
 
@@ -508,7 +538,8 @@ int cnn_set_topology(CnnState& C, int nb, const int* d_nbr, const double* d_h, s
   if (C.d_pred) (void)hipFree(C.d_pred);
   C.d_part = nullptr;
   C.d_pred = nullptr;
-  CCHECK(hipMalloc(&C.d_part, ((nb + 3) / 4 + 8) * sizeof(double)));
+  // partial sums: up to 2 per workgroup of 4 blocks
+  CCHECK(hipMalloc(&C.d_part, (2 * ((nb + 3) / 4) + 16) * sizeof(double)));
   CCHECK(hipMalloc(&C.d_pred, C.n * sizeof(double)));
   return 0;
 }
@@ -580,6 +611,9 @@ int cnn_add_sample(CnnState& C, hipStream_t s, const double* d_res, const double
   S.prev = S.res_h + C.n;
   S.target = S.prev + C.n;
   if (scales(C, s, d_res, d_prev, err)) return -1;
+  const int nwg = (int)((C.n + 4 * BLEN - 1) / (4 * BLEN));
+  k_cnn_dmean<<<nwg, WG_THREADS, 0, s>>>(C.n, d_p, d_prev, C.d_h, C.d_part);
+  k_cnn_dmean_fin<<<1, WG_THREADS, 0, s>>>(C.d_scal, C.d_part, nwg);
   k_cnn_sample<<<4096, 256, 0, s>>>(C.n, d_res, d_prev, d_p, C.d_h, C.d_scal, S.res_h, S.prev,
                                     S.target);
   CCHECK(hipGetLastError());
@@ -779,6 +813,67 @@ int cnn_gradient(CnnState& C, hipStream_t s, int k, float res_weight, float* gra
   for (float v : lh) L += v;
   *loss = 0.5 * L / ((double)m * BLEN);
   (void)hipFree(d_refs);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Step length of the net's increment: x0 + gamma (pred - x0) with the gamma
+// that minimises the FP64 residual, from r0 = b - A x0 and rp = b - A pred
+// (A (pred - x0) = r0 - rp):
+//   gamma = (r0, r0 - rp) / |r0 - rp|^2,  |r(gamma)|^2 = |r0|^2 - (r0, r0 - rp)^2 / |r0 - rp|^2
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG_THREADS) void k_ws_dots(long long n, const double* __restrict__ r0,
+                                                        const double* __restrict__ rp,
+                                                        double* part) {
+  double v[2] = {0.0, 0.0};
+  const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
+  for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
+    const long long i = base + e;
+    if (i < n) {
+      v[0] += r0[i] * rp[i];
+      v[1] += rp[i] * rp[i];
+    }
+  }
+  grid_store_partial<2>(v, part);
+}
+
+__global__ __launch_bounds__(WG_THREADS) void k_ws_dots_fin(double* scal, const double* part,
+                                                            int nparts) {
+  double tot[2];
+  grid_sum_partials<2>(part, nparts, tot);
+  if (threadIdx.x == 0) {
+    scal[3] = tot[0];
+    scal[4] = tot[1];
+  }
+}
+
+__global__ void k_ws_blend(long long n, const double* __restrict__ x0, double* __restrict__ pred,
+                           double gamma) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    pred[i] = fma(gamma, pred[i] - x0[i], x0[i]);
+}
+
+int cnn_step_length(CnnState& C, hipStream_t s, const double* r0, double r0_norm2,
+                    const double* rp, const double* x0, double* pred, double* gamma,
+                    double* resid, std::string& err) {
+  const int nwg = (int)((C.n + 4 * BLEN - 1) / (4 * BLEN));
+  k_ws_dots<<<nwg, WG_THREADS, 0, s>>>(C.n, r0, rp, C.d_part);
+  k_ws_dots_fin<<<1, WG_THREADS, 0, s>>>(C.d_scal, C.d_part, nwg);
+  double d[2];
+  CCHECK(hipMemcpyAsync(d, C.d_scal + 3, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  CCHECK(hipStreamSynchronize(s));
+  const double a = r0_norm2, c = d[0], q = d[1];
+  const double den = a - 2 * c + q;  // |A (pred - x0)|^2
+  *gamma = 0.0;
+  *resid = std::sqrt(a);
+  if (!(den > 0) || !std::isfinite(den)) return 0;
+  const double g = (a - c) / den;
+  const double r2 = a - (a - c) * (a - c) / den;
+  *gamma = g;
+  *resid = std::sqrt(r2 > 0 ? r2 : 0.0);
+  k_ws_blend<<<4096, 256, 0, s>>>(C.n, x0, pred, g);
+  CCHECK(hipGetLastError());
   return 0;
 }
 

@@ -381,8 +381,9 @@ static double resid_norm(cup3d_poisson_t H, const double* d_x) {
   return std::sqrt(H->h_state->true_rr);
 }
 
-// Learned warm start: the net's guess replaces d_x only when its FP64
-// residual is smaller than that of the given guess (fallback otherwise).
+// Learned warm start: the net's increment, scaled by the step length that
+// minimises the FP64 residual, replaces d_x only when that residual is smaller
+// than the given guess's (fallback otherwise).
 static int cnn_warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* used,
                           double* resid) {
   *used = CUP3D_GUESS_GIVEN;
@@ -392,9 +393,14 @@ static int cnn_warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, i
   if (!H->cnn.have_weights) return 0;
   double* pred = H->cnn.d_pred;
   if (cnn_predict(H->cnn, H->stream, H->v64.nu, H->v64.x[0], pred, H->err) != 0) return -1;
-  const double r_net = resid_norm(H, pred);
-  if (r_net < 0 && !std::isnan(r_net)) return -1;
-  if (std::isfinite(r_net) && r_net < r_prev) {
+  const double r_raw = resid_norm(H, pred);  // residual into v64.t
+  if (r_raw < 0 && !std::isnan(r_raw)) return -1;
+  if (!std::isfinite(r_raw)) return 0;
+  double gamma = 0, r_net = r_prev;
+  if (cnn_step_length(H->cnn, H->stream, H->v64.nu, r_prev * r_prev, H->v64.t, H->v64.x[0], pred,
+                      &gamma, &r_net, H->err) != 0)
+    return -1;
+  if (gamma != 0.0 && std::isfinite(r_net) && r_net < r_prev) {
     HIPCHECK(hipMemcpyAsync(d_x, pred, (size_t)H->n * sizeof(double), hipMemcpyDeviceToDevice,
                             H->stream));
     HIPCHECK(hipStreamSynchronize(H->stream));

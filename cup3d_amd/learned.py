@@ -2,10 +2,11 @@
 
 The sequence is the uniform 256x128x128 walls workload with 64 Gaussian
 blobs (sigma 0.04) moving 0.005 per step (``workloads.c4_uniform_sequence``).
-Each step is solved by the FP64 drop-in to ||b - A x|| / ||b|| <= 1e-7 from
-the previous step's pressure; the first ``--train`` steps give the training
-triples (rhs_t, p_{t-1}, p_t), which the library's own HIP kernels learn
-(FP32 forward/backward, Adam). The net is then judged on the remaining steps
+Each step is solved to ||b - A x|| / ||b|| <= 1e-7 (FP64 true residual) from
+the previous step's pressure by the FP64-refined solve (the FP64 drop-in's
+recurrence stagnates above 1e-7 on some steps of this sequence); the first
+``--train`` steps give the training triples (rhs_t, p_{t-1}, p_t), which the
+library's own HIP kernels learn (FP32 forward/backward, Adam). The net is then judged on the remaining steps
 and on a sequence from a fresh seed by what it saves: the number of
 iterations, for the FP64 and the mixed-precision solve, against starting from
 the previous pressure. The solve always finishes to the same tolerance and
@@ -14,7 +15,7 @@ residual, so a worse net costs one inference and two residuals, never
 accuracy.
 
   python -m cup3d_amd.learned [--steps 400] [--train 300] [--fresh 40]
-                              [--train-steps 3000] [--batch 256] [--res-weight 1]
+                              [--train-steps 3000] [--batch 256] [--res-weight 10]
                               [--bpd 32 16 16]
 
 Prints one JSON line.
@@ -30,7 +31,7 @@ import numpy as np
 
 
 def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=3000, batch=256,
-        lr=1e-3, res_weight=1.0, seed=1, tol=1e-7, log=sys.stderr):
+        lr=1e-3, res_weight=10.0, seed=1, tol=1e-7, log=sys.stderr):
     import torch
 
     from . import workloads as W
@@ -38,8 +39,8 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=3000, batc
 
     dev = torch.device("cuda:0")
     w = W.c4_uniform_sequence(tuple(bpd))
-    kw = dict(tol_rel=0.0, tol_rhs=tol, bmean=0, max_iter=5000)
-    ref = solver_for(w, precision="fp64", **kw)
+    kw = dict(tol_rel=0.0, tol_rhs=tol, bmean=0, max_iter=3000)
+    ref = solver_for(w, precision="mixed_fp16", max_refine=40, **kw)
 
     def sequence(n, s):
         """FP64 solutions p_0 .. p_n of steps 0..n from previous pressures."""
@@ -50,11 +51,13 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=3000, batc
             x = p.clone()
             rep = ref.solve_device(b, x)
             if rep["status"] != 0:
-                raise RuntimeError(f"step {t}: FP64 solve did not converge: {rep}")
+                raise RuntimeError(f"step {t}: solve did not converge: {rep}")
             rhs.append(b)
             sols.append(x)
             its.append(rep["iterations"])
             p = x
+            if t % 25 == 0:
+                print(f"  step {t}/{n}: {rep['iterations']} iterations", file=log, flush=True)
         return rhs, sols, its
 
     t0 = time.time()
@@ -77,25 +80,45 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=3000, batc
             base = solver_for(w, precision=form, guess="given", **kw)
             net = solver_for(w, precision=form, guess="cnn", **kw)
             net.cnn_set_weights(weights)
-            it_b, it_n, used, ratio, err = [], [], 0, [], []
+            it_b, it_n, used, ratio, err, raw, fail_b, fail_n = [], [], 0, [], [], [], 0, 0
+            xp = torch.empty(w.ncells, dtype=torch.float64, device=dev)
+            ax = torch.empty_like(xp)
             for t in ts:
+                # the net's own residual, whether or not the solve takes it
+                net.cnn_predict(rhs_[t], sols_[t - 1], xp)
+                net.apply_operator(xp, ax)
+                rn_ = torch.linalg.norm(rhs_[t] - ax)
+                net.apply_operator(sols_[t - 1], ax)
+                raw.append(float(rn_ / torch.linalg.norm(rhs_[t] - ax)))
                 xb = sols_[t - 1].clone()
                 rb = base.solve_device(rhs_[t], xb)
                 xn = sols_[t - 1].clone()
                 rn = net.solve_device(rhs_[t], xn)
-                if rb["status"] != 0 or rn["status"] != 0:
-                    raise RuntimeError(f"step {t}: {form} solve did not converge")
+                # a solve that stops at max_iter counts with its iterations
+                fail_b += int(rb["status"] != 0)
+                fail_n += int(rn["status"] != 0)
+                if t % 25 == 0:
+                    print(f"  {form} step {t}: {rb['iterations']} vs {rn['iterations']} iterations",
+                          file=log, flush=True)
                 it_b.append(rb["iterations"])
                 it_n.append(rn["iterations"])
                 used += int(rn["guess_used"] == 2)
                 ratio.append(rn["guess_resid"] / rb["err_init"])
-                err.append(float(torch.linalg.norm(xn - sols_[t]) / torch.linalg.norm(sols_[t])))
+                # pressure is defined up to a constant here (walls, bMean = 0):
+                # compare after removing the mean, as the caller does
+                dx = (xn - xn.mean()) - (sols_[t] - sols_[t].mean())
+                err.append(float(torch.linalg.norm(dx) / torch.linalg.norm(sols_[t] - sols_[t].mean())))
             out[form] = {
                 "iterations_prev_guess": float(np.mean(it_b)),
                 "iterations_net_guess": float(np.mean(it_n)),
+                "iterations_prev_guess_median": float(np.median(it_b)),
+                "iterations_net_guess_median": float(np.median(it_n)),
                 "iterations_saved_pct": round(100.0 * (1 - np.sum(it_n) / np.sum(it_b)), 2),
                 "net_guess_used": f"{used}/{len(ts)}",
+                "not_converged_prev_guess": fail_b,
+                "not_converged_net_guess": fail_n,
                 "initial_resid_ratio_median": float(np.median(ratio)),
+                "net_resid_ratio_median": float(np.median(raw)),
                 "err_vs_fp64_max": float(np.max(err)),
             }
         return out
@@ -107,7 +130,8 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=3000, batc
         "workload": w.name, "cells": w.ncells, "steps": steps, "train_triples": train,
         "train_steps": train_steps, "batch_blocks": batch, "lr": lr, "res_weight": res_weight,
         "loss_first": l0, "loss_last": l1, "train_seconds": round(train_s, 1),
-        "fp64_iterations_mean_train": float(np.mean(its[1:train + 1])),
+        "sequence_solver": "mixed_fp16 refined to the FP64 true residual",
+        "sequence_iterations_mean_train": float(np.mean(its[1:train + 1])),
         "held_out": held, "fresh_seed": fr,
     }
 
@@ -120,7 +144,7 @@ def main():
     p.add_argument("--train-steps", type=int, default=3000)
     p.add_argument("--batch", type=int, default=256)
     p.add_argument("--lr", type=float, default=1e-3)
-    p.add_argument("--res-weight", type=float, default=1.0,
+    p.add_argument("--res-weight", type=float, default=10.0,
                    help="weight of the local-residual term of the training loss")
     p.add_argument("--bpd", type=int, nargs=3, default=[32, 16, 16])
     a = p.parse_args()
