@@ -2,14 +2,17 @@
 
 A step is one complete solve — rhs resident in HBM in Cubism's 8^3 block
 layout, pressure written back in that layout — to the true relative residual
-||b - A x|| / ||b|| <= 1e-7 (recomputed in FP64 on the device), warm-started
-from the previous step's pressure.
+||b - A x|| / ||b|| <= 1e-7 (recomputed in FP64 on the device). The steps are
+consecutive time steps of a moving-blob sequence, each warm-started from the
+previous step's (mean-free) pressure, as PressureProjection feeds the solver.
 
 Default workload: uniform 256x128x128 cells (32x16x16 blocks), domain
 [0,2]x[0,1]^2, walls, rhs = divergence of 256 Gaussian blobs with a
 penalised sphere; the previous pressure is the solution of the step before
-(blob centres 0.005 earlier). Two forms are timed on the same input:
-  * the FP64 drop-in (the reference algorithm, FP64 MFMA block-Jacobi GEMM);
+(blob centres 0.005 earlier). Two forms are timed on the same inputs:
+  * the FP64 drop-in (the reference algorithm, FP64 MFMA block-Jacobi GEMM,
+    the reference's cap of 1000 iterations; its steps that stagnate above
+    1e-7 are counted, not hidden);
   * the chosen form (headline `value`): FP16 split-precision MFMA GEMM with
     FP32 accumulation, FP32 Krylov vectors, FP64 iterative refinement.
 The line carries the roofline of the headline form's dominant kernel (HIP
@@ -54,8 +57,8 @@ TRAFFIC = {
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=10)
-    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--steps", type=int, default=16)
+    p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--form", default="mixed_fp16", choices=["fp64", "mixed_fp16", "mixed_bf16"])
     p.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"])
     p.add_argument("--bmean", type=int, default=0)
@@ -73,25 +76,25 @@ def make_workload(name):
             "c5": W.c5_amr_school}[name]()
 
 
-def time_form(w, form, bmean, rhs, p_prev, steps, warmup):
+def time_form(w, form, bmean, seq, warmup, max_iter):
+    """Solve every step of the sequence with `form`, each from the previous
+    step's reference pressure; time the steps after the first `warmup`."""
     import torch
     from cup3d_amd.poisson import solver_for
-    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-7, precision=form, bmean=bmean, max_iter=3000)
-    x = torch.empty_like(p_prev)
-    reps = []
-    for _ in range(warmup):
-        x.copy_(p_prev)
-        reps.append(S.solve_device(rhs, x))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        x.copy_(p_prev)
-        reps.append(S.solve_device(rhs, x))
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    rep = reps[-1]
-    ok = all(r["status"] == 0 and r["true_rel_resid"] <= 1.0e-7 for r in reps)
-    return S, x.clone(), rep, dt / steps * 1e3, ok
+    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-7, precision=form, bmean=bmean, max_iter=max_iter)
+    x = torch.empty_like(seq[0]["p_prev"])
+    reps, xs, ms = [], [], 0.0
+    for k, st in enumerate(seq):
+        x.copy_(st["p_prev"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rep = S.solve_device(st["rhs"], x)
+        dt = time.perf_counter() - t0
+        if k >= warmup:
+            ms += dt * 1e3
+            reps.append(rep)
+            xs.append(x.clone())
+    return S, xs, reps, ms / len(reps)
 
 
 def kernel_times(S, which_list, reps=20):
@@ -181,43 +184,64 @@ def main():
         print(json.dumps({"workload": w.name, "form": args.form, "copy_bytes": COPY_BYTES,
                           "kernels": {KERNELS[i]: v["bytes"] for i, v in k.items()}}))
         return
-    rhs_prev = torch.from_numpy(w.rhs(step=0).reshape(-1)).to(dev)
-    rhs = torch.from_numpy(w.rhs(step=1).reshape(-1)).to(dev)
-    # previous-step pressure and a reference solution to ||b - Ax|| <= 1e-10 ||b||
-    # (untimed setup): the FP64 drop-in, or where its recurrence stagnates
-    # above that, the FP64-refined mixed solve (its convergence test is the
-    # same FP64 true residual)
-    ref_form, ref_res = "fp64", None
-    for form in ("fp64", "mixed_fp16"):
-        R = solver_for(w, tol_rel=0.0, tol_rhs=1e-10, bmean=args.bmean, precision=form,
-                       max_iter=5000, max_refine=40)
-        p_prev = torch.zeros(w.ncells, dtype=torch.float64, device=dev)
-        r0 = R.solve_device(rhs_prev, p_prev)
-        x_ref = p_prev.clone()
-        r1 = R.solve_device(rhs, x_ref)
-        ref_form, ref_res = form, r1["true_rel_resid"]
-        if r0["status"] == 0 and r1["status"] == 0:
-            break
+    # A sequence of time steps: step t has rhs_t (blobs moved 0.005 per step)
+    # and starts from the pressure of step t-1. Untimed setup: the reference
+    # solution of every step, ||b - Ax|| <= 1e-9 ||b|| on the FP64 true
+    # residual, by the FP64-refined solve (the FP64 drop-in's recurrence
+    # stagnates above that on these grids); the caller removes the
+    # volume-weighted mean after every solve (PressureProjection.cpp:404-431),
+    # so every guess is mean-free.
     hw = torch.from_numpy(np.repeat((w.h0 / 2.0 ** w.blocks[:, 0]) ** 3, 512)).to(dev)
-
-    S64, x64, rep64, ms64, ok64 = time_form(w, "fp64", args.bmean, rhs, p_prev, args.steps,
-                                            args.warmup)
-    if args.form == "fp64":
-        S, x, rep, ms, ok = S64, x64, rep64, ms64, ok64
-    else:
-        S, x, rep, ms, ok = time_form(w, args.form, args.bmean, rhs, p_prev, args.steps,
-                                      args.warmup)
-    if not ok:
-        raise SystemExit(f"solve did not reach the tolerance: {rep}")
+    R = solver_for(w, tol_rel=0.0, tol_rhs=1e-9, bmean=args.bmean, precision="mixed_fp16",
+                   max_iter=2000, max_refine=8)
+    nsteps = args.warmup + args.steps
+    p_prev = torch.zeros(w.ncells, dtype=torch.float64, device=dev)
+    seq, ref_res = [], []
+    for t in range(nsteps + 1):
+        b = torch.from_numpy(w.rhs(step=t).reshape(-1)).to(dev)
+        x = p_prev.clone()
+        r = R.solve_device(b, x)
+        x -= (hw * x).sum() / hw.sum()
+        if t > 0:
+            seq.append({"rhs": b, "p_prev": p_prev, "x_ref": x})
+            ref_res.append(r["true_rel_resid"])
+        p_prev = x
+        print(f"setup: reference solve of step {t}/{nsteps} done", file=sys.stderr, flush=True)
+    del R
 
     def rel(a, b):
         """Relative L2 distance after removing the volume-weighted means (the
-        caller's post-solve projection, PressureProjection.cpp:404-431)."""
+        caller's post-solve projection)."""
         a = a - (hw * a).sum() / hw.sum()
         b = b - (hw * b).sum() / hw.sum()
         return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
 
-    host_ms, host_status = host_solve_ms(S, rhs, p_prev)
+    timed = seq[args.warmup:]
+
+    def summary(xs, reps):
+        conv = [r["status"] == 0 and r["true_rel_resid"] <= 1.0e-7 for r in reps]
+        return {"iterations_mean": float(np.mean([r["iterations"] for r in reps])),
+                "iterations": [r["iterations"] for r in reps],
+                "converged": f"{sum(conv)}/{len(conv)}",
+                "true_rel_resid_max": max(r["true_rel_resid"] for r in reps),
+                "err_vs_fp64_reference_max": max(rel(x, st["x_ref"])
+                                                 for x, st in zip(xs, timed))}
+
+    # the FP64 drop-in with the reference's iteration cap (BiCGSTAB.cu:474)
+    S64, xs64, reps64, ms64 = time_form(w, "fp64", args.bmean, seq, args.warmup, 1000)
+    print(f"fp64: {ms64:.2f} ms per step", file=sys.stderr, flush=True)
+    if args.form == "fp64":
+        S, xs, reps, ms = S64, xs64, reps64, ms64
+    else:
+        S, xs, reps, ms = time_form(w, args.form, args.bmean, seq, args.warmup, 3000)
+    print(f"{args.form}: {ms:.2f} ms per step", file=sys.stderr, flush=True)
+    sm, sm64 = summary(xs, reps), summary(xs64, reps64)
+    if not all(r["status"] == 0 and r["true_rel_resid"] <= 1.0e-7 for r in reps):
+        raise SystemExit(f"{args.form} did not reach the tolerance on every step: {sm}")
+    rep, rhs = reps[-1], timed[-1]["rhs"]
+    p_last = timed[-1]["p_prev"]
+
+    host_ms, host_status = host_solve_ms(S, rhs, p_last)
     kern = kernel_times(S, [2, 3, 4])
     dom = max(kern, key=lambda k: kern[k]["avg_us"] * CALLS_PER_ITER[k])
     kd = kern[dom]
@@ -244,17 +268,17 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "form": args.form, "cells": w.ncells,
                    "blocks": int(len(w.blocks)), "tolerance": "||b-Ax||/||b|| <= 1e-7",
+                   "steps": "consecutive time steps t = warmup+1 .. warmup+steps",
                    "warm_start": "previous-step pressure", "bMeanConstraint": args.bmean,
                    "parallelism": "single GPU"},
-        "iterations": rep["iterations"],
+        "iterations": sm["iterations_mean"],
         "refinements": rep["refinements"],
-        "cell_iterations_per_s": round(w.ncells * rep["iterations"] / (ms * 1e-3), 1),
-        "true_rel_resid": rep["true_rel_resid"],
-        "err_vs_fp64_reference": rel(x, x_ref),
-        "reference": {"form": ref_form, "true_rel_resid": ref_res},
+        "cell_iterations_per_s": round(w.ncells * sm["iterations_mean"] / (ms * 1e-3), 1),
+        "solves": sm,
+        "reference": {"form": "mixed_fp16 refined to ||b-Ax|| <= 1e-9 ||b|| (FP64 residual)",
+                      "true_rel_resid_max": max(ref_res)},
         "fp64": {"solves_per_s": round(1e3 / ms64, 3), "ms_per_step": round(ms64, 3),
-                 "iterations": rep64["iterations"], "true_rel_resid": rep64["true_rel_resid"],
-                 "err_vs_fp64_reference": rel(x64, x_ref),
+                 "max_iter": 1000, "solves": sm64,
                  "gemm_TFLOP/s": round(g64["TFLOP/s"], 2),
                  "gemm_mfma_frac": round(g64["TFLOP/s"] / FP64_MFMA_PEAK, 4),
                  "kernels_us": {KERNELS[k]: round(v["avg_us"], 2) for k, v in k64.items()}},
@@ -268,8 +292,8 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.bmean, rhs.cpu().numpy(),
-                                           p_prev.cpu().numpy(), rep64["iterations"],
-                                           args.cpu_iters)
+                                           p_last.cpu().numpy(),
+                                           int(round(sm64["iterations_mean"])), args.cpu_iters)
     print(json.dumps(out))
 
 

@@ -141,15 +141,18 @@ static __global__ __launch_bounds__(WG_THREADS) void k_getvec_fin(SolverState* s
 // 167-217 for every cell that touches a coarse-fine face), one thread per
 // row over the whole grid: corr[64 c + lane] = sum_k val_k x[col_k] (FP64),
 // or sum_k val_k (x[col_k] - x_cell) in FP32, where the rows sum to zero and
-// differences keep the precision. Entries are taken 16 at a time, all loads
-// of a batch issued before the first use (most rows fit in one batch).
+// differences keep the precision. Entries are taken 32 at a time, all loads
+// of a batch issued before the first use, so a row costs three dependent
+// memory round trips (table, entries, gathered x).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(WG_THREADS) void k_iface_rows(DevGrid<T> g, const T* __restrict__ x,
                                                            const SolverState* st) {
-  constexpr int U = 16;
+  constexpr int U = 32;  // rows of up to 32 entries (all rows of the school grids) in one batch
   if (st && st->status != ST_RUNNING) return;
-  const int ch = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  // XCD-aware order: consecutive chunks (rows of the same and neighbouring
+  // blocks, which gather the same x values) share one XCD's L2
+  const int ch = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (ch >= g.nch) return;
   const int meta = g.ell_meta[ch * 64 + lane];
   const int K = g.ell_k[ch];  // longest row of the chunk (same in all lanes)

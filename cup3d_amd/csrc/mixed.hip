@@ -279,14 +279,73 @@ __global__ __launch_bounds__(WG_THREADS) void k_inner_init_fin(SolverState* st, 
   fin_store(st, S);
 }
 
-// x64 += scale * (double) x32[xopt]   (refinement update, FP64)
-__global__ __launch_bounds__(WG_THREADS) void k_refine(long long n, double* __restrict__ x64,
-                                                       KrylovVectors<float> v,
-                                                       const SolverState* st, double scale) {
-  const float* __restrict__ d = v.x[st->xopt];
+// Fold the FP32 increment into the FP64 iterate: x64 += scale * x32[k];
+// x32[k] = 0, with k = xcur (zero = 1: the recurrence continues from this
+// iterate) or xopt (zero = 0: the process ends here).
+__global__ __launch_bounds__(WG_THREADS) void k_fold(long long n, double* __restrict__ x64,
+                                                     KrylovVectors<float> v,
+                                                     const SolverState* st, double scale,
+                                                     int zero) {
+  float* __restrict__ d = v.x[zero ? st->xcur : st->xopt];
   for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n;
-       i += (long long)gridDim.x * WG_THREADS)
+       i += (long long)gridDim.x * WG_THREADS) {
     x64[i] = fma(scale, (double)d[i], x64[i]);
+    if (zero) d[i] = 0.f;
+  }
+}
+
+// Residual replacement: r32 = r64 * inv_scale (the true FP64 residual of
+// the folded iterate, in the scaling of the running Krylov process), with
+// (rhat, r) and |r|^2 for the scalars.
+__global__ __launch_bounds__(WG_THREADS) void k_replace(long long n, const double* __restrict__ r64,
+                                                        double inv_scale, KrylovVectors<float> v,
+                                                        double* part) {
+  double s[2] = {0.0, 0.0};
+  const long long i0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN) + 8 * threadIdx.x;
+  if (i0 < n) {
+    double rd[8];
+    float r[8], rh[8];
+    load8(r64 + i0, rd);
+    load8(v.rhat + i0, rh);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      r[e] = (float)(rd[e] * inv_scale);
+      s[0] += (double)rh[e] * (double)r[e];
+      s[1] += (double)r[e] * (double)r[e];
+    }
+    store8(v.r + i0, r);
+  }
+  grid_store_partial<2>(s, part);
+}
+
+// The scalars after a replacement. If the process had stopped (converged on
+// its recurrence residual), the next iteration starts here; otherwise the
+// iteration already begun takes the replaced residual's rho and |r|^2.
+__global__ __launch_bounds__(WG_THREADS) void k_replace_fin(SolverState* st, const double* part,
+                                                            int nparts) {
+  __shared__ SolverState S;
+  double tot[2];
+  fin_load<2>(st, S, part, nparts, tot);
+  if (threadIdx.x == 0) {
+    S.err = sqrt(tot[1]);
+    S.err_opt = S.err;
+    S.xopt = S.xcur;
+    if (S.status == ST_CONVERGED) {
+      S.status = ST_RUNNING;
+      S.rho_prev = S.rho_curr;
+      begin_iteration(&S, tot[0], tot[1]);
+    } else {
+      if (S.restart_pending) {  // rhat = r in the next K1
+        S.rho_curr = tot[1];
+        S.rhrh = tot[1];
+      } else {
+        S.rho_curr = tot[0];
+      }
+      S.rr = tot[1];
+      S.beta = (S.rho_curr / (S.rho_prev + S.eps)) * (S.alpha / (S.omega + S.eps));
+    }
+  }
+  fin_store(st, S);
 }
 
 __global__ void k_copy64(long long n, const double* __restrict__ a, double* __restrict__ b) {
@@ -453,32 +512,50 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
   MCHECK(hipStreamSynchronize(s));
   const double bnorm2 = h_st->bnorm2;
   const double tol_abs = h_st->tol_abs;
-  double err_init = -1, rnorm = 0;
-  int total_iters = 0, total_restarts = 0, steps = 0, status = ST_MAXITER;
+  int total_iters = 0, total_restarts = 0, steps = 0, replacements = 0, status = ST_MAXITER;
   const int poll = cfg.poll_every > 0 ? cfg.poll_every : 8;
-  for (int k = 0; k <= cfg.max_refine; ++k) {
-    // true residual in FP64
+  auto true_resid = [&](double& out) -> int {  // |b - A x64| in FP64, r64 = b - A x64
     launch_spmv64(s, SPMV_RESID, g64, x64, nullptr, b64, r64, nullptr, st, part);
     MCHECK(hipMemcpyAsync(h_st, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
     MCHECK(hipStreamSynchronize(s));
-    rnorm = std::sqrt(h_st->true_rr);
-    if (k == 0) err_init = rnorm;
-    if (!std::isfinite(rnorm)) {
-      status = ST_NONFINITE;
-      break;
-    }
-    const double tol = std::max(tol_abs, cfg.tol_rel * err_init);
-    if (rnorm <= tol) {
-      status = ST_CONVERGED;
-      break;
-    }
-    if (k == cfg.max_refine) break;
-    // inner FP32 solve of A d = r / |r|
-    const double inner_rel = std::max(cfg.inner_tol, 0.5 * tol / rnorm);
+    out = std::sqrt(h_st->true_rr);
+    return 0;
+  };
+  double rnorm = 0;
+  if (true_resid(rnorm)) return -1;
+  const double err_init = rnorm;
+  const double tol = std::max(tol_abs, cfg.tol_rel * err_init);
+  bool done = false;
+  if (!std::isfinite(rnorm)) {
+    status = ST_NONFINITE;
+    done = true;
+  } else if (rnorm <= tol) {
+    status = ST_CONVERGED;
+    done = true;
+  }
+  // One FP32 Krylov process per refinement step, on A d = r / |r|. The
+  // process is not restarted when its recurrence residual has fallen by
+  // REPLACE_DROP: the FP32 increment is folded into the FP64 iterate and the
+  // recurrence residual replaced by the true FP64 residual (reliable updates,
+  // as the residual replacement of PoissonSolverAMR.cpp keeps a long
+  // recurrence accurate), so the Krylov space built so far is kept. A new
+  // refinement step (restart) follows only a breakdown or the iteration cap.
+#ifndef CUP3D_REPLACE_DROP
+#define CUP3D_REPLACE_DROP 1e-2
+#endif
+  constexpr double REPLACE_DROP = CUP3D_REPLACE_DROP;  // 0: restart at every stop
+  constexpr int STALL_ITERS = 200;
+  while (!done && steps <= cfg.max_refine) {
+    const double scale = rnorm;
+    const double inner_rel = std::max(cfg.inner_tol, 0.5 * tol / scale);
     *h_st = inner_state(cfg, inner_rel);
     MCHECK(hipMemcpyAsync(st, h_st, sizeof(SolverState), hipMemcpyHostToDevice, s));
-    k_inner_init<<<grid_vec(n), WG_THREADS, 0, s>>>(n, r64, 1.0 / rnorm, v, st, part);
+    k_inner_init<<<grid_vec(n), WG_THREADS, 0, s>>>(n, r64, 1.0 / scale, v, st, part);
     k_inner_init_fin<<<1, WG_THREADS, 0, s>>>(st, part, grid_vec(n));
+    ++steps;
+    double last = 1.0;  // recurrence residual (scaled) at the last replacement
+    double progress = 1.0;  // residual at the last halving, and its iteration
+    int progress_iter = 0;
     int guard = 0;
     while (true) {
       for (int it = 0; it < poll; ++it) {
@@ -492,16 +569,63 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
       MCHECK(hipGetLastError());
       MCHECK(hipMemcpyAsync(h_st, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
       MCHECK(hipStreamSynchronize(s));
-      if (h_st->status != ST_RUNNING) break;
-      if (++guard > cfg.max_iter / poll + 4) {
+      if (++guard > cfg.max_iter / poll + 4 + 2 * replacements) {
         err = "inner solve did not stop";
         return -1;
       }
+      int ist = h_st->status;
+      if (h_st->err_opt < 0.5 * progress) {
+        progress = h_st->err_opt;
+        progress_iter = h_st->iter;
+      }
+      // a recurrence that has not halved its best residual for STALL_ITERS
+      // iterations is restarted from the FP64 residual of its best iterate
+      const bool stalled = ist == ST_RUNNING && h_st->iter - progress_iter > STALL_ITERS;
+      if (stalled) ist = ST_MAXITER;
+      else if (ist == ST_RUNNING && h_st->err > REPLACE_DROP * last) continue;
+      if (ist == ST_NONFINITE) {
+        status = ST_NONFINITE;
+        done = true;
+        break;
+      }
+      if ((ist == ST_RUNNING || ist == ST_CONVERGED) && REPLACE_DROP > 0) {
+        k_fold<<<grid_1d(n), WG_THREADS, 0, s>>>(n, x64, v, st, scale, 1);
+        if (true_resid(rnorm)) return -1;
+        ++replacements;
+        if (!std::isfinite(rnorm)) {
+          status = ST_NONFINITE;
+          done = true;
+          break;
+        }
+        if (rnorm <= tol) {
+          status = ST_CONVERGED;
+          done = true;
+          break;
+        }
+        k_replace<<<grid_vec(n), WG_THREADS, 0, s>>>(n, r64, 1.0 / scale, v, part);
+        k_replace_fin<<<1, WG_THREADS, 0, s>>>(st, part, grid_vec(n));
+        last = rnorm / scale;
+        progress = last;  // the stall window restarts at every replacement
+        progress_iter = h_st->iter;
+        continue;
+      }
+      // breakdown or iteration cap (or a stop without replacements): keep the
+      // best iterate, restart
+      k_fold<<<grid_1d(n), WG_THREADS, 0, s>>>(n, x64, v, st, scale, 0);
+      if (true_resid(rnorm)) return -1;
+      if (!std::isfinite(rnorm)) {
+        status = ST_NONFINITE;
+        done = true;
+      } else if (rnorm <= tol) {
+        status = ST_CONVERGED;
+        done = true;
+      }
+      break;
     }
+    MCHECK(hipMemcpyAsync(h_st, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+    MCHECK(hipStreamSynchronize(s));
     total_iters += h_st->iters_done;
     total_restarts += h_st->restarts;
-    ++steps;
-    k_refine<<<grid_1d(n), WG_THREADS, 0, s>>>(n, x64, v, st, rnorm);
   }
   k_copy64<<<grid_1d(n), WG_THREADS, 0, s>>>(n, x64, d_x);
   MCHECK(hipGetLastError());
@@ -510,7 +634,7 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
     rep->status = status;
     rep->iterations = total_iters;
     rep->restarts = total_restarts;
-    rep->refinements = steps;
+    rep->refinements = replacements;  // FP64 residual recomputations with a correction
     rep->err_init = err_init;
     rep->err_final = rnorm;
     rep->rhs_norm = std::sqrt(bnorm2);

@@ -191,7 +191,7 @@ int cup3d_poisson_create(const cup3d_poisson_config* cfg, cup3d_poisson_t* out) 
   if (c.device < 0 || c.device >= ndev) { g_create_error = "invalid device ordinal"; return -1; }
   std::unique_ptr<cup3d_poisson> H(new cup3d_poisson());
   H->cfg = c;
-  if (H->cfg.inner_tol <= 0) H->cfg.inner_tol = 1e-4;
+  if (H->cfg.inner_tol <= 0) H->cfg.inner_tol = 1e-7;
   if (H->cfg.max_refine <= 0) H->cfg.max_refine = 10;
   H->mixed.split = c.gemm_passes != 1;
   H->device = c.device;

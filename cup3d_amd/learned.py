@@ -15,7 +15,7 @@ residual, so a worse net costs one inference and two residuals, never
 accuracy.
 
   python -m cup3d_amd.learned [--steps 400] [--train 300] [--fresh 40]
-                              [--train-steps 3000] [--batch 256] [--res-weight 10]
+                              [--train-steps 20000] [--batch 256] [--res-weight 100]
                               [--bpd 32 16 16]
 
 Prints one JSON line.
@@ -30,8 +30,8 @@ import time
 import numpy as np
 
 
-def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=3000, batch=256,
-        lr=1e-3, res_weight=10.0, seed=1, tol=1e-7, log=sys.stderr):
+def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, batch=256,
+        lr=1e-3, res_weight=100.0, seed=1, tol=1e-7, log=sys.stderr):
     import torch
 
     from . import workloads as W
@@ -42,8 +42,12 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=3000, batc
     kw = dict(tol_rel=0.0, tol_rhs=tol, bmean=0, max_iter=3000)
     ref = solver_for(w, precision="mixed_fp16", max_refine=40, **kw)
 
+    hw = torch.from_numpy(np.repeat((w.h0 / 2.0 ** w.blocks[:, 0]) ** 3, 512)).to(dev)
+
     def sequence(n, s):
-        """FP64 solutions p_0 .. p_n of steps 0..n from previous pressures."""
+        """Solutions p_0 .. p_n of steps 0..n, each from the previous pressure
+        and, as the caller does after every solve (PressureProjection.cpp:
+        404-431), with its volume-weighted mean removed."""
         rhs, sols, its = [], [], []
         p = torch.zeros(w.ncells, dtype=torch.float64, device=dev)
         for t in range(n + 1):
@@ -52,6 +56,7 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=3000, batc
             rep = ref.solve_device(b, x)
             if rep["status"] != 0:
                 raise RuntimeError(f"step {t}: solve did not converge: {rep}")
+            x -= (hw * x).sum() / hw.sum()
             rhs.append(b)
             sols.append(x)
             its.append(rep["iterations"])
@@ -106,8 +111,8 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=3000, batc
                 ratio.append(rn["guess_resid"] / rb["err_init"])
                 # pressure is defined up to a constant here (walls, bMean = 0):
                 # compare after removing the mean, as the caller does
-                dx = (xn - xn.mean()) - (sols_[t] - sols_[t].mean())
-                err.append(float(torch.linalg.norm(dx) / torch.linalg.norm(sols_[t] - sols_[t].mean())))
+                dx = (xn - (hw * xn).sum() / hw.sum()) - sols_[t]
+                err.append(float(torch.linalg.norm(dx) / torch.linalg.norm(sols_[t])))
             out[form] = {
                 "iterations_prev_guess": float(np.mean(it_b)),
                 "iterations_net_guess": float(np.mean(it_n)),
@@ -141,10 +146,10 @@ def main():
     p.add_argument("--steps", type=int, default=400)
     p.add_argument("--train", type=int, default=300)
     p.add_argument("--fresh", type=int, default=40)
-    p.add_argument("--train-steps", type=int, default=3000)
+    p.add_argument("--train-steps", type=int, default=20000)
     p.add_argument("--batch", type=int, default=256)
     p.add_argument("--lr", type=float, default=1e-3)
-    p.add_argument("--res-weight", type=float, default=10.0,
+    p.add_argument("--res-weight", type=float, default=100.0,
                    help="weight of the local-residual term of the training loss")
     p.add_argument("--bpd", type=int, nargs=3, default=[32, 16, 16])
     a = p.parse_args()

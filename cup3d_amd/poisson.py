@@ -105,7 +105,7 @@ class PoissonSolver:
 
     def __init__(self, bpd, h0, bc, level_max=0, tol_abs=0.0, tol_rel=1e-7, tol_rhs=0.0,
                  bmean=1, max_iter=1000, max_restarts=100, precision="fp64", guess="given",
-                 device=0, inner_tol=1e-4, max_refine=10, poll_every=8, gemm_passes=3):
+                 device=0, inner_tol=1e-7, max_refine=10, poll_every=8, gemm_passes=3):
         self.L = lib()
         c = Config()
         for d in range(3):

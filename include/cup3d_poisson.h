@@ -82,13 +82,15 @@ typedef struct {
   int precision;     /* CUP3D_PREC_* */
   int guess;         /* CUP3D_GUESS_* */
   int device;        /* HIP device ordinal */
-  /* mixed precision only: inner FP32 solve tolerance relative to the
-     refinement residual, and the cap on FP64 refinement steps */
+  /* mixed precision only: floor of the FP32 Krylov tolerance relative to the
+     residual it starts from (it targets max(inner_tol, 0.5 tol / |r|);
+     0 = 1e-7), and the cap on restarts of the FP32 Krylov process (each
+     breakdown or iteration cap starts a new one from the FP64 residual) */
   double inner_tol;
   int max_refine;
   int poll_every;    /* iterations between host polls of the device status (0 = default) */
-  int gemm_passes;   /* mixed only: 3 (or 0) = split hi/lo operands, three MFMA passes,
-                        FP32-accurate; 1 = one FP16/BF16 pass */
+  int gemm_passes;   /* mixed only: 0 (or 2, 3) = input split into hi + lo, two MFMA
+                        passes, linear to FP32 precision; 1 = one FP16/BF16 pass */
 } cup3d_poisson_config;
 
 typedef struct {
@@ -100,7 +102,8 @@ typedef struct {
   int status;            /* CUP3D_CONVERGED ... */
   int iterations;        /* Krylov iterations (sum over refinement steps) */
   int restarts;
-  int refinements;       /* FP64 refinement steps (1 for FP64) */
+  int refinements;       /* mixed: FP64 corrections (fold of the FP32 increment into
+                            the FP64 iterate + true residual); 1 for FP64 */
   double err_init;       /* ||b - A x0||_2 (BiCGSTAB.cu:461) */
   double err_final;      /* best recurrence residual ||r||_2 */
   double rhs_norm;       /* ||b||_2 */
