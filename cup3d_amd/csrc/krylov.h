@@ -137,51 +137,122 @@ static __global__ __launch_bounds__(WG_THREADS) void k_getvec_fin(SolverState* s
 }
 
 // ---------------------------------------------------------------------------
-// Coarse-fine interface rows (the merged rows of PoissonSolverExp.cpp:
-// 167-217 for every cell that touches a coarse-fine face), one thread per
-// row over the whole grid: corr[64 c + lane] = sum_k val_k x[col_k] (FP64),
-// or sum_k val_k (x[col_k] - x_cell) in FP32, where the rows sum to zero and
-// differences keep the precision. Entries are taken 32 at a time, all loads
-// of a batch issued before the first use, so a row costs three dependent
-// memory round trips (table, entries, gathered x).
+// Coarse-fine faces, matrix-free (PoissonSolverExp.cpp:167-217 with the
+// interpolation of PoissonSolverExp.h:384-478).
+//
+// Ghost of a fine cell next to a coarser block: 2/3 p_close - 1/5 p_far +
+// 8/15 T, T the Taylor value of the coarse face-layer cell cc with
+//   T = sum W[a][b] P(cc + (a-2) e_t1 + (b-2) e_t2),
+//   W = dd + s1 D1(k1) d + s2 d D1(k2) + D2(k1) d + d D2(k2) + s1 s2 M(k1) M(k2),
+// k the stencil class of the tangential coarse index (one-sided next to a
+// half-block edge: BWD at 3 and 7, FWD at 0 and 4), s = +-1 the side of the
+// fine cell inside the coarse cell. D1/D2/M over three points (M: two):
+//   D1 CEN (-1,0,1): -1/8 0 1/8   BWD (-2,-1,0): 1/8 -1/2 3/8   FWD (0,1,2): -3/8 1/2 -1/8
+//   D2 all classes: 1/32 -1/16 1/32
+//   M  CEN (-1,+1): -1/8 1/8      BWD (-1,0): -1/4 1/4           FWD (0,1): -1/4 1/4
+// The fine side's row is h (ghost - p); the coarse side's row is the sum over
+// its four fine cells of h_f (p_f,close - ghost_f). Both rows sum to zero, so
+// FP32 uses differences to the row's own cell value (every weight sum of a
+// Taylor value is 1).
 // ---------------------------------------------------------------------------
+enum { TK_CEN = 0, TK_BWD = 1, TK_FWD = 2 };
+
+__device__ __forceinline__ int taylor_kind(int t) {
+  return (t == BS - 1 || t == BS / 2 - 1) ? TK_BWD : ((t == 0 || t == BS / 2) ? TK_FWD : TK_CEN);
+}
+
+// sum W[a][b] (P(a-2, b-2) - ref); get(o1, o2) reads the face layer at
+// tangential offsets from the centre cell
+template <typename T, typename F>
+__device__ __forceinline__ T taylor_sum(F&& get, int k1, int k2, T s1, T s2, T ref) {
+  const int o1 = k1 == TK_CEN ? -1 : (k1 == TK_BWD ? -2 : 0);
+  const int o2 = k2 == TK_CEN ? -1 : (k2 == TK_BWD ? -2 : 0);
+  const T d1a[3] = {k1 == TK_CEN ? (T)(-0.125) : (k1 == TK_BWD ? (T)0.125 : (T)(-0.375)),
+                    k1 == TK_CEN ? (T)0 : (k1 == TK_BWD ? (T)(-0.5) : (T)0.5),
+                    k1 == TK_CEN ? (T)0.125 : (k1 == TK_BWD ? (T)0.375 : (T)(-0.125))};
+  const T d1b[3] = {k2 == TK_CEN ? (T)(-0.125) : (k2 == TK_BWD ? (T)0.125 : (T)(-0.375)),
+                    k2 == TK_CEN ? (T)0 : (k2 == TK_BWD ? (T)(-0.5) : (T)0.5),
+                    k2 == TK_CEN ? (T)0.125 : (k2 == TK_BWD ? (T)0.375 : (T)(-0.125))};
+  const T d2[3] = {(T)0.03125, (T)(-0.0625), (T)0.03125};
+  T sum = get(0, 0) - ref;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    sum += (s1 * d1a[i] + d2[i]) * (get(o1 + i, 0) - ref);
+    sum += (s2 * d1b[i] + d2[i]) * (get(0, o2 + i) - ref);
+  }
+  const int m1a = k1 == TK_FWD ? 0 : -1, m1b = k1 == TK_BWD ? 0 : 1;
+  const int m2a = k2 == TK_FWD ? 0 : -1, m2b = k2 == TK_BWD ? 0 : 1;
+  const T w1 = k1 == TK_CEN ? (T)0.125 : (T)0.25, w2 = k2 == TK_CEN ? (T)0.125 : (T)0.25;
+  const T sm = s1 * s2 * w1 * w2;  // M(k) = w (-1 at ma, +1 at mb)
+  sum += sm * (((get(m1b, m2b) - ref) - (get(m1a, m2b) - ref)) -
+               ((get(m1b, m2a) - ref) - (get(m1a, m2a) - ref)));
+  return sum;
+}
+
+// Contribution of coarse-fine face f of block b to the rows of its 64 face
+// cells (lane = cell (t1, t2) = (lane & 7, lane >> 3)), added into ys.
+// xs holds block b, pl is this wavefront's 64-value staging plane.
 template <typename T>
-__global__ __launch_bounds__(WG_THREADS) void k_iface_rows(DevGrid<T> g, const T* __restrict__ x,
-                                                           const SolverState* st) {
-  constexpr int U = 32;  // rows of up to 32 entries (all rows of the school grids) in one batch
-  if (st && st->status != ST_RUNNING) return;
-  // XCD-aware order: consecutive chunks (rows of the same and neighbouring
-  // blocks, which gather the same x values) share one XCD's L2
-  const int ch = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (ch >= g.nch) return;
-  const int meta = g.ell_meta[ch * 64 + lane];
-  const int K = g.ell_k[ch];  // longest row of the chunk (same in all lanes)
-  const int len = meta >> 16;
-  const size_t eo = (size_t)g.ell_off[ch] + lane;
-  const T xc = x[(size_t)g.ich_blk[ch] * BLEN + (meta & 0xffff)];
+__device__ __forceinline__ void iface_face(const DevGrid<T>& g, const T* __restrict__ x,
+                                           const T* xs, T* pl, T* ys, int b, int f, int fk,
+                                           int par, int lane) {
+  const int d = f >> 1;
+  const bool minus = (f & 1) == 0;
+  const int t1 = d == 0 ? 1 : (d == 1 ? 2 : 0), t2 = d == 0 ? 2 : (d == 1 ? 0 : 1);
+  const int st1 = t1 == 0 ? 1 : (t1 == 1 ? 8 : 64), st2 = t2 == 0 ? 1 : (t2 == 1 ? 8 : 64);
+  const int sd = d == 0 ? 1 : (d == 1 ? 8 : 64);
+  const int u = lane & 7, v = lane >> 3;
+  const int cl = (minus ? 0 : 7) * sd + u * st1 + v * st2;  // this lane's face cell
+  const T xc = xs[cl];
+  const bool diff = sizeof(T) == 4;
+  const T ref = diff ? xc : (T)0;
   T acc = 0;
-  for (int k0 = 0; k0 < K; k0 += U) {
-    int ci[U];
-    T cv[U], xv[U];
+  if (fk == FACE_COARSE) {
+    // stage the coarse block's face layer, [t2][t1]
+    const int cb = g.fnb[(b * 6 + f) * 4];
+    pl[lane] = x[(size_t)cb * BLEN + (minus ? 7 : 0) * sd + u * st1 + v * st2];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const T h = g.h[b];
+    const int c1 = (u >> 1) + (((par >> t1) & 1) ? BS / 2 : 0);
+    const int c2 = (v >> 1) + (((par >> t2) & 1) ? BS / 2 : 0);
+    const T s1 = (u & 1) ? (T)1 : (T)(-1), s2 = (v & 1) ? (T)1 : (T)(-1);
+    const T tay = taylor_sum<T>([&](int o1, int o2) { return pl[(c2 + o2) * 8 + c1 + o1]; },
+                                taylor_kind(c1), taylor_kind(c2), s1, s2, ref);
+    const T xin = xs[cl + (minus ? sd : -sd)];
+    if (diff)
+      acc = h * ((T)(8.0 / 15.0) * tay - (xin - xc) * (T)0.2);
+    else
+      acc = h * ((T)(8.0 / 15.0) * tay - xin * (T)0.2 - xc * (T)(1.0 / 3.0));
+  } else {
+    // four fine cells of the finer block that covers this cell's quadrant
+    const int fb = g.fnb[(b * 6 + f) * 4 + (u >= BS / 2 ? 1 : 0) + (v >= BS / 2 ? 2 : 0)];
+    const T hf = g.h[fb];
+    const size_t fo = (size_t)fb * BLEN;
+    const int lc = minus ? 7 : 0, lf = minus ? 6 : 1;
+    T xfc[4], xff[4];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const bool on = k0 + u < len;
-      ci[u] = on ? g.ell_col[eo + (size_t)(k0 + u) * 64] : 0;
-      cv[u] = on ? g.ell_val[eo + (size_t)(k0 + u) * 64] : (T)0;
+    for (int q = 0; q < 4; ++q) {
+      const int f1 = (u & 3) * 2 + (q & 1), f2 = (v & 3) * 2 + (q >> 1);
+      xfc[q] = x[fo + lc * sd + f1 * st1 + f2 * st2];
+      xff[q] = x[fo + lf * sd + f1 * st1 + f2 * st2];
     }
+    const int k1 = taylor_kind(u), k2 = taylor_kind(v);
+    auto own = [&](int o1, int o2) { return xs[cl + o1 * st1 + o2 * st2]; };
 #pragma unroll
-    for (int u = 0; u < U; ++u) xv[u] = k0 + u < len ? x[ci[u]] : xc;
-    if (sizeof(T) == 8) {
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (k0 + u < len) acc += cv[u] * xv[u];
-    } else {
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (k0 + u < len) acc += cv[u] * (xv[u] - xc);
+    for (int q = 0; q < 4; ++q) {
+      const T s1 = (q & 1) ? (T)1 : (T)(-1), s2 = (q & 2) ? (T)1 : (T)(-1);
+      const T tay = taylor_sum<T>(own, k1, k2, s1, s2, ref);
+      if (diff)
+        acc += hf * ((xfc[q] - xc) * (T)(1.0 / 3.0) + (xff[q] - xc) * (T)0.2 -
+                     (T)(8.0 / 15.0) * tay);
+      else
+        acc += hf * (xfc[q] * (T)(1.0 / 3.0) + xff[q] * (T)0.2 - (T)(8.0 / 15.0) * tay);
     }
   }
-  if (len > 0) g.corr[ch * 64 + lane] = acc;
+  ys[cl] += acc;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 }
 
 // ---------------------------------------------------------------------------
@@ -204,6 +275,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
   constexpr bool NEED_A = MODE != SPMV_PLAIN;
   __shared__ T xs[4][BLEN];
   __shared__ T ys[IFACE ? 4 : 1][IFACE ? BLEN : 1];
+  __shared__ T pl[IFACE ? 4 : 1][IFACE ? 64 : 1];
   if (MODE != SPMV_PLAIN && MODE != SPMV_RESID && st->status != ST_RUNNING) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ix = lane & 7, iy = lane >> 3;
@@ -267,21 +339,11 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
     }
     __syncthreads();
     if (valid) {
-      // interface-row sums of k_iface_rows (<= 296 rows = 5 chunks per block);
-      // every load is issued before the first add
-      constexpr int MAXCH = 5;
-      const int c0 = g.ich_ptr[b], c1 = g.ich_ptr[b + 1];
-      int meta[MAXCH];
-      T cs[MAXCH];
-#pragma unroll
-      for (int j = 0; j < MAXCH; ++j) {
-        const bool on = c0 + j < c1;
-        meta[j] = on ? g.ell_meta[(c0 + j) * 64 + lane] : 0;
-        cs[j] = on ? g.corr[(c0 + j) * 64 + lane] : (T)0;
+      const int par = g.bpar[b];
+      for (int f = 0; f < 6; ++f) {
+        const int fk = g.fkind[b * 6 + f];
+        if (fk >= FACE_COARSE) iface_face<T>(g, x, xs[w], pl[w], ys[w], b, f, fk, par, lane);
       }
-#pragma unroll
-      for (int j = 0; j < MAXCH; ++j)
-        if ((meta[j] >> 16) > 0) ys[w][meta[j] & 0xffff] += cs[j];
     }
     __syncthreads();
     if (valid) {
@@ -398,8 +460,6 @@ inline void spmv_launch_iface(hipStream_t s, const DevGrid<T>& g, const T* x, T*
                               T* c, T* c2, SolverState* st, double* part) {
   const int nwg = (g.nb + 3) / 4;
   if (g.has_iface) {
-    const bool gated = MODE != SPMV_PLAIN && MODE != SPMV_RESID;
-    k_iface_rows<T><<<(g.nch + 3) / 4, WG_THREADS, 0, s>>>(g, x, gated ? st : nullptr);
     k_spmv<T, MODE, BM, true><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
   } else {
     k_spmv<T, MODE, BM, false><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);

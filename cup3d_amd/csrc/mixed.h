@@ -32,6 +32,10 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
                 const KrylovVectors<double>& v64, SolverState* st, SolverState* h_st,
                 double* part, int origin_block, const double* d_rhs, double* d_x,
                 const double* d_x0, cup3d_solve_report* rep, std::string& err);
+// y = A x with the FP32 operator of the inner solve (x, y FP64 device vectors).
+int mixed_apply_operator32(MixedWorkspace& M, hipStream_t s, const DevGrid<float>& g32,
+                           SolverState* st, double* part, const double* d_x, double* d_y,
+                           std::string& err);
 int mixed_apply_precond(MixedWorkspace& M, hipStream_t s, int precision, const DevGrid<float>& g32,
                         const double* d_p, double* d_z, std::string& err);
 int mixed_time_kernel(MixedWorkspace& M, hipStream_t s, int which, int reps,

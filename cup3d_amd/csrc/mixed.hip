@@ -644,6 +644,18 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
   return status;
 }
 
+int mixed_apply_operator32(MixedWorkspace& M, hipStream_t s, const DevGrid<float>& g32,
+                           SolverState* st, double* part, const double* d_x, double* d_y,
+                           std::string& err) {
+  float* in = reinterpret_cast<float*>(M.d_res64);
+  float* out = in + M.n;
+  k_cast64to32<<<grid_1d(M.n), WG_THREADS, 0, s>>>(M.n, d_x, in);
+  spmv32<SPMV_PLAIN>(s, g32, in, out, nullptr, st, part);
+  k_cast32to64<<<grid_1d(M.n), WG_THREADS, 0, s>>>(M.n, out, d_y);
+  MCHECK(hipGetLastError());
+  return 0;
+}
+
 int mixed_apply_precond(MixedWorkspace& M, hipStream_t s, int precision, const DevGrid<float>& g32,
                         const double* d_p, double* d_z, std::string& err) {
   float* in = reinterpret_cast<float*>(M.d_res64);

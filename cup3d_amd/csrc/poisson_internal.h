@@ -80,19 +80,14 @@ struct DevGrid {
   const T* h = nullptr;             // [nb] cell size
   const T* invh = nullptr;          // [nb]
   const double* h3 = nullptr;       // [nb] h^3 (always FP64: mean constraint)
-  // coarse-fine interface rows, sliced ELL: the rows of block b form chunks
-  // [ich_ptr[b], ich_ptr[b+1]) of 64 rows (one per lane); chunk c holds
-  // ell_meta[64 c + lane] = cell | len << 16 (len 0: no row) and its entries
-  // slot-major at ell_off[c] + 64 k + lane, k < len <= ell_k[c]
-  const int* ich_ptr = nullptr;     // [nb+1]
-  const int* ich_blk = nullptr;     // [nchunks] block of the chunk
-  int nch = 0;                      // nchunks
-  T* corr = nullptr;                // [nchunks * 64] interface-row sums of the current SpMV
-  const int* ell_meta = nullptr;    // [nchunks * 64]
-  const int* ell_off = nullptr;     // [nchunks]
-  const int* ell_k = nullptr;       // [nchunks]
-  const int* ell_col = nullptr;     // global cell index
-  const T* ell_val = nullptr;
+  // coarse-fine faces, applied matrix-free by the SpMV: kind of face f of
+  // block b (FaceKind), its partners (coarse: fnb[4(6b+f)]; finer: the four
+  // children fnb[4(6b+f) + q], q = (t1 half) + 2 (t2 half)), and the parity
+  // of the block index (bit a = index[a] odd) that places a fine block in
+  // its coarse neighbour
+  const int8_t* fkind = nullptr;    // [nb*6]
+  const int* fnb = nullptr;         // [nb*24]
+  const int8_t* bpar = nullptr;     // [nb]
   long long bmean_row = -1;         // cell replaced by sum h^3 x (bMean == 1)
   int bmean = 1;
   int has_iface = 0;                // any coarse-fine interface rows
@@ -105,9 +100,9 @@ struct HostTopology {
   std::vector<int8_t> kind;         // nb*6
   std::vector<double> h, invh, h3;
   std::vector<int> irow_ptr, irow_cell, irow_ent, ent_col;  // CSR form of the interface rows
-  std::vector<double> ent_val;
-  std::vector<int> ich_ptr, ich_blk, ell_meta, ell_off, ell_k, ell_col;  // device (sliced ELL) form
-  std::vector<double> ell_val;
+  std::vector<double> ent_val;                              // (debug matrix, introspection)
+  std::vector<int> fnb;             // nb*24 coarse/fine partners per face
+  std::vector<int8_t> bpar;         // nb index parities
   long long bmean_row = -1;
   int origin_block = -1;
   long long n_coarse_faces = 0, n_fine_faces = 0;

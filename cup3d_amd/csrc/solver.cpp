@@ -74,8 +74,7 @@ struct cup3d_poisson {
   std::vector<int> blocks;  // nb x 4
   int nb = 0;
   long long n = 0;
-  DevBuf d_nbr, d_h, d_invh, d_h3, d_hf, d_invhf, d_ich_ptr, d_ich_blk, d_ell_meta, d_ell_off,
-      d_ell_k, d_ell_col, d_ell_val, d_ell_valf, d_corr64, d_corr32, d_blocks;
+  DevBuf d_nbr, d_h, d_invh, d_h3, d_hf, d_invhf, d_fkind, d_fnb, d_bpar, d_blocks;
   DevGrid<double> g64;
   DevGrid<float> g32;
 
@@ -243,30 +242,20 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
   HostTopology& T = H->topo;
   H->nb = (int)nblocks;
   H->n = (long long)nblocks * BLEN;
-  std::vector<float> hf(T.h.begin(), T.h.end()), invhf(T.invh.begin(), T.invh.end()),
-      valf(T.ell_val.begin(), T.ell_val.end());
+  std::vector<float> hf(T.h.begin(), T.h.end()), invhf(T.invh.begin(), T.invh.end());
   if (upload(H, H->d_nbr, T.nbr.data(), T.nbr.size() * sizeof(int)) ||
       upload(H, H->d_h, T.h.data(), T.h.size() * sizeof(double)) ||
       upload(H, H->d_invh, T.invh.data(), T.invh.size() * sizeof(double)) ||
       upload(H, H->d_h3, T.h3.data(), T.h3.size() * sizeof(double)) ||
       upload(H, H->d_hf, hf.data(), hf.size() * sizeof(float)) ||
       upload(H, H->d_invhf, invhf.data(), invhf.size() * sizeof(float)) ||
-      upload(H, H->d_ich_ptr, T.ich_ptr.data(), T.ich_ptr.size() * sizeof(int)) ||
-      upload(H, H->d_ich_blk, T.ich_blk.data(), T.ich_blk.size() * sizeof(int)) ||
-      upload(H, H->d_ell_meta, T.ell_meta.data(), T.ell_meta.size() * sizeof(int)) ||
-      upload(H, H->d_ell_off, T.ell_off.data(), T.ell_off.size() * sizeof(int)) ||
-      upload(H, H->d_ell_k, T.ell_k.data(), T.ell_k.size() * sizeof(int)) ||
-      upload(H, H->d_ell_col, T.ell_col.data(), T.ell_col.size() * sizeof(int)) ||
-      upload(H, H->d_ell_val, T.ell_val.data(), T.ell_val.size() * sizeof(double)) ||
-      upload(H, H->d_ell_valf, valf.data(), valf.size() * sizeof(float)) ||
+      upload(H, H->d_fkind, T.kind.data(), T.kind.size() * sizeof(int8_t)) ||
+      upload(H, H->d_fnb, T.fnb.data(), T.fnb.size() * sizeof(int)) ||
+      upload(H, H->d_bpar, T.bpar.data(), T.bpar.size() * sizeof(int8_t)) ||
       upload(H, H->d_blocks, H->blocks.data(), H->blocks.size() * sizeof(int)))
     return -1;
   const long long origin_row = T.origin_block >= 0 ? (long long)T.origin_block * BLEN : -1;
-  const size_t nslots = T.ich_blk.size() * 64;
-  if (H->d_corr64.alloc(nslots * sizeof(double)) != hipSuccess ||
-      H->d_corr32.alloc(nslots * sizeof(float)) != hipSuccess)
-    return fail(H, "hipMalloc(interface sums) failed");
-  auto fill = [&](auto& g, const void* h, const void* invh, const void* val, void* corr) {
+  auto fill = [&](auto& g, const void* h, const void* invh) {
     using G = std::remove_reference_t<decltype(g)>;
     using TT = std::remove_const_t<std::remove_pointer_t<decltype(G::h)>>;
     g.nb = H->nb;
@@ -275,21 +264,15 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
     g.h = static_cast<const TT*>(h);
     g.invh = static_cast<const TT*>(invh);
     g.h3 = H->d_h3.as<double>();
-    g.ich_ptr = H->d_ich_ptr.as<int>();
-    g.ich_blk = H->d_ich_blk.as<int>();
-    g.nch = (int)T.ich_blk.size();
-    g.corr = static_cast<TT*>(corr);
-    g.ell_meta = H->d_ell_meta.as<int>();
-    g.ell_off = H->d_ell_off.as<int>();
-    g.ell_k = H->d_ell_k.as<int>();
-    g.ell_col = H->d_ell_col.as<int>();
-    g.ell_val = static_cast<const TT*>(val);
+    g.fkind = H->d_fkind.as<int8_t>();
+    g.fnb = H->d_fnb.as<int>();
+    g.bpar = H->d_bpar.as<int8_t>();
     g.bmean = H->cfg.bmean;
     g.has_iface = T.irow_cell.empty() ? 0 : 1;
     g.bmean_row = (H->cfg.bmean == 1 || H->cfg.bmean == 3) ? origin_row : -1;
   };
-  fill(H->g64, H->d_h.p, H->d_invh.p, H->d_ell_val.p, H->d_corr64.p);
-  fill(H->g32, H->d_hf.p, H->d_invhf.p, H->d_ell_valf.p, H->d_corr32.p);
+  fill(H->g64, H->d_h.p, H->d_invh.p);
+  fill(H->g32, H->d_hf.p, H->d_invhf.p);
   // FP64 Krylov vectors: x[3], r, rhat, p, nu, t, z1, z2, b
   const size_t n = (size_t)H->n;
   if (H->d_vec64.alloc(11 * n * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(vectors) failed");
@@ -586,6 +569,17 @@ int cup3d_debug_cnn_gradient(cup3d_poisson_t H, int k, float res_weight, float* 
   if (!H || !grad || !loss) return -1;
   HIPCHECK(hipSetDevice(H->device));
   return cnn_gradient(H->cnn, H->stream, k, res_weight, grad, loss, H->err);
+}
+
+int cup3d_debug_apply_operator32(cup3d_poisson_t H, const double* d_x, double* d_y) {
+  if (!H || !d_x || !d_y) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  HIPCHECK(hipSetDevice(H->device));
+  if (mixed_apply_operator32(H->mixed, H->stream, H->g32, H->d_state.as<SolverState>(),
+                             H->d_part.as<double>(), d_x, d_y, H->err) != 0)
+    return -1;
+  HIPCHECK(hipStreamSynchronize(H->stream));
+  return 0;
 }
 
 int cup3d_debug_time_kernel(cup3d_poisson_t H, int which, int reps, double* avg_ms,

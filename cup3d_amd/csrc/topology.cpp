@@ -80,64 +80,6 @@ void taylor_patch(int k1, int k2, double s1, double s2, double W[5][5]) {
 
 }  // namespace
 
-// Sliced-ELL copy of the interface rows for the SpMV kernel: per block, rows
-// sorted by length (longest first, then by cell) and cut into chunks of 64,
-// one row per lane; a chunk's entries are stored slot-major so that the
-// lanes' k-th entries are contiguous and all slots can be loaded at once.
-// The entries of a row keep their CSR order, so the row sums are formed in
-// the same order as from the CSR tables.
-static void build_interface_ell(HostTopology& T) {
-  const int nb = T.nb;
-  T.ich_ptr.assign(nb + 1, 0);
-  T.ich_blk.clear();
-  T.ell_meta.clear();
-  T.ell_off.clear();
-  T.ell_k.clear();
-  T.ell_col.clear();
-  T.ell_val.clear();
-  std::vector<int> order;
-  for (int b = 0; b < nb; ++b) {
-    const int r0 = T.irow_ptr[b], r1 = T.irow_ptr[b + 1];
-    order.resize(r1 - r0);
-    for (int r = r0; r < r1; ++r) order[r - r0] = r;
-    auto len = [&](int r) { return T.irow_ent[r + 1] - T.irow_ent[r]; };
-    std::sort(order.begin(), order.end(), [&](int a, int c) {
-      return len(a) != len(c) ? len(a) > len(c) : T.irow_cell[a] < T.irow_cell[c];
-    });
-    for (size_t c0 = 0; c0 < order.size(); c0 += 64) {
-      const size_t m = std::min<size_t>(64, order.size() - c0);
-      int K = 0;
-      for (size_t j = 0; j < m; ++j) K = std::max(K, len(order[c0 + j]));
-      const size_t off = T.ell_col.size();
-      T.ell_off.push_back((int)off);
-      T.ell_k.push_back(K);
-      T.ich_blk.push_back(b);
-      T.ell_col.resize(off + (size_t)K * 64, 0);
-      T.ell_val.resize(off + (size_t)K * 64, 0.0);
-      for (size_t j = 0; j < 64; ++j) {
-        if (j >= m) {
-          T.ell_meta.push_back(0);
-          continue;
-        }
-        const int r = order[c0 + j];
-        T.ell_meta.push_back(T.irow_cell[r] | (len(r) << 16));
-        for (int k = 0; k < len(r); ++k) {
-          T.ell_col[off + (size_t)k * 64 + j] = T.ent_col[T.irow_ent[r] + k];
-          T.ell_val[off + (size_t)k * 64 + j] = T.ent_val[T.irow_ent[r] + k];
-        }
-      }
-    }
-    T.ich_ptr[b + 1] = (int)T.ell_k.size();
-  }
-}
-
-// The SpMV adds at most 5 chunks (296 surface cells) per block.
-static bool interface_chunks_fit(const HostTopology& T) {
-  for (int b = 0; b < T.nb; ++b)
-    if (T.ich_ptr[b + 1] - T.ich_ptr[b] > 5) return false;
-  return true;
-}
-
 bool build_topology(const TopologyInput& in, HostTopology& T, std::string& err) {
   const int nb = in.nb;
   T = HostTopology();
@@ -326,14 +268,17 @@ bool build_topology(const TopologyInput& in, HostTopology& T, std::string& err) 
     }
     T.irow_ptr[b + 1] = (int)T.irow_cell.size();
   }
-  build_interface_ell(T);
-  if (!interface_chunks_fit(T)) {
-    err = "more than 296 interface rows in one block";
-    return false;
-  }
-  if (T.ell_col.size() > 0x7fffffffULL) {
-    err = "grid too large for 32-bit interface table offsets";
-    return false;
+  // partners and parities for the matrix-free coarse-fine stencils
+  T.fnb.assign(size_t(nb) * 24, -1);
+  T.bpar.assign(nb, 0);
+  for (int b = 0; b < nb; ++b) {
+    const int* B = in.blocks + 4 * b;
+    T.bpar[b] = int8_t((B[1] & 1) | ((B[2] & 1) << 1) | ((B[3] & 1) << 2));
+    for (int f = 0; f < 6; ++f) {
+      if (T.kind[b * 6 + f] == FACE_COARSE) T.fnb[(b * 6 + f) * 4] = coarse[b * 6 + f];
+      if (T.kind[b * 6 + f] == FACE_FINE)
+        for (int q = 0; q < 4; ++q) T.fnb[(b * 6 + f) * 4 + q] = fine[(b * 6 + f) * 4 + q];
+    }
   }
   return true;
 }

@@ -36,6 +36,11 @@ int64_t cup3d_debug_operator_csr(const cup3d_poisson_config* cfg, int64_t nblock
  * the algorithmic bytes (vectors read + written) and flops of one launch
  * (GEMM flops count the MFMA passes actually issued).
  * Must follow a solve on the current topology. Returns 0 or < 0. */
+/* y = A x with the FP32 operator of the mixed-precision inner solve (x cast
+ * to FP32, y cast back; FP64 device vectors), for checking it against the
+ * FP64 operator. */
+int cup3d_debug_apply_operator32(cup3d_poisson_t h, const double* d_x, double* d_y);
+
 int cup3d_debug_time_kernel(cup3d_poisson_t h, int which, int reps, double* avg_ms,
                             double* bytes, double* flops);
 

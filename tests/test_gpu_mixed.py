@@ -1,13 +1,14 @@
 """Mixed-precision form on the GPU: FP16/BF16 block-Jacobi GEMM with FP32
 accumulation, FP32 Krylov vectors, FP64 refinement. The error against the FP64
 oracle is stated by each assertion."""
+import ctypes
 import numpy as np
 import pyoracle
 import pytest
 import torch
 
 from cup3d_amd import workloads as W
-from cup3d_amd.poisson import PoissonSolver, solver_for
+from cup3d_amd.poisson import lib, PoissonSolver, solver_for
 from helpers import school4_small, fft_solution_periodic, grid_cases, random_field, tiny_amr
 
 pytestmark = pytest.mark.gpu
@@ -83,6 +84,25 @@ def test_mixed_school4_residual_checked_by_oracle(gpu, prec):
     O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
     r = b.reshape(-1) - O.spmv(x.cpu().numpy())
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.0001e-9
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp32_operator_matches_fp64(gpu, name):
+    """The inner solve's FP32 operator (difference form, matrix-free
+    coarse-fine faces) against the FP64 operator, which the oracle pins:
+    agreement to FP32 rounding of the input and the stencil sums."""
+    blocks, bpd, h0, bc = CASES[name]
+    S = PoissonSolver(bpd, h0, bc, precision="mixed_fp16", bmean=0).set_topology(blocks)
+    x = dev(random_field(len(blocks), 11))
+    y64 = torch.zeros_like(x)
+    y32 = torch.zeros_like(x)
+    S.apply_operator(x, y64)
+    f = lib().cup3d_debug_apply_operator32
+    f.argtypes = [ctypes.c_void_p] * 3
+    assert f(S.h, x.data_ptr(), y32.data_ptr()) == 0
+    torch.cuda.synchronize()
+    err = float((y32 - y64).abs().max() / y64.abs().max())
+    assert err < 2e-6, err
 
 
 def test_mixed_c1_error_vs_exact(gpu):
