@@ -541,10 +541,13 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
   // recurrence accurate), so the Krylov space built so far is kept. A new
   // refinement step (restart) follows only a breakdown or the iteration cap.
 #ifndef CUP3D_REPLACE_DROP
-#define CUP3D_REPLACE_DROP 1e-2
+#define CUP3D_REPLACE_DROP 1e-1
 #endif
   constexpr double REPLACE_DROP = CUP3D_REPLACE_DROP;  // 0: restart at every stop
-  constexpr int STALL_ITERS = 200;
+#ifndef CUP3D_STALL_ITERS
+#define CUP3D_STALL_ITERS 50
+#endif
+  constexpr int STALL_ITERS = CUP3D_STALL_ITERS;
   while (!done && steps <= cfg.max_refine) {
     const double scale = rnorm;
     const double inner_rel = std::max(cfg.inner_tol, 0.5 * tol / scale);
