@@ -191,11 +191,13 @@ __device__ __forceinline__ T taylor_sum(F&& get, int k1, int k2, T s1, T s2, T r
 
 // Contribution of coarse-fine face f of block b to the rows of its 64 face
 // cells (lane = cell (t1, t2) = (lane & 7, lane >> 3)), added into ys.
-// xs holds block b, pl is this wavefront's 64-value staging plane.
+// xs holds block b, pl is this wavefront's 64-value staging plane; nb0 is the
+// face's first partner (the coarse block), nbq the finer block of this
+// lane's quadrant (both prefetched by the caller).
 template <typename T>
 __device__ __forceinline__ void iface_face(const DevGrid<T>& g, const T* __restrict__ x,
                                            const T* xs, T* pl, T* ys, int b, int f, int fk,
-                                           int par, int lane) {
+                                           int par, int lane, int nb0, int nbq) {
   const int d = f >> 1;
   const bool minus = (f & 1) == 0;
   const int t1 = d == 0 ? 1 : (d == 1 ? 2 : 0), t2 = d == 0 ? 2 : (d == 1 ? 0 : 1);
@@ -209,7 +211,7 @@ __device__ __forceinline__ void iface_face(const DevGrid<T>& g, const T* __restr
   T acc = 0;
   if (fk == FACE_COARSE) {
     // stage the coarse block's face layer, [t2][t1]
-    const int cb = g.fnb[(b * 6 + f) * 4];
+    const int cb = nb0;
     pl[lane] = x[(size_t)cb * BLEN + (minus ? 7 : 0) * sd + u * st1 + v * st2];
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -226,7 +228,7 @@ __device__ __forceinline__ void iface_face(const DevGrid<T>& g, const T* __restr
       acc = h * ((T)(8.0 / 15.0) * tay - xin * (T)0.2 - xc * (T)(1.0 / 3.0));
   } else {
     // four fine cells of the finer block that covers this cell's quadrant
-    const int fb = g.fnb[(b * 6 + f) * 4 + (u >= BS / 2 ? 1 : 0) + (v >= BS / 2 ? 2 : 0)];
+    const int fb = nbq;
     const T hf = g.h[fb];
     const size_t fo = (size_t)fb * BLEN;
     const int lc = minus ? 7 : 0, lf = minus ? 6 : 1;
@@ -284,6 +286,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
   const size_t off = (size_t)b * BLEN;
   T col[BS], av[BS], hx[BS], hy[BS], zlo = 0, zhi = 0;
   T h = 0;
+  int fkv[IFACE ? 6 : 1], fn0[IFACE ? 6 : 1], fnq[IFACE ? 6 : 1];
   if (valid) {
     const int* nb6 = g.nbr + b * 6;
     const int nxm = nb6[0], nxp = nb6[1], nym = nb6[2], nyp = nb6[3], nzm = nb6[4], nzp = nb6[5];
@@ -306,6 +309,15 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
     }
     zlo = nzm >= 0 ? x[(size_t)nzm * BLEN + 7 * 64 + lane] : col[0];
     zhi = nzp >= 0 ? x[(size_t)nzp * BLEN + lane] : col[BS - 1];
+    if (IFACE) {  // face kinds and partners, loaded with the column
+      const int fq = (ix >= BS / 2 ? 1 : 0) + (iy >= BS / 2 ? 2 : 0);
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        fkv[f] = g.fkind[b * 6 + f];
+        fn0[f] = g.fnb[(b * 6 + f) * 4];
+        fnq[f] = g.fnb[(b * 6 + f) * 4 + fq];
+      }
+    }
 #pragma unroll
     for (int iz = 0; iz < BS; ++iz) xs[w][iz * 64 + lane] = col[iz];
   }
@@ -340,10 +352,10 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
     __syncthreads();
     if (valid) {
       const int par = g.bpar[b];
-      for (int f = 0; f < 6; ++f) {
-        const int fk = g.fkind[b * 6 + f];
-        if (fk >= FACE_COARSE) iface_face<T>(g, x, xs[w], pl[w], ys[w], b, f, fk, par, lane);
-      }
+#pragma unroll
+      for (int f = 0; f < 6; ++f)
+        if (fkv[f] >= FACE_COARSE)
+          iface_face<T>(g, x, xs[w], pl[w], ys[w], b, f, fkv[f], par, lane, fn0[f], fnq[f]);
     }
     __syncthreads();
     if (valid) {
