@@ -32,25 +32,30 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // ---------------------------------------------------------------------------
 // K1 / K3: fused vector update + block-Jacobi GEMM on the FP64 matrix cores.
 //
-// A workgroup owns 16 blocks. Their updated vectors (p or s) are written to
-// HBM and staged in LDS as 16 rows of 512 doubles (row stride 514, which
-// keeps the fragment reads free of bank conflicts). Each wavefront computes
-// 128 output cells x 16 blocks as 8 tiles of v_mfma_f64_16x16x4f64:
+// A workgroup of 8 wavefronts owns 32 blocks (one workgroup per CU at 8192
+// blocks). Their updated vectors (p or s) are written to HBM and staged in
+// LDS as 32 rows of 512 doubles (row stride 514, which keeps the fragment
+// reads free of bank conflicts). Each wavefront computes 64 output cells x
+// 32 blocks as 4 x 2 tiles of v_mfma_f64_16x16x4f64:
 //   D[j][i] += sum_k V[j][k] * P[k][i]     (j: block, i: output cell)
 // with the A operand (V) read from LDS and the B operand (P_inv) streamed
 // from L2 in a fragment-packed layout: fragment (it, kt) holds
-// P[4 kt + l/16][16 it + l%16] in lane l, 64 consecutive doubles.
+// P[4 kt + l/16][16 it + l%16] in lane l, 64 consecutive doubles. P_inv
+// (2 MiB) is streamed once per 32 blocks.
 // The reference computes z_b = invh_b * P_inv p_b with cublasDgemm(OP_T, OP_N)
 // + blockDscal (BiCGSTAB.cu:526-528,553-555); P_inv is symmetric.
 // ---------------------------------------------------------------------------
-constexpr int PC_NB = 16;       // blocks per workgroup
-constexpr int PC_LD = BLEN + 2; // LDS row stride (doubles)
+constexpr int PC_NB = 32;        // blocks per workgroup
+constexpr int PC_W = 8;          // wavefronts per workgroup
+constexpr int PC_T = PC_W * 64;  // threads per workgroup
+constexpr int PC_LD = BLEN + 2;  // LDS row stride (doubles)
+
+typedef double double2_t __attribute__((ext_vector_type(2)));
 
 template <int MODE>
-__global__ __launch_bounds__(WG_THREADS) void k_precond_f64(int nb, const double* __restrict__ Ppk,
-                                                            const double* __restrict__ invh,
-                                                            KrylovVectors<double> v,
-                                                            SolverState* st) {
+__global__ __launch_bounds__(PC_T) void k_precond_f64(int nb, const double* __restrict__ Ppk,
+                                                      const double* __restrict__ invh,
+                                                      KrylovVectors<double> v, SolverState* st) {
   __shared__ double Vs[PC_NB * PC_LD];
   if (MODE != PREC_PLAIN && st->status != ST_RUNNING) return;
   const int j0 = xcd_remap(blockIdx.x, gridDim.x) * PC_NB;
@@ -60,67 +65,107 @@ __global__ __launch_bounds__(WG_THREADS) void k_precond_f64(int nb, const double
   const int pzero = MODE == PREC_P ? st->p_zero : 0;
   const int restart = MODE == PREC_P ? st->restart_pending : 0;
   double* __restrict__ out = MODE == PREC_S ? v.z2 : v.z1;
-  for (int e = threadIdx.x; e < PC_NB * BLEN; e += WG_THREADS) {
-    const int j = e >> 9, k = e & (BLEN - 1);
-    const int b = j0 + j;
-    double val = 0.0;
-    if (b < nb) {
-      const size_t i = (size_t)b * BLEN + k;
-      if (MODE == PREC_P) {
-        // p = r + beta (p - omega nu)   (BiCGSTAB.cu:518-522)
-        const double r = v.r[i];
-        val = pzero ? r : fma(beta, fma(-omega, v.nu[i], v.p[i]), r);
-        v.p[i] = val;
-        if (restart) v.rhat[i] = r;
-      } else if (MODE == PREC_S) {
-        // s = r - alpha nu   (BiCGSTAB.cu:546-549)
-        val = fma(-alpha, v.nu[i], v.r[i]);
-        v.r[i] = val;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  // phase 1: wave w owns blocks [4 w, 4 w + 4), two at a time; element u of
+  // a lane covers cells 128 (u % 4) + 2 l, +1 of block 2 h + u / 4. All loads
+  // of a pair are issued before the first use.
+  constexpr int NPB = PC_NB / PC_W;  // 4
+#pragma unroll
+  for (int hh = 0; hh < NPB / 2; ++hh) {
+    double2_t ra[8], rb[8], rc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int b = j0 + NPB * w + 2 * hh + u / 4;
+      const size_t i2 = ((size_t)b * BLEN + 128 * (u & 3) + 2 * l) / 2;
+      if (b < nb) {
+        if (MODE == PREC_P) {
+          ra[u] = reinterpret_cast<const double2_t*>(v.r)[i2];
+          rb[u] = reinterpret_cast<const double2_t*>(v.nu)[i2];
+          rc[u] = reinterpret_cast<const double2_t*>(v.p)[i2];
+        } else if (MODE == PREC_S) {
+          ra[u] = reinterpret_cast<const double2_t*>(v.r)[i2];
+          rb[u] = reinterpret_cast<const double2_t*>(v.nu)[i2];
+        } else {
+          ra[u] = reinterpret_cast<const double2_t*>(v.p)[i2];
+        }
       } else {
-        val = v.p[i];
+        ra[u] = rb[u] = rc[u] = (double2_t){0.0, 0.0};
       }
     }
-    Vs[j * PC_LD + k] = val;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int jj = NPB * w + 2 * hh + u / 4;
+      const int b = j0 + jj;
+      const size_t i2 = ((size_t)b * BLEN + 128 * (u & 3) + 2 * l) / 2;
+      double2_t val = ra[u];
+      if (MODE == PREC_P) {
+        // p = r + beta (p - omega nu)   (BiCGSTAB.cu:518-522)
+        if (!pzero) {
+          val.x = fma(beta, fma(-omega, rb[u].x, rc[u].x), ra[u].x);
+          val.y = fma(beta, fma(-omega, rb[u].y, rc[u].y), ra[u].y);
+        }
+        if (b < nb) {
+          reinterpret_cast<double2_t*>(v.p)[i2] = val;
+          if (restart) reinterpret_cast<double2_t*>(v.rhat)[i2] = ra[u];
+        }
+      } else if (MODE == PREC_S) {
+        // s = r - alpha nu   (BiCGSTAB.cu:546-549)
+        val.x = fma(-alpha, rb[u].x, ra[u].x);
+        val.y = fma(-alpha, rb[u].y, ra[u].y);
+        if (b < nb) reinterpret_cast<double2_t*>(v.r)[i2] = val;
+      }
+      const int k = 128 * (u & 3) + 2 * l;
+      Vs[jj * PC_LD + k] = val.x;
+      Vs[jj * PC_LD + k + 1] = val.y;
+    }
   }
   __syncthreads();
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  // phase 2: wave w owns output cells [64 w, 64 w + 64) of all 32 blocks
   const int lr = l & 15, lk = l >> 4;
-  double4_t acc[8];
+  double4_t acc[4][2];
 #pragma unroll
-  for (int t = 0; t < 8; ++t) acc[t] = (double4_t){0.0, 0.0, 0.0, 0.0};
-  const double* __restrict__ pb = Ppk + (size_t)(8 * w) * 128 * 64 + l;
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int jt = 0; jt < 2; ++jt) acc[t][jt] = (double4_t){0.0, 0.0, 0.0, 0.0};
+  const double* __restrict__ pb = Ppk + (size_t)(4 * w) * 128 * 64 + l;
   // P fragments stream from L2 through a 4-stage register ring: the loads for
   // k-step kt+4 are issued while the MFMAs of kt run, hiding L2 latency.
   constexpr int NST = 4;
-  double bq[NST][8];
+  double bq[NST][4];
 #pragma unroll
-  for (int st_ = 0; st_ < NST; ++st_)
+  for (int s_ = 0; s_ < NST; ++s_)
 #pragma unroll
-    for (int t = 0; t < 8; ++t) bq[st_][t] = pb[((size_t)t * 128 + st_) * 64];
+    for (int t = 0; t < 4; ++t) bq[s_][t] = pb[((size_t)t * 128 + s_) * 64];
   for (int k0 = 0; k0 < 128; k0 += NST) {
 #pragma unroll
-    for (int st_ = 0; st_ < NST; ++st_) {
-      const int kt = k0 + st_;
-      const double av = Vs[lr * PC_LD + 4 * kt + lk];
+    for (int s_ = 0; s_ < NST; ++s_) {
+      const int kt = k0 + s_;
+      const double a0 = Vs[lr * PC_LD + 4 * kt + lk];
+      const double a1 = Vs[(16 + lr) * PC_LD + 4 * kt + lk];
 #pragma unroll
-      for (int t = 0; t < 8; ++t)
-        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bq[st_][t], acc[t], 0, 0, 0);
+      for (int t = 0; t < 4; ++t) {
+        acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bq[s_][t], acc[t][0], 0, 0, 0);
+        acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bq[s_][t], acc[t][1], 0, 0, 0);
+      }
       const int kn = kt + NST < 128 ? kt + NST : kt;
 #pragma unroll
-      for (int t = 0; t < 8; ++t) bq[st_][t] = pb[((size_t)t * 128 + kn) * 64];
+      for (int t = 0; t < 4; ++t) bq[s_][t] = pb[((size_t)t * 128 + kn) * 64];
     }
   }
   // D layout of v_mfma_f64_16x16x4f64: register q of lane l holds
   // D[l/16 + 4q][l%16]
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int b = j0 + lk + 4 * q;
-    if (b < nb) {
-      const double s = invh[b];
+  for (int jt = 0; jt < 2; ++jt)
 #pragma unroll
-      for (int t = 0; t < 8; ++t) out[(size_t)b * BLEN + (8 * w + t) * 16 + lr] = acc[t][q] * s;
+    for (int q = 0; q < 4; ++q) {
+      const int b = j0 + jt * 16 + lk + 4 * q;
+      if (b < nb) {
+        const double s = invh[b];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          out[(size_t)b * BLEN + (4 * w + t) * 16 + lr] = acc[t][jt][q] * s;
+      }
     }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -151,9 +196,9 @@ void launch_precond64(hipStream_t s, int mode, int nb, const double* Ppk, const 
                       const KrylovVectors<double>& v, SolverState* st) {
   const int nwg = wg_for_blocks(nb, PC_NB);
   switch (mode) {
-    case PREC_P: k_precond_f64<PREC_P><<<nwg, WG_THREADS, 0, s>>>(nb, Ppk, invh, v, st); break;
-    case PREC_S: k_precond_f64<PREC_S><<<nwg, WG_THREADS, 0, s>>>(nb, Ppk, invh, v, st); break;
-    default: k_precond_f64<PREC_PLAIN><<<nwg, WG_THREADS, 0, s>>>(nb, Ppk, invh, v, st); break;
+    case PREC_P: k_precond_f64<PREC_P><<<nwg, PC_T, 0, s>>>(nb, Ppk, invh, v, st); break;
+    case PREC_S: k_precond_f64<PREC_S><<<nwg, PC_T, 0, s>>>(nb, Ppk, invh, v, st); break;
+    default: k_precond_f64<PREC_PLAIN><<<nwg, PC_T, 0, s>>>(nb, Ppk, invh, v, st); break;
   }
 }
 

@@ -91,6 +91,20 @@ def test_solve_bmean1_first_iterations(gpu, name):
     assert abs(rep["err_final"] - info["err_final"]) <= 1e-10 * info["err_final"]
 
 
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "amr3_walls", "school4_small"])
+def test_solve_bmean3_first_iterations(gpu, name):
+    """bMeanConstraint = 3 (the origin row replaced by h x, getVec zeroing b
+    and x there): first iterates of the GPU solve against the oracle."""
+    S, O, blocks, h0 = pair(name, 3, tol_rel=0.0, max_iter=6)
+    b = compatible_rhs(blocks, h0, 6)
+    x0 = random_field(len(blocks), 7).reshape(-1) * 1e-3
+    x = dev(x0)
+    rep = S.solve_device(dev(b), x)
+    xo, info = O.bicgstab(b, x0=x0, tol_rel=0.0, max_iter=6)
+    assert rep["status"] == info["status"] == 1 and rep["iterations"] == info["iters"] == 6
+    assert np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) <= 1e-10
+
+
 def test_solve_bmean1_converged_c1(gpu):
     """32^3 periodic Fourier workload, reference default settings: both
     solvers converge, and both land within the tolerance-implied distance of
