@@ -45,17 +45,17 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // The reference computes z_b = invh_b * P_inv p_b with cublasDgemm(OP_T, OP_N)
 // + blockDscal (BiCGSTAB.cu:526-528,553-555); P_inv is symmetric.
 // ---------------------------------------------------------------------------
-constexpr int PC_NB = 32;        // blocks per workgroup
 constexpr int PC_W = 8;          // wavefronts per workgroup
 constexpr int PC_T = PC_W * 64;  // threads per workgroup
 constexpr int PC_LD = BLEN + 2;  // LDS row stride (doubles)
 
 typedef double double2_t __attribute__((ext_vector_type(2)));
 
-template <int MODE>
+template <int MODE, int PC_NB>
 __global__ __launch_bounds__(PC_T) void k_precond_f64(int nb, const double* __restrict__ Ppk,
                                                       const double* __restrict__ invh,
                                                       KrylovVectors<double> v, SolverState* st) {
+  constexpr int JT = PC_NB / 16;  // block tiles
   __shared__ double Vs[PC_NB * PC_LD];
   if (MODE != PREC_PLAIN && st->status != ST_RUNNING) return;
   const int j0 = xcd_remap(blockIdx.x, gridDim.x) * PC_NB;
@@ -66,10 +66,10 @@ __global__ __launch_bounds__(PC_T) void k_precond_f64(int nb, const double* __re
   const int restart = MODE == PREC_P ? st->restart_pending : 0;
   double* __restrict__ out = MODE == PREC_S ? v.z2 : v.z1;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  // phase 1: wave w owns blocks [4 w, 4 w + 4), two at a time; element u of
-  // a lane covers cells 128 (u % 4) + 2 l, +1 of block 2 h + u / 4. All loads
-  // of a pair are issued before the first use.
-  constexpr int NPB = PC_NB / PC_W;  // 4
+  // phase 1: wave w owns blocks [NPB w, NPB w + NPB), two at a time; element
+  // u of a lane covers cells 128 (u % 4) + 2 l, +1 of block 2 h + u / 4. All
+  // loads of a pair are issued before the first use.
+  constexpr int NPB = PC_NB / PC_W;  // 4 or 2
 #pragma unroll
   for (int hh = 0; hh < NPB / 2; ++hh) {
     double2_t ra[8], rb[8], rc[8];
@@ -120,13 +120,13 @@ __global__ __launch_bounds__(PC_T) void k_precond_f64(int nb, const double* __re
     }
   }
   __syncthreads();
-  // phase 2: wave w owns output cells [64 w, 64 w + 64) of all 32 blocks
+  // phase 2: wave w owns output cells [64 w, 64 w + 64) of all PC_NB blocks
   const int lr = l & 15, lk = l >> 4;
-  double4_t acc[4][2];
+  double4_t acc[4][JT];
 #pragma unroll
   for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int jt = 0; jt < 2; ++jt) acc[t][jt] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    for (int jt = 0; jt < JT; ++jt) acc[t][jt] = (double4_t){0.0, 0.0, 0.0, 0.0};
   const double* __restrict__ pb = Ppk + (size_t)(4 * w) * 128 * 64 + l;
   // P fragments stream from L2 through a 4-stage register ring: the loads for
   // k-step kt+4 are issued while the MFMAs of kt run, hiding L2 latency.
@@ -140,13 +140,14 @@ __global__ __launch_bounds__(PC_T) void k_precond_f64(int nb, const double* __re
 #pragma unroll
     for (int s_ = 0; s_ < NST; ++s_) {
       const int kt = k0 + s_;
-      const double a0 = Vs[lr * PC_LD + 4 * kt + lk];
-      const double a1 = Vs[(16 + lr) * PC_LD + 4 * kt + lk];
+      double a[JT];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bq[s_][t], acc[t][0], 0, 0, 0);
-        acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bq[s_][t], acc[t][1], 0, 0, 0);
-      }
+      for (int jt = 0; jt < JT; ++jt) a[jt] = Vs[(16 * jt + lr) * PC_LD + 4 * kt + lk];
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt)
+          acc[t][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[jt], bq[s_][t], acc[t][jt], 0, 0, 0);
       const int kn = kt + NST < 128 ? kt + NST : kt;
 #pragma unroll
       for (int t = 0; t < 4; ++t) bq[s_][t] = pb[((size_t)t * 128 + kn) * 64];
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(PC_T) void k_precond_f64(int nb, const double* __re
   // D layout of v_mfma_f64_16x16x4f64: register q of lane l holds
   // D[l/16 + 4q][l%16]
 #pragma unroll
-  for (int jt = 0; jt < 2; ++jt)
+  for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int b = j0 + jt * 16 + lk + 4 * q;
@@ -192,14 +193,26 @@ void launch_spmv64(hipStream_t s, int mode, const DevGrid<double>& g, const doub
   }
 }
 
+// Blocks per workgroup: 32 halves the L2 stream of P_inv but needs 8192
+// blocks to give every CU a workgroup; smaller grids use 16.
+static inline int precond64_width(int nb) { return nb >= 32 * 256 ? 32 : 16; }
+
+template <int NB>
+static void launch_precond64_w(hipStream_t s, int mode, int nb, const double* Ppk,
+                               const double* invh, const KrylovVectors<double>& v,
+                               SolverState* st) {
+  const int nwg = wg_for_blocks(nb, NB);
+  switch (mode) {
+    case PREC_P: k_precond_f64<PREC_P, NB><<<nwg, PC_T, 0, s>>>(nb, Ppk, invh, v, st); break;
+    case PREC_S: k_precond_f64<PREC_S, NB><<<nwg, PC_T, 0, s>>>(nb, Ppk, invh, v, st); break;
+    default: k_precond_f64<PREC_PLAIN, NB><<<nwg, PC_T, 0, s>>>(nb, Ppk, invh, v, st); break;
+  }
+}
+
 void launch_precond64(hipStream_t s, int mode, int nb, const double* Ppk, const double* invh,
                       const KrylovVectors<double>& v, SolverState* st) {
-  const int nwg = wg_for_blocks(nb, PC_NB);
-  switch (mode) {
-    case PREC_P: k_precond_f64<PREC_P><<<nwg, PC_T, 0, s>>>(nb, Ppk, invh, v, st); break;
-    case PREC_S: k_precond_f64<PREC_S><<<nwg, PC_T, 0, s>>>(nb, Ppk, invh, v, st); break;
-    default: k_precond_f64<PREC_PLAIN><<<nwg, PC_T, 0, s>>>(nb, Ppk, invh, v, st); break;
-  }
+  if (precond64_width(nb) == 32) launch_precond64_w<32>(s, mode, nb, Ppk, invh, v, st);
+  else launch_precond64_w<16>(s, mode, nb, Ppk, invh, v, st);
 }
 
 void launch_update64(hipStream_t s, long long n, const KrylovVectors<double>& v, SolverState* st,

@@ -370,25 +370,33 @@ __global__ void k_cast32to64(long long n, const float* __restrict__ a, double* _
 // ---------------------------------------------------------------------------
 namespace {
 
-#ifndef CUP3D_GEMM_NBW
-#define CUP3D_GEMM_NBW 32
-#endif
 // Blocks per GEMM workgroup: P_inv (512 KiB in FP16) is streamed from L2 once
-// per workgroup, so wider groups cut that traffic; 32 gives one workgroup per
-// CU at 8192 blocks.
-constexpr int GEMM_NBW = CUP3D_GEMM_NBW;
+// per workgroup, so 32 blocks halve that traffic against 16; 32 is also the
+// faster choice on grids too small to give every CU a workgroup (4440 blocks:
+// 14.4 us against 15.9 us for K1).
+static inline int gemm_width(int) { return 32; }
+
+template <typename H, bool SPLIT, int NBW>
+void launch_precond_hw(hipStream_t s, int mode, int nb, const H* P, float unscale,
+                       const float* invh, const KrylovVectors<float>& v, const float* pin,
+                       float* pout, SolverState* st) {
+  const int nwg = (nb + NBW - 1) / NBW;
+  if (mode == PREC_P)
+    k_precond_h<PREC_P, H, NBW, SPLIT><<<nwg, GT, 0, s>>>(nb, P, unscale, invh, v, pin, pout, st);
+  else if (mode == PREC_S)
+    k_precond_h<PREC_S, H, NBW, SPLIT><<<nwg, GT, 0, s>>>(nb, P, unscale, invh, v, pin, pout, st);
+  else
+    k_precond_h<PREC_PLAIN, H, NBW, SPLIT><<<nwg, GT, 0, s>>>(nb, P, unscale, invh, v, pin, pout, st);
+}
 
 template <typename H, bool SPLIT>
 void launch_precond_hh(hipStream_t s, int mode, int nb, const H* P, float unscale,
                        const float* invh, const KrylovVectors<float>& v, const float* pin,
                        float* pout, SolverState* st) {
-  const int nwg = (nb + GEMM_NBW - 1) / GEMM_NBW;
-  if (mode == PREC_P)
-    k_precond_h<PREC_P, H, GEMM_NBW, SPLIT><<<nwg, GT, 0, s>>>(nb, P, unscale, invh, v, pin, pout, st);
-  else if (mode == PREC_S)
-    k_precond_h<PREC_S, H, GEMM_NBW, SPLIT><<<nwg, GT, 0, s>>>(nb, P, unscale, invh, v, pin, pout, st);
+  if (gemm_width(nb) == 32)
+    launch_precond_hw<H, SPLIT, 32>(s, mode, nb, P, unscale, invh, v, pin, pout, st);
   else
-    k_precond_h<PREC_PLAIN, H, GEMM_NBW, SPLIT><<<nwg, GT, 0, s>>>(nb, P, unscale, invh, v, pin, pout, st);
+    launch_precond_hw<H, SPLIT, 16>(s, mode, nb, P, unscale, invh, v, pin, pout, st);
 }
 
 template <typename H>

@@ -135,6 +135,25 @@ def test_mixed_deterministic(gpu):
     assert torch.equal(out[0], out[1])
 
 
+def test_preconditioners_at_full_width(gpu):
+    """8192 blocks select the 32-block FP64 GEMM workgroups (smaller grids use
+    16): the FP64 GEMM against the oracle, the FP16 GEMM against it."""
+    from helpers import uniform
+    blocks, bpd, h0, bc = uniform((32, 16, 16), (2, 2, 2))
+    n = len(blocks) * 512
+    p = random_field(len(blocks), 9).reshape(-1)
+    S64 = PoissonSolver(bpd, h0, bc, bmean=0).set_topology(blocks)
+    z64 = S64.apply_preconditioner(dev(p), torch.zeros(n, dtype=torch.float64, device=gpu))
+    O = pyoracle.Oracle(blocks[:64], (4, 4, 4), h0, (2, 2, 2), 0)
+    zo = O.precond(p[:64 * 512])  # block-local: any 64 blocks at the same level
+    assert np.abs(z64.cpu().numpy()[:64 * 512] - zo).max() <= 1e-13 * np.abs(zo).max()
+    S16 = PoissonSolver(bpd, h0, bc, bmean=0, precision="mixed_fp16").set_topology(blocks)
+    z16 = S16.apply_preconditioner(dev(p), torch.zeros(n, dtype=torch.float64, device=gpu))
+    d = (z16 - z64).reshape(-1, 512)
+    rel = torch.linalg.norm(d, dim=1) / torch.linalg.norm(z64.reshape(-1, 512), dim=1)
+    assert float(rel.max()) < 1e-3
+
+
 @pytest.mark.parametrize("prec", ["mixed_fp16", "mixed_bf16"])
 def test_split_preconditioner_is_linear(gpu, prec):
     """M(u + v) = M(u) + M(v) to the precision of the hi + lo split of the
