@@ -16,7 +16,10 @@ accuracy.
 
   python -m cup3d_amd.learned [--steps 400] [--train 300] [--fresh 40]
                               [--train-steps 20000] [--batch 256] [--res-weight 100]
-                              [--bpd 32 16 16]
+                              [--bpd 32 16 16] [--workload c4|c5] [--fp64-max-iter N]
+
+With --workload c5 the sequence is the moving-blob rhs on the 4-level AMR
+school grid (30.2 M cells) instead.
 
 Prints one JSON line.
 """
@@ -31,14 +34,15 @@ import numpy as np
 
 
 def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, batch=256,
-        lr=1e-3, res_weight=100.0, seed=1, tol=1e-7, log=sys.stderr):
+        lr=1e-3, res_weight=100.0, seed=1, tol=1e-7, log=sys.stderr, workload="c4",
+        fp64_max_iter=3000):
     import torch
 
     from . import workloads as W
     from .poisson import solver_for
 
     dev = torch.device("cuda:0")
-    w = W.c4_uniform_sequence(tuple(bpd))
+    w = W.c4_uniform_sequence(tuple(bpd)) if workload == "c4" else W.c5_amr_school()
     kw = dict(tol_rel=0.0, tol_rhs=tol, bmean=0, max_iter=3000)
     ref = solver_for(w, precision="mixed_fp16", max_refine=40, **kw)
 
@@ -82,8 +86,9 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, bat
     def judge(rhs_, sols_, ts):
         out = {}
         for form in ("fp64", "mixed_fp16"):
-            base = solver_for(w, precision=form, guess="given", **kw)
-            net = solver_for(w, precision=form, guess="cnn", **kw)
+            kf = dict(kw, max_iter=fp64_max_iter) if form == "fp64" else kw
+            base = solver_for(w, precision=form, guess="given", **kf)
+            net = solver_for(w, precision=form, guess="cnn", **kf)
             net.cnn_set_weights(weights)
             it_b, it_n, used, ratio, err, raw, fail_b, fail_n = [], [], 0, [], [], [], 0, 0
             xp = torch.empty(w.ncells, dtype=torch.float64, device=dev)
@@ -151,12 +156,16 @@ def main():
     p.add_argument("--lr", type=float, default=1e-3)
     p.add_argument("--res-weight", type=float, default=100.0,
                    help="weight of the local-residual term of the training loss")
-    p.add_argument("--bpd", type=int, nargs=3, default=[32, 16, 16])
+    p.add_argument("--bpd", type=int, nargs=3, default=[32, 16, 16],
+                   help="blocks per dimension of the uniform sequence")
+    p.add_argument("--workload", default="c4", choices=["c4", "c5"],
+                   help="c4: uniform blob sequence; c5: the 4-level AMR school grid")
+    p.add_argument("--fp64-max-iter", type=int, default=3000)
     a = p.parse_args()
     if not 0 < a.train < a.steps:
         raise SystemExit("need 0 < --train < --steps")
     print(json.dumps(run(a.bpd, a.steps, a.train, a.fresh, a.train_steps, a.batch, a.lr,
-                         a.res_weight)))
+                         a.res_weight, workload=a.workload, fp64_max_iter=a.fp64_max_iter)))
 
 
 if __name__ == "__main__":
