@@ -250,6 +250,9 @@ def main():
             "traffic": TRAFFIC.get((w.name, args.form, KERNELS[dom])),
             "kernel": KERNELS[dom], "avg_us": round(kd["avg_us"], 2),
             "alg_bytes_per_launch": kd["bytes"]}
+    ceiling = copy_ceiling(dev)
+    roof["copy_ceiling_GB/s"] = ceiling
+    roof["frac_of_copy_ceiling"] = round(kd["GB/s"] / ceiling, 4)
     k64 = kernel_times(S64, [1, 2, 3, 4])
     g64 = k64[1]
     out = {
@@ -283,9 +286,11 @@ def main():
                  "gemm_mfma_frac": round(g64["TFLOP/s"] / FP64_MFMA_PEAK, 4),
                  "kernels_us": {KERNELS[k]: round(v["avg_us"], 2) for k, v in k64.items()}},
         "speedup_vs_fp64": round(ms64 / ms, 3),
+        "per_iteration_speedup_vs_fp64": round((ms64 / sm64["iterations_mean"]) /
+                                               (ms / sm["iterations_mean"]), 3),
         "host_to_host": {"ms_per_solve": host_ms, "status": host_status,
                          "note": "cup3d_poisson_solve from host arrays, PCIe copies included"},
-        "hbm_copy_GB/s": copy_ceiling(dev),
+        "hbm_copy_GB/s": ceiling,
         "kernels": {KERNELS[k]: {kk: round(vv, 3) for kk, vv in v.items()}
                     for k, v in kern.items()},
         "roofline": roof,

@@ -466,15 +466,93 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv_fin(long long row, T* __res
   fin_store(st, S);
 }
 
+// bMeanConstraint = 2 (hd_cusparseSpMV, BiCGSTAB.cu:392-411):
+// y = A x + (sum_b h_b^3 sum x) h^3 in every cell. The total is known only
+// once k_spmv has run, so k_spmv stores the plain A x and its partial
+// h^3-sums; k_bmean2_sum reduces them into st->mean_sum, and k_bmean2_apply
+// adds the rank-one term and redoes the epilogue (outputs and partial sums)
+// on the corrected y, for k_spmv_fin to finish as for bMean = 0.
+template <int MODE>
+__global__ __launch_bounds__(WG_THREADS) void k_bmean2_sum(SolverState* st, const double* part,
+                                                           int nparts) {
+  __shared__ SolverState S;
+  double tot[3];
+  fin_load<3>(st, S, part, nparts, tot);
+  if (MODE != SPMV_PLAIN && MODE != SPMV_RESID && S.status != ST_RUNNING) return;
+  if (threadIdx.x == 0) S.mean_sum = tot[0];
+  fin_store(st, S);
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(WG_THREADS) void k_bmean2_apply(DevGrid<T> g, T* __restrict__ y,
+                                                             const T* __restrict__ a,
+                                                             T* __restrict__ c,
+                                                             T* __restrict__ c2,
+                                                             SolverState* st, double* part) {
+  if (MODE != SPMV_PLAIN && MODE != SPMV_RESID && st->status != ST_RUNNING) return;
+  const double red = st->mean_sum;
+  // one thread per 8 consecutive cells (never across a block)
+  const long long j = (long long)blockIdx.x * WG_THREADS + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (j < (long long)g.nb * (BLEN / 8)) {
+    const size_t i = (size_t)j * 8;
+    const double hb = (double)g.h[i / BLEN];
+    const T corr = (T)(red * (hb * hb * hb));
+    if (MODE == SPMV_RESID) {  // only c = a - A x was stored
+      T cv[8];
+      load8(c + i, cv);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        cv[e] -= corr;
+        v[1] += (double)cv[e] * (double)cv[e];
+      }
+      store8(c + i, cv);
+    } else {
+      T yv[8], av[8];
+      load8(y + i, yv);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) yv[e] += corr;
+      store8(y + i, yv);
+      if (MODE != SPMV_PLAIN) load8(a + i, av);
+      if (MODE == SPMV_INIT) {
+        T rr[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          rr[e] = av[e] - yv[e];
+          v[1] += (double)yv[e] * (double)yv[e];
+          v[2] += (double)rr[e] * (double)rr[e];
+        }
+        store8(c + i, rr);
+        store8(c2 + i, rr);
+      } else if (MODE == SPMV_DOT1 || MODE == SPMV_DOT2) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          v[1] += (double)av[e] * (double)yv[e];
+          if (MODE == SPMV_DOT2) v[2] += (double)yv[e] * (double)yv[e];
+        }
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) v[0] = red;  // k_spmv_fin keeps it as mean_sum
+  grid_store_partial<3>(v, part);
+}
+
 // Launch for a grid: picks the bMean and interface variants.
 template <typename T, int MODE, int BM>
 inline void spmv_launch_iface(hipStream_t s, const DevGrid<T>& g, const T* x, T* y, const T* a,
                               T* c, T* c2, SolverState* st, double* part) {
-  const int nwg = (g.nb + 3) / 4;
+  int nwg = (g.nb + 3) / 4;
   if (g.has_iface) {
     k_spmv<T, MODE, BM, true><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
   } else {
     k_spmv<T, MODE, BM, false><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
+  }
+  if (BM == 2) {
+    k_bmean2_sum<MODE><<<1, WG_THREADS, 0, s>>>(st, part, nwg);
+    // nb * 64 threads of 8 cells: (nb + 3) / 4 workgroups again, so the
+    // partial buffer sized for k_spmv holds them
+    nwg = (int)(((long long)g.nb * (BLEN / 8) + WG_THREADS - 1) / WG_THREADS);
+    k_bmean2_apply<T, MODE><<<nwg, WG_THREADS, 0, s>>>(g, y, a, c, c2, st, part);
   }
   k_spmv_fin<T, MODE, BM><<<1, WG_THREADS, 0, s>>>(g.bmean_row, y, a, c, c2, st, part, nwg);
 }
@@ -482,6 +560,7 @@ template <typename T, int MODE>
 inline void spmv_launch(hipStream_t s, const DevGrid<T>& g, const T* x, T* y, const T* a, T* c,
                         T* c2, SolverState* st, double* part) {
   if (g.bmean == 1) spmv_launch_iface<T, MODE, 1>(s, g, x, y, a, c, c2, st, part);
+  else if (g.bmean == 2) spmv_launch_iface<T, MODE, 2>(s, g, x, y, a, c, c2, st, part);
   else if (g.bmean == 3) spmv_launch_iface<T, MODE, 3>(s, g, x, y, a, c, c2, st, part);
   else spmv_launch_iface<T, MODE, 0>(s, g, x, y, a, c, c2, st, part);
 }

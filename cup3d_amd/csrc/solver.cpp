@@ -178,10 +178,6 @@ int cup3d_poisson_create(const cup3d_poisson_config* cfg, cup3d_poisson_t* out) 
   if (c.max_iter <= 0) { g_create_error = "max_iter must be positive"; return -1; }
   if (c.precision < 0 || c.precision > 2) { g_create_error = "unknown precision"; return -1; }
   if (c.guess < 0 || c.guess > 2) { g_create_error = "unknown guess mode"; return -1; }
-  if (c.bmean == 2) {
-    g_create_error = "bMeanConstraint 2 is not supported by the GPU solver";
-    return -1;
-  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     g_create_error = "no HIP device available";

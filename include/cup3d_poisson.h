@@ -134,7 +134,9 @@ int cup3d_poisson_solve(cup3d_poisson_t h, const double* const* rhs_blocks,
 int cup3d_poisson_solve_device(cup3d_poisson_t h, const double* d_rhs, double* d_x,
                                cup3d_solve_report* rep);
 
-/* y = A x with the solver's operator and bMean row (device pointers). */
+/* y = A x with the solver's operator and its bMean handling: the bMean = 1
+ * row, the bMean = 2 term (sum h^3 x) h^3, the bMean = 3 row (device
+ * pointers). */
 int cup3d_poisson_apply_operator(cup3d_poisson_t h, const double* d_x, double* d_y);
 
 /* z = blockdiag(h_b^-1 P_inv) p on the device, in the handle's precision. */

@@ -6,8 +6,8 @@
  * this library applies it matrix-free. cup3d_debug_operator_csr() writes the
  * operator exactly as the SpMV kernel applies it (7-point stencil over
  * same-level faces plus the merged coarse-fine interface rows, bMean = 3 row
- * included; the bMean = 1 row replacement is applied at run time and is not
- * part of it) as CSR with sorted columns, built on the host from the same
+ * included; the bMean = 1 row replacement and the bMean = 2 rank-one term
+ * are applied at run time and are not part of it) as CSR with sorted columns, built on the host from the same
  * topology tables the device uses. It needs no GPU.
  *
  * Call with cap = 0 to get nnz; then with arrays of n+1 / nnz / nnz entries.

@@ -3,12 +3,13 @@
 Bars: operator and preconditioner <= 1e-13 relative (max norm); solver
 <= 1e-10 relative L2 in pressure at the same iteration count wherever the
 recurrence does not amplify rounding (bMean = 0 consistent systems; the first
-iterations of bMean = 1), and for converged bMean = 1 solves both the GPU and
-the oracle within the tolerance-implied distance of the exact solution.
+iterations of bMean = 1, 2 and 3), and for converged bMean = 1 and 2 solves
+within the tolerance-implied distance of the exact solution.
 """
 import numpy as np
 import pyoracle
 import pytest
+import scipy.sparse.linalg as sla
 import torch
 
 from cup3d_amd import workloads as W
@@ -41,7 +42,7 @@ def pair(name, bmean, **kw):
 
 
 @pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("bmean", [0, 1, 3])
+@pytest.mark.parametrize("bmean", [0, 1, 2, 3])
 def test_operator_matches_oracle(gpu, name, bmean):
     S, O, blocks, h0 = pair(name, bmean)
     x = random_field(len(blocks), 1).reshape(-1)
@@ -91,11 +92,13 @@ def test_solve_bmean1_first_iterations(gpu, name):
     assert abs(rep["err_final"] - info["err_final"]) <= 1e-10 * info["err_final"]
 
 
+@pytest.mark.parametrize("bmean", [2, 3])
 @pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "amr3_walls", "school4_small"])
-def test_solve_bmean3_first_iterations(gpu, name):
-    """bMeanConstraint = 3 (the origin row replaced by h x, getVec zeroing b
-    and x there): first iterates of the GPU solve against the oracle."""
-    S, O, blocks, h0 = pair(name, 3, tol_rel=0.0, max_iter=6)
+def test_solve_bmean23_first_iterations(gpu, name, bmean):
+    """bMeanConstraint = 2 (every row plus (sum h^3 x) h^3) and 3 (the origin
+    row replaced by h x, getVec zeroing b and x there): first iterates of the
+    GPU solve against the oracle."""
+    S, O, blocks, h0 = pair(name, bmean, tol_rel=0.0, max_iter=6)
     b = compatible_rhs(blocks, h0, 6)
     x0 = random_field(len(blocks), 7).reshape(-1) * 1e-3
     x = dev(x0)
@@ -103,6 +106,29 @@ def test_solve_bmean3_first_iterations(gpu, name):
     xo, info = O.bicgstab(b, x0=x0, tol_rel=0.0, max_iter=6)
     assert rep["status"] == info["status"] == 1 and rep["iterations"] == info["iters"] == 6
     assert np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) <= 1e-10
+
+
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "tiny_amr3_walls"])
+def test_solve_bmean2_converged(gpu, name):
+    """bMeanConstraint = 2 makes the walls system regular; for a compatible
+    rhs the solution is the one with zero h^3-weighted mean. Known answer:
+    a direct sparse solve with one cell pinned, shifted to that mean."""
+    S, O, blocks, h0 = pair(name, 2, tol_rel=1e-11)
+    b = compatible_rhs(blocks, h0, 8)
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(dev(b), x)
+    assert rep["status"] == 0, rep
+    A = O.scipy_matrix().tocsr()
+    keep = np.ones(O.n, bool)
+    keep[0] = False
+    xe = np.zeros(O.n)
+    xe[keep] = sla.spsolve(A[keep][:, keep].tocsc(), b[keep])
+    h3 = np.repeat(O.h ** 3, 512)
+    xe -= (h3 @ xe) / h3.sum()
+    xg = x.cpu().numpy()
+    assert abs(h3 @ xg) <= 1e-7 * np.abs(h3 * xg).sum()
+    # residual 1e-11 times a condition number below 1e4
+    assert np.linalg.norm(xg - xe) / np.linalg.norm(xe) < 1e-7
 
 
 def test_solve_bmean1_converged_c1(gpu):

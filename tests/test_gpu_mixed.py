@@ -5,6 +5,7 @@ import ctypes
 import numpy as np
 import pyoracle
 import pytest
+import scipy.sparse.linalg as sla
 import torch
 
 from cup3d_amd import workloads as W
@@ -103,6 +104,54 @@ def test_fp32_operator_matches_fp64(gpu, name):
     torch.cuda.synchronize()
     err = float((y32 - y64).abs().max() / y64.abs().max())
     assert err < 2e-6, err
+
+
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "tiny_amr3_walls"])
+def test_fp32_operator_bmean2(gpu, name):
+    """bMeanConstraint = 2 in the FP32 operator: A x + (sum h^3 x) h^3 against
+    the FP64 operator. The field carries a constant 10 so that the rank-one
+    term is well above the tolerance."""
+    blocks, bpd, h0, bc = CASES[name]
+    S = PoissonSolver(bpd, h0, bc, precision="mixed_fp16", bmean=2).set_topology(blocks)
+    x = dev(random_field(len(blocks), 12)) + 10.0
+    y64 = torch.zeros_like(x)
+    y32 = torch.zeros_like(x)
+    S.apply_operator(x, y64)
+    f = lib().cup3d_debug_apply_operator32
+    f.argtypes = [ctypes.c_void_p] * 3
+    assert f(S.h, x.data_ptr(), y32.data_ptr()) == 0
+    torch.cuda.synchronize()
+    h3 = torch.from_numpy(np.repeat((h0 / 2.0 ** np.asarray(blocks)[:, 0]) ** 3, 512)).cuda()
+    term = float((h3 * x).sum()) * h3
+    assert float(term.abs().max() / y64.abs().max()) > 1e-4
+    err = float((y32 - y64).abs().max() / y64.abs().max())
+    assert err < 2e-6, err
+
+
+@pytest.mark.parametrize("prec", ["mixed_fp16", "mixed_bf16"])
+def test_mixed_solve_bmean2(gpu, prec):
+    """Refined solve with bMeanConstraint = 2 on a 3-level walls grid, against
+    the exact solution with zero h^3-weighted mean (direct sparse solve)."""
+    blocks, bpd, h0, bc = CASES["tiny_amr3_walls"]
+    b = compatible_rhs(blocks, h0, 13)
+    S = PoissonSolver(bpd, h0, bc, precision=prec, bmean=2, tol_rel=0.0,
+                      tol_rhs=1e-10).set_topology(blocks)
+    x = torch.zeros(len(blocks) * 512, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(dev(b), x)
+    assert rep["status"] == 0 and rep["true_rel_resid"] <= 1e-10, rep
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 2)
+    A = O.scipy_matrix().tocsr()
+    keep = np.ones(O.n, bool)
+    keep[0] = False
+    xe = np.zeros(O.n)
+    xe[keep] = sla.spsolve(A[keep][:, keep].tocsc(), b[keep])
+    h3 = np.repeat(O.h ** 3, 512)
+    xe -= (h3 @ xe) / h3.sum()
+    err = np.linalg.norm(x.cpu().numpy() - xe) / np.linalg.norm(xe)
+    assert err < 1e-7, err
+    # and the oracle's own bMean = 2 operator confirms the residual
+    r = b - O.spmv(x.cpu().numpy())
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.0001e-10
 
 
 def test_mixed_c1_error_vs_exact(gpu):

@@ -10,7 +10,8 @@ import pyoracle
 import pytest
 import scipy.sparse as sp
 
-from helpers import PER, WALL, amr_sphere, boundary_cells, cell_centres, grid_cases, uniform
+from helpers import (PER, WALL, amr_sphere, boundary_cells, cell_centres, grid_cases,
+                     random_field, uniform)
 
 
 def make(case, bmean=0):
@@ -163,3 +164,23 @@ def test_coarse_fine_rows_known_answer():
     assert abs(r[cc + 72] - tf * (1 / 64)) < 1e-15           # (+1, +1)
     assert abs(r[cc + 56] - tf * (-1 / 64)) < 1e-15          # (-1 in y, +1 in z)
     assert len(r) == 6 + 9
+
+
+@pytest.mark.parametrize("name", ["uniform_periodic_2x3x2", "amr2_walls", "amr3_walls"])
+def test_bmean2_rank_one_term(name):
+    """bMeanConstraint = 2 adds (sum_b h_b^3 sum x) h^3 to every row
+    (BiCGSTAB.cu:392-411). Rows sum to zero, so a constant c gives
+    y = c V h^3 (V the domain volume), and a field f of zero h^3-weighted
+    mean gives A f unchanged: y(f + c) = A f + c V h^3."""
+    blocks, bpd, h0, bc = case = grid_cases()[name]
+    O = make(case, bmean=2)
+    _, h = cell_centres(blocks, h0)
+    h3 = np.repeat(h ** 3, 512)
+    V = np.prod([8 * b * h0 for b in bpd])
+    assert abs(h3.sum() - V) < 1e-12 * V  # the blocks tile the domain
+    f = random_field(len(blocks), 21).reshape(-1)
+    f -= h3 @ f / V
+    for c in (0.0, 1.0, -2.5):
+        y = O.spmv(f + c)
+        expect = O.spmv_raw(f) + c * V * h3
+        assert np.abs(y - expect).max() <= 1e-12 * np.abs(expect).max()

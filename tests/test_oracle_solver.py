@@ -161,3 +161,22 @@ def test_precond_is_scaled_local_inverse():
     out = O.precond(p)
     assert np.abs(out[512:1024] - z).max() < 1e-12
     assert np.abs(out[:512]).max() == 0
+
+
+@pytest.mark.parametrize("name", ["uniform_periodic_2x3x2", "tiny_amr_walls"])
+def test_bicgstab_bmean2_solution(name):
+    """With bMeanConstraint = 2 the operator A + h^3 (h^3)^T is regular; for a
+    compatible rhs (sum b = 0) its solution is the one of A x = b with zero
+    h^3-weighted mean, since 1^T A = 0 forces (h^3)^T x = 0. Known answers:
+    the FFT solution (periodic) and the pinned direct solve (AMR walls)."""
+    blocks, bpd, h0, bc = SMALL[name]()
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 2)
+    rhs = compatible_rhs(O, 23)
+    x, info = O.bicgstab(rhs, tol_rel=1e-12)
+    assert info["status"] == 0
+    if name.startswith("uniform"):
+        xe = fft_solution_periodic(rhs, blocks, bpd, h0, 0)
+    else:
+        xe = direct_solution(O, rhs)
+    # residual 1e-12 times a condition number of about 1e3
+    assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-8
