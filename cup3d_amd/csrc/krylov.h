@@ -239,18 +239,23 @@ __device__ __forceinline__ void iface_face(const DevGrid<T>& g, const T* __restr
       xfc[q] = x[fo + lc * sd + f1 * st1 + f2 * st2];
       xff[q] = x[fo + lf * sd + f1 * st1 + f2 * st2];
     }
+    // The four fine ghosts share this cell's Taylor expansion and differ in
+    // the signs s1, s2 = +-1 of the first-derivative and mixed terms, which
+    // therefore cancel in the sum over the four: sum_q T_q = 4 (P + D2 terms).
     const int k1 = taylor_kind(u), k2 = taylor_kind(v);
-    auto own = [&](int o1, int o2) { return xs[cl + o1 * st1 + o2 * st2]; };
+    const int o1 = k1 == TK_CEN ? -1 : (k1 == TK_BWD ? -2 : 0);
+    const int o2 = k2 == TK_CEN ? -1 : (k2 == TK_BWD ? -2 : 0);
+    auto own = [&](int a1, int a2) { return xs[cl + a1 * st1 + a2 * st2] - ref; };
+    const T tsum = (T)4 * (own(0, 0) + (T)0.03125 * ((own(o1, 0) + own(o1 + 2, 0)) +
+                                                    (own(0, o2) + own(0, o2 + 2))) -
+                           (T)0.0625 * (own(o1 + 1, 0) + own(0, o2 + 1)));
+    T sc = 0, sf = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const T s1 = (q & 1) ? (T)1 : (T)(-1), s2 = (q & 2) ? (T)1 : (T)(-1);
-      const T tay = taylor_sum<T>(own, k1, k2, s1, s2, ref);
-      if (diff)
-        acc += hf * ((xfc[q] - xc) * (T)(1.0 / 3.0) + (xff[q] - xc) * (T)0.2 -
-                     (T)(8.0 / 15.0) * tay);
-      else
-        acc += hf * (xfc[q] * (T)(1.0 / 3.0) + xff[q] * (T)0.2 - (T)(8.0 / 15.0) * tay);
+      sc += xfc[q] - ref;
+      sf += xff[q] - ref;
     }
+    acc = hf * (sc * (T)(1.0 / 3.0) + sf * (T)0.2 - (T)(8.0 / 15.0) * tsum);
   }
   ys[cl] += acc;
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -278,6 +283,12 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
   __shared__ T xs[4][BLEN];
   __shared__ T ys[IFACE ? 4 : 1][IFACE ? BLEN : 1];
   __shared__ T pl[IFACE ? 4 : 1][IFACE ? 64 : 1];
+  // per wavefront: the face partners fnb[24 b .. 24 b + 23], the six face
+  // kinds and the index parities of its block
+  __shared__ int fmeta[IFACE ? 4 : 1][IFACE ? 31 : 1];
+  // with coarse-fine faces the epilogue operand waits in LDS, not registers
+  constexpr bool A_LDS = IFACE && NEED_A;
+  __shared__ T as[A_LDS ? 4 : 1][A_LDS ? BLEN : 1];
   if (MODE != SPMV_PLAIN && MODE != SPMV_RESID && st->status != ST_RUNNING) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ix = lane & 7, iy = lane >> 3;
@@ -286,7 +297,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
   const size_t off = (size_t)b * BLEN;
   T col[BS], av[BS], hx[BS], hy[BS], zlo = 0, zhi = 0;
   T h = 0;
-  int fkv[IFACE ? 6 : 1], fn0[IFACE ? 6 : 1], fnq[IFACE ? 6 : 1];
+  int meta = 0;
   if (valid) {
     const int* nb6 = g.nbr + b * 6;
     const int nxm = nb6[0], nxp = nb6[1], nym = nb6[2], nyp = nb6[3], nzm = nb6[4], nzp = nb6[5];
@@ -309,17 +320,18 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
     }
     zlo = nzm >= 0 ? x[(size_t)nzm * BLEN + 7 * 64 + lane] : col[0];
     zhi = nzp >= 0 ? x[(size_t)nzp * BLEN + lane] : col[BS - 1];
-    if (IFACE) {  // face kinds and partners, loaded with the column
-      const int fq = (ix >= BS / 2 ? 1 : 0) + (iy >= BS / 2 ? 2 : 0);
-#pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        fkv[f] = g.fkind[b * 6 + f];
-        fn0[f] = g.fnb[(b * 6 + f) * 4];
-        fnq[f] = g.fnb[(b * 6 + f) * 4 + fq];
-      }
+    if (IFACE) {  // face kinds and partners, loaded with the column (one per lane)
+      if (lane < 24) meta = g.fnb[(size_t)b * 24 + lane];
+      else if (lane < 30) meta = g.fkind[b * 6 + lane - 24];
+      else if (lane == 30) meta = g.bpar[b];
     }
 #pragma unroll
     for (int iz = 0; iz < BS; ++iz) xs[w][iz * 64 + lane] = col[iz];
+    if (A_LDS) {
+#pragma unroll
+      for (int iz = 0; iz < BS; ++iz) as[w][iz * 64 + lane] = av[iz];
+    }
+    if (IFACE && lane < 31) fmeta[w][lane] = meta;
   }
   __syncthreads();
   T yv[BS];
@@ -351,11 +363,17 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
     }
     __syncthreads();
     if (valid) {
-      const int par = g.bpar[b];
-#pragma unroll
-      for (int f = 0; f < 6; ++f)
-        if (fkv[f] >= FACE_COARSE)
-          iface_face<T>(g, x, xs[w], pl[w], ys[w], b, f, fkv[f], par, lane, fn0[f], fnq[f]);
+      const int par = fmeta[w][30];
+      const int fq = (ix >= BS / 2 ? 1 : 0) + (iy >= BS / 2 ? 2 : 0);
+      // one face at a time: the face code is instantiated once, which keeps
+      // the register count (and so the occupancy) close to the uniform kernel's
+#pragma unroll 1
+      for (int f = 0; f < 6; ++f) {
+        const int fk = __builtin_amdgcn_readfirstlane(fmeta[w][24 + f]);
+        if (fk >= FACE_COARSE)
+          iface_face<T>(g, x, xs[w], pl[w], ys[w], b, f, fk, par, lane, fmeta[w][4 * f],
+                        fmeta[w][4 * f + fq]);
+      }
     }
     __syncthreads();
     if (valid) {
@@ -377,6 +395,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
       // workgroups (possibly on different XCDs) ever store the same address
       if ((BM == 1) && origin && lane == 0 && iz == 0) continue;
       const T yy = yv[iz];
+      if (A_LDS) av[iz] = as[w][iz * 64 + lane];
       if (MODE == SPMV_PLAIN) {
         y[i] = yy;
       } else if (MODE == SPMV_INIT) {
