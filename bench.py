@@ -6,15 +6,23 @@ layout, pressure written back in that layout — to the true relative residual
 consecutive time steps of a moving-blob sequence, each warm-started from the
 previous step's (mean-free) pressure, as PressureProjection feeds the solver.
 
-Default workload: uniform 256x128x128 cells (32x16x16 blocks), domain
-[0,2]x[0,1]^2, walls, rhs = divergence of 256 Gaussian blobs with a
-penalised sphere; the previous pressure is the solution of the step before
-(blob centres 0.005 earlier). Two forms are timed on the same inputs:
+Default workload: the 4-level AMR "school" grid (base 16x8x8 blocks, domain
+[0,2]x[0,1]^2, walls, refined up to 3 levels near the surfaces of 50
+ellipsoids: 59 012 blocks, 30.2 M cells), rhs = divergence of a moving
+Gaussian-blob velocity with penalised ellipsoid interiors; the previous
+pressure is the solution of the step before (blob centres 0.005 earlier).
+--workload c2 selects the uniform 256x128x128 walls grid (256 blobs and a
+penalised sphere), c3 the 3-level AMR sphere grid. The forms timed on the
+same inputs:
   * the FP64 drop-in (the reference algorithm, FP64 MFMA block-Jacobi GEMM,
     the reference's cap of 1000 iterations; its steps that stagnate above
     1e-7 are counted, not hidden);
-  * the chosen form (headline `value`): FP16 split-precision MFMA GEMM with
-    FP32 accumulation, FP32 Krylov vectors, FP64 iterative refinement.
+  * the chosen form: FP16 split-precision MFMA GEMM with FP32 accumulation,
+    FP32 Krylov vectors, FP64 iterative refinement, from the previous
+    pressure (`prev_guess`) and, the headline `value`, from the learned
+    warm start: a 3-layer CNN trained (untimed setup, the library's HIP
+    training kernels) on the (rhs, p_prev, p) triples of the --train steps
+    before the timed ones, with the previous pressure as the fallback.
 The line carries the roofline of the headline form's dominant kernel (HIP
 events on the library's stream) and a CPU baseline (the oracle's restatement
 of the reference CPU solver on a bounded sample of the same workload).
@@ -45,9 +53,13 @@ CALLS_PER_ITER = {2: 1, 3: 2, 4: 2}  # K1 and K3 share the GEMM kernel, K2/K4 th
 COPY_BYTES = 256 << 20  # calibration copy for the FETCH_SIZE / WRITE_SIZE counters
 # HBM bytes per launch from two rocprofv3 counter passes (FETCH_SIZE x 2.000,
 # WRITE_SIZE x 1.000, calibrated on the 256 MiB copy of --kernels-only), kept
-# in profiles/mixed_fp16_uniform_256x128x128_traffic.md: (workload, form,
+# in profiles/mixed_fp16_amr4_school_traffic.md and
+# profiles/mixed_fp16_uniform_256x128x128_traffic.md: (workload, form,
 # kernel) -> bytes.
 TRAFFIC = {
+    ("amr4_school50_blobs64_sequence", "mixed_fp16", "precond_update_K1"): 615.04e6,
+    ("amr4_school50_blobs64_sequence", "mixed_fp16", "spmv_dot_K2"): 402.09e6,
+    ("amr4_school50_blobs64_sequence", "mixed_fp16", "update_K5"): 967.39e6,
     ("uniform_256x128x128_walls_blobs256_sphere", "mixed_fp16", "precond_update_K1"): 88.25e6,
     ("uniform_256x128x128_walls_blobs256_sphere", "mixed_fp16", "spmv_dot_K2"): 53.10e6,
     ("uniform_256x128x128_walls_blobs256_sphere", "mixed_fp16", "update_K5"): 134.3e6,
@@ -57,13 +69,21 @@ TRAFFIC = {
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=16)
+    p.add_argument("--steps", type=int, default=8)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--form", default="mixed_fp16", choices=["fp64", "mixed_fp16", "mixed_bf16"])
-    p.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c5"])
+    p.add_argument("--workload", default="c5", choices=["c1", "c2", "c3", "c5"],
+                   help="c5: 4-level AMR school grid (default); c2: uniform 256x128x128 walls; "
+                        "c3: 3-level AMR sphere; c1: 32^3 periodic")
     p.add_argument("--bmean", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-iters", type=int, default=10)
+    p.add_argument("--train", type=int, default=24,
+                   help="sequence steps before the timed ones whose (rhs, p_prev, p) triples "
+                        "train the warm-start net")
+    p.add_argument("--train-steps", type=int, default=4000, help="Adam steps of the net")
+    p.add_argument("--no-learned", action="store_true",
+                   help="headline form without the learned warm start")
     p.add_argument("--kernels-only", action="store_true",
                    help="launch the timed kernels a few times and a 256 MiB copy (for "
                         "hardware-counter runs), print their names, and exit")
@@ -76,15 +96,44 @@ def make_workload(name):
             "c5": W.c5_amr_school}[name]()
 
 
-def time_form(w, form, bmean, seq, warmup, max_iter):
+def in_solve_profile(S, enable=None):
+    """HIP-event durations (us) of K1, K3, K5 inside the mixed form's solves
+    since the last call (cup3d_debug_profile*), with the sample count."""
+    from cup3d_amd.poisson import lib
+    L = lib()
+    if enable is not None:
+        L.cup3d_debug_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        if L.cup3d_debug_profile(S.h, int(enable)) != 0:
+            raise RuntimeError(L.cup3d_poisson_last_error(S.h).decode())
+        return None
+    L.cup3d_debug_profile_read.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_longlong)]
+    us = (ctypes.c_double * 3)()
+    n = ctypes.c_longlong()
+    if L.cup3d_debug_profile_read(S.h, us, ctypes.byref(n)) != 0:
+        raise RuntimeError("cup3d_debug_profile_read failed")
+    return {"K1": us[0], "K3": us[1], "K5": us[2], "samples": n.value}
+
+
+def time_form(w, form, bmean, seq, warmup, max_iter, weights=None, profile=False):
     """Solve every step of the sequence with `form`, each from the previous
-    step's reference pressure; time the steps after the first `warmup`."""
+    step's reference pressure (or, given net weights, from the learned guess
+    with that pressure as the fallback); time the steps after the first
+    `warmup`. With `profile`, the library also times K1, K3 and K5 with HIP
+    events on its stream during the timed solves (one iteration in eight)."""
     import torch
     from cup3d_amd.poisson import solver_for
-    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-7, precision=form, bmean=bmean, max_iter=max_iter)
+    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-7, precision=form, bmean=bmean, max_iter=max_iter,
+                   guess="given" if weights is None else "cnn")
+    if weights is not None:
+        S.cnn_set_weights(weights)
+    if profile:
+        in_solve_profile(S, True)
     x = torch.empty_like(seq[0]["p_prev"])
     reps, xs, ms = [], [], 0.0
     for k, st in enumerate(seq):
+        if profile and k == warmup:
+            in_solve_profile(S)  # drop the warm-up samples
         x.copy_(st["p_prev"])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -94,6 +143,9 @@ def time_form(w, form, bmean, seq, warmup, max_iter):
             ms += dt * 1e3
             reps.append(rep)
             xs.append(x.clone())
+    if profile:
+        S.prof = in_solve_profile(S)
+        in_solve_profile(S, False)
     return S, xs, reps, ms / len(reps)
 
 
@@ -194,7 +246,9 @@ def main():
     hw = torch.from_numpy(np.repeat((w.h0 / 2.0 ** w.blocks[:, 0]) ** 3, 512)).to(dev)
     R = solver_for(w, tol_rel=0.0, tol_rhs=1e-9, bmean=args.bmean, precision="mixed_fp16",
                    max_iter=2000, max_refine=8)
-    nsteps = args.warmup + args.steps
+    learned = args.form != "fp64" and not args.no_learned
+    ntrain = args.train if learned else 0
+    nsteps = ntrain + args.warmup + args.steps
     p_prev = torch.zeros(w.ncells, dtype=torch.float64, device=dev)
     seq, ref_res = [], []
     for t in range(nsteps + 1):
@@ -216,6 +270,25 @@ def main():
         b = b - (hw * b).sum() / hw.sum()
         return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
 
+    # the warm-start net learns from the triples of the steps before the
+    # timed ones (untimed setup, the library's own HIP training kernels)
+    weights, learn = None, {}
+    if learned:
+        T = solver_for(w, precision=args.form, bmean=args.bmean, guess="cnn")
+        for st in seq[:ntrain]:
+            T.cnn_add_sample(st["rhs"], st["p_prev"], st["x_ref"])
+        t0 = time.perf_counter()
+        l0, _ = T.cnn_train(1, batch=256, lr=1e-3, seed=1, res_weight=100.0)
+        _, l1 = T.cnn_train(args.train_steps - 1, batch=256, lr=1e-3, seed=2, res_weight=100.0)
+        torch.cuda.synchronize()
+        learn = {"train_triples": ntrain, "train_steps": args.train_steps, "batch_blocks": 256,
+                 "train_seconds": round(time.perf_counter() - t0, 2), "loss_first": l0,
+                 "loss_last": l1}
+        weights = T.cnn_get_weights()
+        del T
+        print(f"setup: net trained on {ntrain} triples, loss {l0:.3e} -> {l1:.3e}",
+              file=sys.stderr, flush=True)
+    seq = seq[ntrain:]
     timed = seq[args.warmup:]
 
     def summary(xs, reps):
@@ -233,9 +306,27 @@ def main():
     if args.form == "fp64":
         S, xs, reps, ms = S64, xs64, reps64, ms64
     else:
-        S, xs, reps, ms = time_form(w, args.form, args.bmean, seq, args.warmup, 3000)
+        S, xs, reps, ms = time_form(w, args.form, args.bmean, seq, args.warmup, 3000,
+                                    profile=not learned)
     print(f"{args.form}: {ms:.2f} ms per step", file=sys.stderr, flush=True)
     sm, sm64 = summary(xs, reps), summary(xs64, reps64)
+    prev_guess = None
+    if learned:
+        # the same form from the learned guess (fallback: previous pressure)
+        Sp, xsp, repsp, msp = S, xs, reps, ms
+        S, xs, reps, ms = time_form(w, args.form, args.bmean, seq, args.warmup, 3000, weights,
+                                    profile=True)
+        print(f"{args.form} + learned: {ms:.2f} ms per step", file=sys.stderr, flush=True)
+        smp, sm = sm, summary(xs, reps)
+        prev_guess = {"solves_per_s": round(1e3 / msp, 3), "ms_per_step": round(msp, 3),
+                      "solves": smp}
+        learn.update({
+            "net_guess_used": f"{sum(r['guess_used'] == 2 for r in reps)}/{len(reps)}",
+            "initial_resid_ratio_median": float(np.median(
+                [r["guess_resid"] / rp["err_init"] for r, rp in zip(reps, repsp)])),
+            "iterations_saved_pct": round(100.0 * (1.0 - sum(r["iterations"] for r in reps) /
+                                                   sum(r["iterations"] for r in repsp)), 2),
+            "speedup_vs_prev_guess": round(msp / ms, 3)})
     if not all(r["status"] == 0 and r["true_rel_resid"] <= 1.0e-7 for r in reps):
         raise SystemExit(f"{args.form} did not reach the tolerance on every step: {sm}")
     rep, rhs = reps[-1], timed[-1]["rhs"]
@@ -245,14 +336,21 @@ def main():
     kern = kernel_times(S, [2, 3, 4])
     dom = max(kern, key=lambda k: kern[k]["avg_us"] * CALLS_PER_ITER[k])
     kd = kern[dom]
-    roof = {"bound": "hbm", "achieved": round(kd["GB/s"], 1), "peak": HBM_PEAK, "unit": "GB/s",
-            "frac": round(kd["GB/s"] / HBM_PEAK, 4),
-            "traffic": TRAFFIC.get((w.name, args.form, KERNELS[dom])),
-            "kernel": KERNELS[dom], "avg_us": round(kd["avg_us"], 2),
-            "alg_bytes_per_launch": kd["bytes"]}
     ceiling = copy_ceiling(dev)
-    roof["copy_ceiling_GB/s"] = ceiling
-    roof["frac_of_copy_ceiling"] = round(kd["GB/s"] / ceiling, 4)
+    prof = getattr(S, "prof", None)
+    if prof and dom == 3 and prof["samples"] > 0:
+        # K1 timed inside the timed solves (HIP events on the library's stream)
+        avg_us, timing = prof["K1"], f"HIP events around K1 inside the timed solves " \
+                                     f"({prof['samples']} launches)"
+    else:
+        avg_us, timing = kd["avg_us"], "HIP events around back-to-back launches"
+    gbs = kd["bytes"] / (avg_us * 1e-6) / 1e9
+    roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK, 4),
+            "traffic": TRAFFIC.get((w.name, args.form, KERNELS[dom])),
+            "kernel": KERNELS[dom], "avg_us": round(avg_us, 2), "timing": timing,
+            "alg_bytes_per_launch": kd["bytes"],
+            "copy_ceiling_GB/s": ceiling, "frac_of_copy_ceiling": round(gbs / ceiling, 4)}
     k64 = kernel_times(S64, [1, 2, 3, 4])
     g64 = k64[1]
     out = {
@@ -272,7 +370,9 @@ def main():
         "config": {"workload": w.name, "form": args.form, "cells": w.ncells,
                    "blocks": int(len(w.blocks)), "tolerance": "||b-Ax||/||b|| <= 1e-7",
                    "steps": "consecutive time steps t = warmup+1 .. warmup+steps",
-                   "warm_start": "previous-step pressure", "bMeanConstraint": args.bmean,
+                   "warm_start": ("learned CNN guess, previous-step pressure as fallback"
+                                  if learned else "previous-step pressure"),
+                   "bMeanConstraint": args.bmean,
                    "parallelism": "single GPU"},
         "iterations": sm["iterations_mean"],
         "refinements": rep["refinements"],
@@ -293,8 +393,13 @@ def main():
         "hbm_copy_GB/s": ceiling,
         "kernels": {KERNELS[k]: {kk: round(vv, 3) for kk, vv in v.items()}
                     for k, v in kern.items()},
+        "kernels_in_solve_us": ({k: round(v, 2) for k, v in prof.items() if k != "samples"}
+                                if prof else None),
         "roofline": roof,
     }
+    if learned:
+        out["learned"] = learn
+        out["prev_guess"] = prev_guess
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.bmean, rhs.cpu().numpy(),
                                            p_last.cpu().numpy(),

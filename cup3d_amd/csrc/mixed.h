@@ -23,6 +23,12 @@ struct MixedWorkspace {
   int nb = 0;
   int timing_precision = 1;
   KrylovVectors<float> v32{};
+  // in-solve kernel timing: HIP events around K1, K3 and K5 of the first
+  // iteration of every polled batch, summed while prof is set
+  bool prof = false;
+  hipEvent_t pev[6] = {};
+  double prof_ms[3] = {0.0, 0.0, 0.0};
+  long long prof_n = 0;
 };
 
 int mixed_init_pinv(MixedWorkspace& M, const std::vector<double>& P, std::string& err);

@@ -570,16 +570,33 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
     int guard = 0;
     while (true) {
       for (int it = 0; it < poll; ++it) {
+        // the first iteration of a batch always runs (the batch starts only
+        // while the state is RUNNING), so its kernels are the ones timed
+        const bool tm = M.prof && it == 0;
+        if (tm) MCHECK(hipEventRecord(M.pev[0], s));
         launch_precond_mixed(M, s, cfg.precision, PREC_P, g32, nullptr, nullptr, st);
+        if (tm) MCHECK(hipEventRecord(M.pev[1], s));
         spmv32<SPMV_DOT1>(s, g32, v.z1, v.nu, v.rhat, st, part);
+        if (tm) MCHECK(hipEventRecord(M.pev[2], s));
         launch_precond_mixed(M, s, cfg.precision, PREC_S, g32, nullptr, nullptr, st);
+        if (tm) MCHECK(hipEventRecord(M.pev[3], s));
         spmv32<SPMV_DOT2>(s, g32, v.z2, v.t, v.r, st, part);
+        if (tm) MCHECK(hipEventRecord(M.pev[4], s));
         k_update<float><<<grid_vec(n), WG_THREADS, 0, s>>>(n, v, st, part);
+        if (tm) MCHECK(hipEventRecord(M.pev[5], s));
         k_update_fin<<<1, WG_THREADS, 0, s>>>(st, part, grid_vec(n));
       }
       MCHECK(hipGetLastError());
       MCHECK(hipMemcpyAsync(h_st, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
       MCHECK(hipStreamSynchronize(s));
+      if (M.prof) {
+        for (int k = 0; k < 3; ++k) {
+          float ms = 0.f;
+          MCHECK(hipEventElapsedTime(&ms, M.pev[2 * k], M.pev[2 * k + 1]));
+          M.prof_ms[k] += ms;
+        }
+        ++M.prof_n;
+      }
       if (++guard > cfg.max_iter / poll + 4 + 2 * replacements) {
         err = "inner solve did not stop";
         return -1;
@@ -721,6 +738,11 @@ void mixed_free(MixedWorkspace& M) {
   if (M.d_vec32) (void)hipFree(M.d_vec32);
   if (M.d_res64) (void)hipFree(M.d_res64);
   M.d_P16 = M.d_Pbf = M.d_vec32 = M.d_res64 = nullptr;
+  for (hipEvent_t& e : M.pev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
+  M.prof = false;
 }
 
 }  // namespace cup3d

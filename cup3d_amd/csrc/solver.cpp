@@ -93,6 +93,12 @@ struct cup3d_poisson {
   MixedWorkspace mixed;
   // learned warm start
   CnnState cnn;
+
+  // staging of the host entry point: pinned rhs + x and their device copies,
+  // allocated by the first host solve on a topology and kept
+  double* h_stage = nullptr;  // 2 n doubles, pinned
+  DevBuf d_stage;             // 2 n doubles
+  long long stage_n = 0;
 };
 
 namespace {
@@ -434,11 +440,17 @@ int cup3d_poisson_solve(cup3d_poisson_t H, const double* const* rhs_blocks,
   if ((!rhs_blocks && !rhs_contig) || (!x_blocks && !x_contig)) return fail(H, "null vector");
   HIPCHECK(hipSetDevice(H->device));
   const size_t n = (size_t)H->n, bytes = n * sizeof(double);
-  double *hb = nullptr, *hx = nullptr, *db = nullptr, *dx = nullptr;
-  HIPCHECK(hipHostMalloc((void**)&hb, bytes, hipHostMallocDefault));
-  HIPCHECK(hipHostMalloc((void**)&hx, bytes, hipHostMallocDefault));
-  HIPCHECK(hipMalloc(&db, bytes));
-  HIPCHECK(hipMalloc(&dx, bytes));
+  if (H->stage_n != H->n) {
+    if (H->h_stage) (void)hipHostFree(H->h_stage);
+    H->h_stage = nullptr;
+    H->stage_n = 0;
+    H->d_stage.release();
+    HIPCHECK(hipHostMalloc((void**)&H->h_stage, 2 * bytes, hipHostMallocDefault));
+    if (H->d_stage.alloc(2 * bytes) != hipSuccess) return fail(H, "hipMalloc(staging) failed");
+    H->stage_n = H->n;
+  }
+  double *hb = H->h_stage, *hx = H->h_stage + n;
+  double *db = H->d_stage.as<double>(), *dx = db + n;
   // getVec gather (PoissonSolverExp.cpp:389-417)
   for (int b = 0; b < H->nb; ++b) {
     std::memcpy(hb + (size_t)b * BLEN, rhs_blocks ? rhs_blocks[b] : rhs_contig + (size_t)b * BLEN,
@@ -461,10 +473,6 @@ int cup3d_poisson_solve(cup3d_poisson_t H, const double* const* rhs_blocks,
   } else {
     st = fail(H, "hipMemcpy H2D failed");
   }
-  (void)hipFree(db);
-  (void)hipFree(dx);
-  (void)hipHostFree(hb);
-  (void)hipHostFree(hx);
   return st;
 }
 
@@ -648,10 +656,34 @@ void cup3d_poisson_destroy(cup3d_poisson_t H) {
   cnn_free(H->cnn);
   mixed_free(H->mixed);
   if (H->h_state) (void)hipHostFree(H->h_state);
+  if (H->h_stage) (void)hipHostFree(H->h_stage);
   if (H->ev0) (void)hipEventDestroy(H->ev0);
   if (H->ev1) (void)hipEventDestroy(H->ev1);
   if (H->stream) (void)hipStreamDestroy(H->stream);
   delete H;
+}
+
+int cup3d_debug_profile(cup3d_poisson_t H, int enable) {
+  if (!H) return -1;
+  if (H->cfg.precision == CUP3D_PREC_FP64) return fail(H, "in-solve profiling covers the mixed form");
+  HIPCHECK(hipSetDevice(H->device));
+  MixedWorkspace& M = H->mixed;
+  if (enable && !M.pev[0])
+    for (hipEvent_t& e : M.pev) HIPCHECK(hipEventCreate(&e));
+  M.prof = enable != 0;
+  for (double& m : M.prof_ms) m = 0.0;
+  M.prof_n = 0;
+  return 0;
+}
+
+int cup3d_debug_profile_read(cup3d_poisson_t H, double* avg_us, long long* samples) {
+  if (!H || !avg_us || !samples) return -1;
+  MixedWorkspace& M = H->mixed;
+  for (int k = 0; k < 3; ++k) avg_us[k] = M.prof_n ? 1e3 * M.prof_ms[k] / M.prof_n : 0.0;
+  *samples = M.prof_n;
+  for (double& m : M.prof_ms) m = 0.0;
+  M.prof_n = 0;
+  return 0;
 }
 
 const char* cup3d_poisson_last_error(cup3d_poisson_t H) {

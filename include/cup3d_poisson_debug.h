@@ -28,6 +28,11 @@ int64_t cup3d_debug_operator_csr(const cup3d_poisson_config* cfg, int64_t nblock
                                  const cup3d_block_desc* blocks, int64_t* rowptr, int32_t* col,
                                  double* val, int64_t cap);
 
+/* y = A x with the FP32 operator of the mixed-precision inner solve (x cast
+ * to FP32, y cast back; FP64 device vectors), for checking it against the
+ * FP64 operator. */
+int cup3d_debug_apply_operator32(cup3d_poisson_t h, const double* d_x, double* d_y);
+
 /* Kernel timing for roofline reports: runs one kernel class `reps` times on
  * the handle's stream between two HIP events, on the data of the last solve,
  * and returns the average duration in ms. which: 0 = operator (SpMV),
@@ -36,13 +41,19 @@ int64_t cup3d_debug_operator_csr(const cup3d_poisson_config* cfg, int64_t nblock
  * the algorithmic bytes (vectors read + written) and flops of one launch
  * (GEMM flops count the MFMA passes actually issued).
  * Must follow a solve on the current topology. Returns 0 or < 0. */
-/* y = A x with the FP32 operator of the mixed-precision inner solve (x cast
- * to FP32, y cast back; FP64 device vectors), for checking it against the
- * FP64 operator. */
-int cup3d_debug_apply_operator32(cup3d_poisson_t h, const double* d_x, double* d_y);
-
 int cup3d_debug_time_kernel(cup3d_poisson_t h, int which, int reps, double* avg_ms,
                             double* bytes, double* flops);
+
+/* In-solve kernel timing of the mixed-precision form. While enabled, every
+ * solve on the handle records HIP events on its stream around K1 (p update +
+ * GEMM), K3 (s update + GEMM) and K5 (x/r update) of the first iteration of
+ * each polled batch of iterations (one batch = 8 iterations by default).
+ * cup3d_debug_profile(h, 1) enables and clears, (h, 0) disables;
+ * cup3d_debug_profile_read returns the average durations in microseconds
+ * (avg_us[3]: K1, K3, K5) and the number of samples, and clears them.
+ * The FP64 form returns < 0. */
+int cup3d_debug_profile(cup3d_poisson_t h, int enable);
+int cup3d_debug_profile_read(cup3d_poisson_t h, double* avg_us, long long* samples);
 
 /* Full-batch FP32 gradient (n_weights floats) and loss of the learned warm
  * start over every block of training sample k (grids up to 4096 blocks), for
