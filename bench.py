@@ -78,6 +78,8 @@ def parse():
     p.add_argument("--bmean", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-iters", type=int, default=10)
+    p.add_argument("--tol", type=float, default=1e-7,
+                   help="true relative residual every timed solve must reach")
     p.add_argument("--train", type=int, default=24,
                    help="sequence steps before the timed ones whose (rhs, p_prev, p) triples "
                         "train the warm-start net")
@@ -115,7 +117,7 @@ def in_solve_profile(S, enable=None):
     return {"K1": us[0], "K3": us[1], "K5": us[2], "samples": n.value}
 
 
-def time_form(w, form, bmean, seq, warmup, max_iter, weights=None, profile=False):
+def time_form(w, form, bmean, seq, warmup, max_iter, weights=None, profile=False, tol=1e-7):
     """Solve every step of the sequence with `form`, each from the previous
     step's reference pressure (or, given net weights, from the learned guess
     with that pressure as the fallback); time the steps after the first
@@ -123,7 +125,7 @@ def time_form(w, form, bmean, seq, warmup, max_iter, weights=None, profile=False
     events on its stream during the timed solves (one iteration in eight)."""
     import torch
     from cup3d_amd.poisson import solver_for
-    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-7, precision=form, bmean=bmean, max_iter=max_iter,
+    S = solver_for(w, tol_rel=0.0, tol_rhs=tol, precision=form, bmean=bmean, max_iter=max_iter,
                    guess="given" if weights is None else "cnn")
     if weights is not None:
         S.cnn_set_weights(weights)
@@ -292,7 +294,7 @@ def main():
     timed = seq[args.warmup:]
 
     def summary(xs, reps):
-        conv = [r["status"] == 0 and r["true_rel_resid"] <= 1.0e-7 for r in reps]
+        conv = [r["status"] == 0 and r["true_rel_resid"] <= args.tol for r in reps]
         return {"iterations_mean": float(np.mean([r["iterations"] for r in reps])),
                 "iterations": [r["iterations"] for r in reps],
                 "converged": f"{sum(conv)}/{len(conv)}",
@@ -301,13 +303,14 @@ def main():
                                                  for x, st in zip(xs, timed))}
 
     # the FP64 drop-in with the reference's iteration cap (BiCGSTAB.cu:474)
-    S64, xs64, reps64, ms64 = time_form(w, "fp64", args.bmean, seq, args.warmup, 1000)
+    S64, xs64, reps64, ms64 = time_form(w, "fp64", args.bmean, seq, args.warmup, 1000,
+                                        tol=args.tol)
     print(f"fp64: {ms64:.2f} ms per step", file=sys.stderr, flush=True)
     if args.form == "fp64":
         S, xs, reps, ms = S64, xs64, reps64, ms64
     else:
         S, xs, reps, ms = time_form(w, args.form, args.bmean, seq, args.warmup, 3000,
-                                    profile=not learned)
+                                    profile=not learned, tol=args.tol)
     print(f"{args.form}: {ms:.2f} ms per step", file=sys.stderr, flush=True)
     sm, sm64 = summary(xs, reps), summary(xs64, reps64)
     prev_guess = None
@@ -315,7 +318,7 @@ def main():
         # the same form from the learned guess (fallback: previous pressure)
         Sp, xsp, repsp, msp = S, xs, reps, ms
         S, xs, reps, ms = time_form(w, args.form, args.bmean, seq, args.warmup, 3000, weights,
-                                    profile=True)
+                                    profile=True, tol=args.tol)
         print(f"{args.form} + learned: {ms:.2f} ms per step", file=sys.stderr, flush=True)
         smp, sm = sm, summary(xs, reps)
         prev_guess = {"solves_per_s": round(1e3 / msp, 3), "ms_per_step": round(msp, 3),
@@ -327,7 +330,7 @@ def main():
             "iterations_saved_pct": round(100.0 * (1.0 - sum(r["iterations"] for r in reps) /
                                                    sum(r["iterations"] for r in repsp)), 2),
             "speedup_vs_prev_guess": round(msp / ms, 3)})
-    if not all(r["status"] == 0 and r["true_rel_resid"] <= 1.0e-7 for r in reps):
+    if not all(r["status"] == 0 and r["true_rel_resid"] <= args.tol for r in reps):
         raise SystemExit(f"{args.form} did not reach the tolerance on every step: {sm}")
     rep, rhs = reps[-1], timed[-1]["rhs"]
     p_last = timed[-1]["p_prev"]
@@ -368,7 +371,7 @@ def main():
                   "mixed_bf16": "f32+bf16mfma(f64 refine)"}[args.form],
         "data": "synthetic",
         "config": {"workload": w.name, "form": args.form, "cells": w.ncells,
-                   "blocks": int(len(w.blocks)), "tolerance": "||b-Ax||/||b|| <= 1e-7",
+                   "blocks": int(len(w.blocks)), "tolerance": f"||b-Ax||/||b|| <= {args.tol:g}",
                    "steps": "consecutive time steps t = warmup+1 .. warmup+steps",
                    "warm_start": ("learned CNN guess, previous-step pressure as fallback"
                                   if learned else "previous-step pressure"),

@@ -14,7 +14,8 @@ import torch
 
 from cup3d_amd import workloads as W
 from cup3d_amd.poisson import PoissonSolver, solver_for
-from helpers import school4_small, fft_solution_periodic, grid_cases, random_field, tiny_amr
+from helpers import (WALL, fft_solution_periodic, grid_cases, random_field, school4_small,
+                     tiny_amr, uniform)
 
 pytestmark = pytest.mark.gpu
 
@@ -164,6 +165,25 @@ def test_host_paths_match_device_path(gpu):
     assert np.array_equal(x.cpu().numpy(), xh)
     assert np.array_equal(xh, np.concatenate(xb))
     assert rep["iterations"] == reph["iterations"] == repb["iterations"]
+
+
+def test_host_staging_follows_topology_change(gpu):
+    """The host entry point keeps its staging buffers between solves and
+    replaces them when a new topology changes the cell count: host and
+    device paths agree bitwise on every topology of the handle."""
+    amr = CASES["amr2_walls"]
+    uni = uniform((4, 4, 4), (WALL, WALL, WALL))
+    assert amr[1:] == uni[1:]
+    S = PoissonSolver(amr[1], amr[2], amr[3], bmean=0, tol_rel=1e-9)
+    for blocks in (amr[0], uni[0], amr[0]):
+        S.set_topology(blocks)
+        b = compatible_rhs(blocks, amr[2], len(blocks))
+        x = torch.zeros(len(blocks) * 512, dtype=torch.float64, device=gpu)
+        rep = S.solve_device(dev(b), x)
+        xh = np.zeros(len(blocks) * 512)
+        reph = S.solve_host(b, xh)
+        assert rep["status"] == reph["status"] == 0
+        assert np.array_equal(x.cpu().numpy(), xh)
 
 
 def test_deterministic(gpu):

@@ -154,6 +154,36 @@ def test_mixed_solve_bmean2(gpu, prec):
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.0001e-10
 
 
+def test_in_solve_profile_leaves_solve_unchanged(gpu):
+    """cup3d_debug_profile times K1, K3 and K5 with HIP events inside the
+    mixed solve: samples are reported, and the solve is bitwise the same with
+    and without them. The FP64 form refuses it."""
+    blocks, bpd, h0, bc = CASES["amr3_walls"]
+    b = dev(compatible_rhs(blocks, h0, 14))
+    S = PoissonSolver(bpd, h0, bc, precision="mixed_fp16", bmean=0, tol_rel=0.0,
+                      tol_rhs=1e-9).set_topology(blocks)
+    L = lib()
+    L.cup3d_debug_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.cup3d_debug_profile_read.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_longlong)]
+    x0 = torch.zeros(len(blocks) * 512, dtype=torch.float64, device=gpu)
+    rep0 = S.solve_device(b, x0)
+    assert L.cup3d_debug_profile(S.h, 1) == 0
+    x1 = torch.zeros_like(x0)
+    rep1 = S.solve_device(b, x1)
+    us = (ctypes.c_double * 3)()
+    n = ctypes.c_longlong()
+    assert L.cup3d_debug_profile_read(S.h, us, ctypes.byref(n)) == 0
+    assert L.cup3d_debug_profile(S.h, 0) == 0
+    assert rep0["status"] == rep1["status"] == 0
+    assert rep0["iterations"] == rep1["iterations"] and torch.equal(x0, x1)
+    # one sample per polled batch of 8 iterations
+    assert rep1["iterations"] // 8 <= n.value <= rep1["iterations"]
+    assert all(0.0 < us[k] < 1e4 for k in range(3))
+    S64 = PoissonSolver(bpd, h0, bc, bmean=0).set_topology(blocks)
+    assert L.cup3d_debug_profile(S64.h, 1) < 0
+
+
 def test_mixed_c1_error_vs_exact(gpu):
     """32^3 periodic Fourier workload at the benchmark tolerance 1e-7: the
     mixed solution is as close to the exact (FFT) solution as the FP64 one."""
