@@ -73,12 +73,6 @@ int cnn_eval(CnnState& C, hipStream_t s, int k, double* loss, std::string& err);
 // Full-batch FP32 gradient of the loss over every block of sample k.
 int cnn_gradient(CnnState& C, hipStream_t s, int k, float res_weight, float* grad, double* loss,
                  std::string& err);
-// pred <- x0 + gamma (pred - x0) with the residual-minimising gamma, given
-// r0 = b - A x0 (|r0|^2 = r0_norm2) and rp = b - A pred; reports gamma and the
-// residual norm of the result (pred untouched when no step helps, gamma 0).
-int cnn_step_length(CnnState& C, hipStream_t s, const double* r0, double r0_norm2,
-                    const double* rp, const double* x0, double* pred, double* gamma,
-                    double* resid, std::string& err);
 void cnn_free(CnnState& C);
 
 }  // namespace cup3d

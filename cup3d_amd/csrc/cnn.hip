@@ -816,67 +816,6 @@ int cnn_gradient(CnnState& C, hipStream_t s, int k, float res_weight, float* gra
   return 0;
 }
 
-// ---------------------------------------------------------------------------
-// Step length of the net's increment: x0 + gamma (pred - x0) with the gamma
-// that minimises the FP64 residual, from r0 = b - A x0 and rp = b - A pred
-// (A (pred - x0) = r0 - rp):
-//   gamma = (r0, r0 - rp) / |r0 - rp|^2,  |r(gamma)|^2 = |r0|^2 - (r0, r0 - rp)^2 / |r0 - rp|^2
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(WG_THREADS) void k_ws_dots(long long n, const double* __restrict__ r0,
-                                                        const double* __restrict__ rp,
-                                                        double* part) {
-  double v[2] = {0.0, 0.0};
-  const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
-  for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
-    const long long i = base + e;
-    if (i < n) {
-      v[0] += r0[i] * rp[i];
-      v[1] += rp[i] * rp[i];
-    }
-  }
-  grid_store_partial<2>(v, part);
-}
-
-__global__ __launch_bounds__(WG_THREADS) void k_ws_dots_fin(double* scal, const double* part,
-                                                            int nparts) {
-  double tot[2];
-  grid_sum_partials<2>(part, nparts, tot);
-  if (threadIdx.x == 0) {
-    scal[3] = tot[0];
-    scal[4] = tot[1];
-  }
-}
-
-__global__ void k_ws_blend(long long n, const double* __restrict__ x0, double* __restrict__ pred,
-                           double gamma) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    pred[i] = fma(gamma, pred[i] - x0[i], x0[i]);
-}
-
-int cnn_step_length(CnnState& C, hipStream_t s, const double* r0, double r0_norm2,
-                    const double* rp, const double* x0, double* pred, double* gamma,
-                    double* resid, std::string& err) {
-  const int nwg = (int)((C.n + 4 * BLEN - 1) / (4 * BLEN));
-  k_ws_dots<<<nwg, WG_THREADS, 0, s>>>(C.n, r0, rp, C.d_part);
-  k_ws_dots_fin<<<1, WG_THREADS, 0, s>>>(C.d_scal, C.d_part, nwg);
-  double d[2];
-  CCHECK(hipMemcpyAsync(d, C.d_scal + 3, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  CCHECK(hipStreamSynchronize(s));
-  const double a = r0_norm2, c = d[0], q = d[1];
-  const double den = a - 2 * c + q;  // |A (pred - x0)|^2
-  *gamma = 0.0;
-  *resid = std::sqrt(a);
-  if (!(den > 0) || !std::isfinite(den)) return 0;
-  const double g = (a - c) / den;
-  const double r2 = a - (a - c) * (a - c) / den;
-  *gamma = g;
-  *resid = std::sqrt(r2 > 0 ? r2 : 0.0);
-  k_ws_blend<<<4096, 256, 0, s>>>(C.n, x0, pred, g);
-  CCHECK(hipGetLastError());
-  return 0;
-}
-
 void cnn_free(CnnState& C) {
   cnn_clear_samples(C);
   hipFree(C.d_w);

@@ -24,4 +24,16 @@ void launch_update64(hipStream_t s, long long n, const KrylovVectors<double>& v,
 void launch_copy_opt64(hipStream_t s, long long n, const KrylovVectors<double>& v,
                        const SolverState* st, double* out);
 
+// Warm start from several guesses (device vectors of n doubles):
+// out = 2 x - hist (linear extrapolation in time from the last two steps);
+void launch_extrapolate(hipStream_t s, long long n, const double* x, const double* hist,
+                        double* out);
+// dots[0..4] = (r0, r1), (r1, r1), (r0, r2), (r2, r2), (r1, r2) in a fixed
+// order (r1 or r2 may be null: their dots are 0);
+void launch_guess_dots(hipStream_t s, long long n, const double* r0, const double* r1,
+                       const double* r2, double* part, double* dots);
+// x = x0 + g1 (c1 - x0) + g2 (c2 - x0) (a null candidate must have weight 0).
+void launch_guess_blend(hipStream_t s, long long n, const double* x0, const double* c1,
+                        const double* c2, double g1, double g2, double* x);
+
 }  // namespace cup3d

@@ -228,6 +228,82 @@ void launch_copy_opt64(hipStream_t s, long long n, const KrylovVectors<double>& 
   k_copy_opt<double><<<nwg, WG_THREADS, 0, s>>>(n, v, st, out);
 }
 
+// ---------------------------------------------------------------------------
+// Warm start from the previous steps: candidates and their least-squares
+// weights (the weights themselves are solved on the host from five dots).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG_THREADS) void k_extrapolate(long long n,
+                                                            const double* __restrict__ x,
+                                                            const double* __restrict__ hist,
+                                                            double* __restrict__ out) {
+  for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n;
+       i += (long long)gridDim.x * WG_THREADS)
+    out[i] = 2.0 * x[i] - hist[i];
+}
+
+__global__ __launch_bounds__(WG_THREADS) void k_guess_dots(long long n,
+                                                           const double* __restrict__ r0,
+                                                           const double* __restrict__ r1,
+                                                           const double* __restrict__ r2,
+                                                           double* part) {
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
+  for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
+    const long long i = base + e;
+    if (i < n) {
+      const double a = r0[i], p = r1 ? r1[i] : 0.0, q = r2 ? r2[i] : 0.0;
+      v[0] += a * p;
+      v[1] += p * p;
+      v[2] += a * q;
+      v[3] += q * q;
+      v[4] += p * q;
+    }
+  }
+  grid_store_partial<5>(v, part);
+}
+
+__global__ __launch_bounds__(WG_THREADS) void k_guess_dots_fin(const double* part, int nparts,
+                                                               double* dots) {
+  double tot[5];
+  grid_sum_partials<5>(part, nparts, tot);
+  if (threadIdx.x < 5) dots[threadIdx.x] = tot[threadIdx.x];
+}
+
+__global__ __launch_bounds__(WG_THREADS) void k_guess_blend(long long n,
+                                                            const double* __restrict__ x0,
+                                                            const double* __restrict__ c1,
+                                                            const double* __restrict__ c2,
+                                                            double g1, double g2,
+                                                            double* __restrict__ x) {
+  for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n;
+       i += (long long)gridDim.x * WG_THREADS) {
+    const double b = x0[i];
+    double v = b;
+    if (c1) v = fma(g1, c1[i] - b, v);
+    if (c2) v = fma(g2, c2[i] - b, v);
+    x[i] = v;
+  }
+}
+
+void launch_extrapolate(hipStream_t s, long long n, const double* x, const double* hist,
+                        double* out) {
+  const int nwg = (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 8192);
+  k_extrapolate<<<nwg, WG_THREADS, 0, s>>>(n, x, hist, out);
+}
+
+void launch_guess_dots(hipStream_t s, long long n, const double* r0, const double* r1,
+                       const double* r2, double* part, double* dots) {
+  const int nwg = (int)((n + 4 * BLEN - 1) / (4 * BLEN));
+  k_guess_dots<<<nwg, WG_THREADS, 0, s>>>(n, r0, r1, r2, part);
+  k_guess_dots_fin<<<1, WG_THREADS, 0, s>>>(part, nwg, dots);
+}
+
+void launch_guess_blend(hipStream_t s, long long n, const double* x0, const double* c1,
+                        const double* c2, double g1, double g2, double* x) {
+  const int nwg = (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 8192);
+  k_guess_blend<<<nwg, WG_THREADS, 0, s>>>(n, x0, c1, c2, g1, g2, x);
+}
+
 // Fragment-packed P for the B operand: Ppk[(it*128 + kt)*64 + l] = P[4kt + l/16][16it + l%16].
 void pack_pinv_f64(const std::vector<double>& P, std::vector<double>& pk) {
   pk.assign((size_t)BLEN * BLEN, 0.0);

@@ -99,6 +99,11 @@ struct cup3d_poisson {
   double* h_stage = nullptr;  // 2 n doubles, pinned
   DevBuf d_stage;             // 2 n doubles
   long long stage_n = 0;
+
+  // warm start: the guess handed in by the previous solve on this topology
+  // (the caller's pressure one step earlier), and the five guess dots
+  DevBuf d_hist, d_gdots;
+  bool hist_valid = false;
 };
 
 namespace {
@@ -184,6 +189,7 @@ int cup3d_poisson_create(const cup3d_poisson_config* cfg, cup3d_poisson_t* out) 
   if (c.max_iter <= 0) { g_create_error = "max_iter must be positive"; return -1; }
   if (c.precision < 0 || c.precision > 2) { g_create_error = "unknown precision"; return -1; }
   if (c.guess < 0 || c.guess > 2) { g_create_error = "unknown guess mode"; return -1; }
+  if (c.extrapolate < 0 || c.extrapolate > 1) { g_create_error = "extrapolate must be 0 or 1"; return -1; }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     g_create_error = "no HIP device available";
@@ -289,8 +295,13 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
   v.z1 = base + 8 * n;
   v.z2 = base + 9 * n;
   v.b = base + 10 * n;
-  H->part_len = (size_t)((H->nb + 3) / 4 + 8) * 4;
+  H->part_len = (size_t)((H->nb + 3) / 4 + 8) * 6;  // up to 5 partials per workgroup
   if (H->d_part.alloc(H->part_len * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(partials) failed");
+  H->hist_valid = false;
+  if (H->cfg.extrapolate && H->d_hist.alloc(n * sizeof(double)) != hipSuccess)
+    return fail(H, "hipMalloc(history) failed");
+  if (!H->d_gdots.p && H->d_gdots.alloc(8 * sizeof(double)) != hipSuccess)
+    return fail(H, "hipMalloc(guess dots) failed");
   HIPCHECK(hipMemset(H->d_state.p, 0, sizeof(SolverState)));
   if (mixed_set_topology(H->mixed, H->nb, H->n, H->err) != 0) return -1;
   if (cnn_set_topology(H->cnn, H->nb, H->d_nbr.as<int>(), H->d_h.as<double>(), H->err) != 0)
@@ -354,44 +365,85 @@ static double increment_residual(cup3d_poisson_t H, const double* d_rhs, const d
   return std::sqrt(H->h_state->true_rr);
 }
 
-// ||b - A x|| for b in v64.b (FP64), the residual into v64.t.
-static double resid_norm(cup3d_poisson_t H, const double* d_x) {
-  hipStream_t s = H->stream;
-  SolverState* st = H->d_state.as<SolverState>();
-  launch_spmv64(s, SPMV_RESID, H->g64, d_x, nullptr, H->v64.b, H->v64.t, nullptr, st,
-                H->d_part.as<double>());
-  if (hipMemcpyAsync(H->h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess)
-    return fail(H, "residual failed");
-  return std::sqrt(H->h_state->true_rr);
+// r = b - A x for b in v64.b (FP64).
+static int resid_into(cup3d_poisson_t H, const double* d_x, double* r) {
+  launch_spmv64(H->stream, SPMV_RESID, H->g64, d_x, nullptr, H->v64.b, r, nullptr,
+                H->d_state.as<SolverState>(), H->d_part.as<double>());
+  return hipGetLastError() == hipSuccess ? 0 : fail(H, "residual launch failed");
 }
 
-// Learned warm start: the net's increment, scaled by the step length that
-// minimises the FP64 residual, replaces d_x only when that residual is smaller
-// than the given guess's (fallback otherwise).
-static int cnn_warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* used,
-                          double* resid) {
+// Warm start from up to two candidates next to the given guess x0 (the
+// caller's pressure of the previous step): the learned net's prediction
+// (CUP3D_GUESS_CNN, with trained weights) and the linear extrapolation in
+// time 2 x0 - x_hist from the guess of the previous solve (cfg.extrapolate).
+// The guess becomes x0 + g1 (c1 - x0) + g2 (c2 - x0) with the weights that
+// minimise the FP64 residual ||b - A x|| (a 2x2 least-squares problem on
+// the candidates' residuals); g = 0 is one of the choices, so the result is
+// never worse than x0, which is kept (the fallback) when no candidate helps.
+static int warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* used,
+                      double* resid, double* w_net, double* w_time) {
+  hipStream_t s = H->stream;
   *used = CUP3D_GUESS_GIVEN;
-  const double r_prev = increment_residual(H, d_rhs, d_x);
+  *w_net = *w_time = 0.0;
+  const double r_prev = increment_residual(H, d_rhs, d_x);  // v64.b, v64.x[0], r0 in v64.nu
   if (r_prev < 0) return -1;
   *resid = r_prev;
-  if (!H->cnn.have_weights) return 0;
-  double* pred = H->cnn.d_pred;
-  if (cnn_predict(H->cnn, H->stream, H->v64.nu, H->v64.x[0], pred, H->err) != 0) return -1;
-  const double r_raw = resid_norm(H, pred);  // residual into v64.t
-  if (r_raw < 0 && !std::isnan(r_raw)) return -1;
-  if (!std::isfinite(r_raw)) return 0;
-  double gamma = 0, r_net = r_prev;
-  if (cnn_step_length(H->cnn, H->stream, H->v64.nu, r_prev * r_prev, H->v64.t, H->v64.x[0], pred,
-                      &gamma, &r_net, H->err) != 0)
-    return -1;
-  if (gamma != 0.0 && std::isfinite(r_net) && r_net < r_prev) {
-    HIPCHECK(hipMemcpyAsync(d_x, pred, (size_t)H->n * sizeof(double), hipMemcpyDeviceToDevice,
-                            H->stream));
-    HIPCHECK(hipStreamSynchronize(H->stream));
-    *used = CUP3D_GUESS_CNN;
-    *resid = r_net;
+  const KrylovVectors<double>& v = H->v64;
+  const double* x0 = v.x[0];
+  double *c1 = nullptr, *c2 = nullptr, *r1 = nullptr, *r2 = nullptr;
+  if (H->cfg.extrapolate && H->hist_valid) {
+    c2 = v.z2;
+    r2 = v.p;
+    launch_extrapolate(s, H->n, x0, H->d_hist.as<double>(), c2);
+    if (resid_into(H, c2, r2) != 0) return -1;
   }
+  if (H->cfg.extrapolate) {  // this step's pressure is the next solve's history
+    HIPCHECK(hipMemcpyAsync(H->d_hist.p, d_x, (size_t)H->n * sizeof(double),
+                            hipMemcpyDeviceToDevice, s));
+    H->hist_valid = true;
+  }
+  if (H->cfg.guess == CUP3D_GUESS_CNN && H->cnn.have_weights) {
+    c1 = H->cnn.d_pred;
+    r1 = v.t;
+    if (cnn_predict(H->cnn, s, v.nu, x0, c1, H->err) != 0) return -1;
+    if (resid_into(H, c1, r1) != 0) return -1;
+  }
+  if (!c1 && !c2) return 0;
+  double d[5];
+  launch_guess_dots(s, H->n, v.nu, r1, r2, H->d_part.as<double>(), H->d_gdots.as<double>());
+  HIPCHECK(hipMemcpyAsync(d, H->d_gdots.p, sizeof(d), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  // u_k = A (c_k - x0) = r0 - r_k; minimise |r0 - g1 u1 - g2 u2|^2
+  const double a = r_prev * r_prev;
+  const double G11 = a - 2 * d[0] + d[1], G22 = a - 2 * d[2] + d[3];
+  const double G12 = a - d[0] - d[2] + d[4];
+  const double f1 = a - d[0], f2 = a - d[2];
+  double g1 = 0, g2 = 0;
+  auto one = [](double G, double f) { return G > 0 && std::isfinite(G) ? f / G : 0.0; };
+  const double det = G11 * G22 - G12 * G12;
+  if (c1 && c2 && det > 1e-12 * G11 * G22 && std::isfinite(det)) {
+    g1 = (f1 * G22 - f2 * G12) / det;
+    g2 = (f2 * G11 - f1 * G12) / det;
+  } else if (c1 && c2) {  // (nearly) parallel candidates: the better single one
+    const double e1 = one(G11, f1) * f1, e2 = one(G22, f2) * f2;  // residual^2 drops
+    if (e1 >= e2) g1 = one(G11, f1);
+    else g2 = one(G22, f2);
+  } else if (c1) {
+    g1 = one(G11, f1);
+  } else {
+    g2 = one(G22, f2);
+  }
+  const double r2n = a - (g1 * f1 + g2 * f2);  // residual^2 at the minimiser
+  if (!std::isfinite(g1) || !std::isfinite(g2) || !std::isfinite(r2n) || (g1 == 0 && g2 == 0) ||
+      !(r2n < a))
+    return 0;  // fallback: the given guess
+  // a candidate with weight 0 is not read (it may hold non-finite values)
+  launch_guess_blend(s, H->n, x0, g1 != 0 ? c1 : nullptr, g2 != 0 ? c2 : nullptr, g1, g2, d_x);
+  HIPCHECK(hipGetLastError());
+  *used = g1 != 0 ? CUP3D_GUESS_CNN : CUP3D_GUESS_GIVEN;
+  *w_net = g1;
+  *w_time = g2;
+  *resid = std::sqrt(r2n > 0 ? r2n : 0.0);
   return 0;
 }
 
@@ -408,11 +460,12 @@ int cup3d_poisson_solve_device(cup3d_poisson_t H, const double* d_rhs, double* d
   const double* x0 = d_x;
   int guess_used = H->cfg.guess;
   double guess_resid = -1;
+  double w_net = 0, w_time = 0;
   if (H->cfg.guess == CUP3D_GUESS_ZERO) x0 = nullptr;
   int st;
-  if (H->cfg.guess == CUP3D_GUESS_CNN) {
-    // learned warm start with fallback to the given guess
-    if (cnn_warm_start(H, d_rhs, d_x, &guess_used, &guess_resid) != 0) return -1;
+  if (H->cfg.guess == CUP3D_GUESS_CNN || (H->cfg.extrapolate && H->cfg.guess == CUP3D_GUESS_GIVEN)) {
+    // learned and/or extrapolated warm start, the given guess as the fallback
+    if (warm_start(H, d_rhs, d_x, &guess_used, &guess_resid, &w_net, &w_time) != 0) return -1;
   }
   if (H->cfg.precision == CUP3D_PREC_FP64)
     st = solve_fp64(H, d_rhs, d_x, x0, &local);
@@ -428,6 +481,8 @@ int cup3d_poisson_solve_device(cup3d_poisson_t H, const double* d_rhs, double* d
   local.solve_ms = ms;
   local.guess_used = guess_used;
   if (guess_resid >= 0) local.guess_resid = guess_resid;
+  local.guess_weight_net = w_net;
+  local.guess_weight_time = w_time;
   if (rep) *rep = local;
   return st;
 }

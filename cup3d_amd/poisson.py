@@ -26,7 +26,8 @@ class Config(ctypes.Structure):
                 ("max_iter", ctypes.c_int), ("max_restarts", ctypes.c_int),
                 ("precision", ctypes.c_int), ("guess", ctypes.c_int), ("device", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("max_refine", ctypes.c_int),
-                ("poll_every", ctypes.c_int), ("gemm_passes", ctypes.c_int)]
+                ("poll_every", ctypes.c_int), ("gemm_passes", ctypes.c_int),
+                ("extrapolate", ctypes.c_int)]
 
 
 class BlockDesc(ctypes.Structure):
@@ -39,7 +40,8 @@ class Report(ctypes.Structure):
                 ("err_init", ctypes.c_double), ("err_final", ctypes.c_double),
                 ("rhs_norm", ctypes.c_double), ("true_rel_resid", ctypes.c_double),
                 ("solve_ms", ctypes.c_double), ("guess_used", ctypes.c_int),
-                ("guess_resid", ctypes.c_double)]
+                ("guess_resid", ctypes.c_double), ("guess_weight_net", ctypes.c_double),
+                ("guess_weight_time", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -105,7 +107,8 @@ class PoissonSolver:
 
     def __init__(self, bpd, h0, bc, level_max=0, tol_abs=0.0, tol_rel=1e-7, tol_rhs=0.0,
                  bmean=1, max_iter=1000, max_restarts=100, precision="fp64", guess="given",
-                 device=0, inner_tol=1e-7, max_refine=10, poll_every=8, gemm_passes=3):
+                 device=0, inner_tol=1e-7, max_refine=10, poll_every=8, gemm_passes=3,
+                 extrapolate=False):
         self.L = lib()
         c = Config()
         for d in range(3):
@@ -119,6 +122,7 @@ class PoissonSolver:
         c.device, c.inner_tol, c.max_refine = int(device), float(inner_tol), int(max_refine)
         c.poll_every = int(poll_every)
         c.gemm_passes = int(gemm_passes)
+        c.extrapolate = int(bool(extrapolate))
         self.cfg = c
         h = _vp()
         if self.L.cup3d_poisson_create(ctypes.byref(c), ctypes.byref(h)) != 0:

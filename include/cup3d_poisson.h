@@ -66,6 +66,14 @@ enum {
   CUP3D_GUESS_CNN = 2     /* learned warm start from (rhs, x on input = previous pressure);
                              falls back to the given x when its residual is larger */
 };
+/* With cfg.extrapolate = 1 (and guess GIVEN or CNN) the solver keeps the x
+ * handed to the previous solve on the same topology (the caller's pressure
+ * one step earlier) and also tries the linear extrapolation in time
+ * 2 x - x_previous. The guess used is x + g_net (net - x) + g_time
+ * (extrapolation - x) with the weights that minimise the FP64 residual
+ * ||b - A guess|| (least squares over the candidates' residuals); weights 0
+ * (the given x) are one of the choices, so the guess is never worse than x.
+ * set_topology clears the history. */
 
 typedef struct {
   int bpd[3];        /* blocks per direction at level 0 (-bpdx/-bpdy/-bpdz) */
@@ -91,6 +99,8 @@ typedef struct {
   int poll_every;    /* iterations between host polls of the device status (0 = default) */
   int gemm_passes;   /* mixed only: 0 (or 2, 3) = input split into hi + lo, two MFMA
                         passes, linear to FP32 precision; 1 = one FP16/BF16 pass */
+  int extrapolate;   /* 0 = off (the reference contract), 1 = also extrapolate the guess
+                        in time from the previous solve's (see above) */
 } cup3d_poisson_config;
 
 typedef struct {
@@ -111,6 +121,8 @@ typedef struct {
   double solve_ms;       /* device time of the solve (events around it) */
   int guess_used;        /* CUP3D_GUESS_* actually used (after CNN fallback) */
   double guess_resid;    /* ||b - A x0||_2 of the guess used */
+  double guess_weight_net;   /* g_net of the warm start (0: the net's guess not used) */
+  double guess_weight_time;  /* g_time of the warm start (0: no extrapolation) */
 } cup3d_solve_report;
 
 /* Create a solver on cfg->device. Returns 0 or < 0. */

@@ -1,0 +1,123 @@
+"""Warm start from the previous steps on the GPU (cfg.extrapolate): the
+linear extrapolation in time 2 p_{t-1} - p_{t-2}, weighted by the FP64
+least-squares step, checked against what the mathematics fixes — exactness
+on sequences that are linear in time, the 1-D least-squares optimum and the
+residual it reaches computed independently with the oracle's operator, and
+never a worse residual than the given guess."""
+import numpy as np
+import pyoracle
+import pytest
+import torch
+
+from cup3d_amd.poisson import PoissonSolver
+from helpers import grid_cases, random_field, tiny_amr
+
+pytestmark = pytest.mark.gpu
+
+CASES = dict(grid_cases())
+CASES["tiny_amr3_walls"] = tiny_amr(three_levels=True)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64).reshape(-1)).cuda()
+
+
+def mean_free(f, blocks, h0):
+    h3 = np.repeat((h0 / 2.0 ** np.asarray(blocks)[:, 0]) ** 3, 512)
+    f = np.asarray(f, np.float64).reshape(-1)
+    return f - (h3 @ f) / h3.sum()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed_fp16"])
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "tiny_amr3_walls"])
+def test_extrapolation_exact_on_linear_sequence(gpu, name, prec):
+    """p_t = p0 + t d and b_t = A p_t: from the third step on, the extrapolated
+    guess is the solution up to the previous solves' error, with weight 1."""
+    blocks, bpd, h0, bc = CASES[name]
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
+    p0 = mean_free(random_field(len(blocks), 31), blocks, h0)
+    d = 0.1 * mean_free(random_field(len(blocks), 32), blocks, h0)
+    S = PoissonSolver(bpd, h0, bc, bmean=0, tol_rel=0.0, tol_rhs=1e-10, precision=prec,
+                      max_iter=3000, extrapolate=True).set_topology(blocks)
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    for t in range(5):
+        b = O.spmv(p0 + t * d)
+        prev = x.cpu().numpy()
+        rep = S.solve_device(dev(b), x)
+        assert rep["status"] == 0, (t, rep)
+        r_prev = np.linalg.norm(b - O.spmv(prev))
+        if t == 0:
+            assert rep["guess_weight_time"] == 0.0  # no history yet
+        elif t >= 2:  # (t = 1 extrapolates from the zero guess of t = 0)
+            assert abs(rep["guess_weight_time"] - 1.0) < 1e-6, rep
+            assert rep["guess_resid"] < 1e-7 * r_prev, (rep["guess_resid"], r_prev)
+        xm = mean_free(x.cpu().numpy(), blocks, h0)
+        x.copy_(dev(xm))  # the caller's projection (PressureProjection.cpp:404-431)
+
+
+@pytest.mark.parametrize("name", ["amr3_walls", "tiny_amr3_walls"])
+def test_extrapolation_weight_is_the_least_squares_step(gpu, name):
+    """On a sequence that is not linear in time the weight is the minimiser of
+    |b - A (x + g (x - x_prev))| and the reported guess residual its value,
+    both recomputed on the CPU with the oracle's operator; the guess is never
+    worse than the given one."""
+    blocks, bpd, h0, bc = CASES[name]
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
+    S = PoissonSolver(bpd, h0, bc, bmean=0, tol_rel=0.0, tol_rhs=1e-9, max_iter=3000,
+                      extrapolate=True).set_topology(blocks)
+    rng = np.random.default_rng(5)
+    base = [mean_free(random_field(len(blocks), 40 + k), blocks, h0) for k in range(3)]
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    hist = None
+    for t in range(4):
+        c = [np.cos(0.7 * t + k) for k in range(3)] + rng.uniform(-0.05, 0.05, 3)
+        b = O.spmv(sum(ck * bk for ck, bk in zip(c, base)))
+        xin = x.cpu().numpy()
+        rep = S.solve_device(dev(b), x)
+        assert rep["status"] == 0
+        if hist is not None:
+            r0 = b - O.spmv(xin)
+            u = O.spmv(xin - hist)
+            g = (r0 @ u) / (u @ u)
+            resid = np.linalg.norm(r0 - g * u)
+            assert abs(rep["guess_weight_time"] - g) <= 1e-9 * max(1.0, abs(g)), (rep, g)
+            assert abs(rep["guess_resid"] - resid) <= 1e-9 * resid
+            assert rep["guess_resid"] <= np.linalg.norm(r0)
+        hist = xin
+        x.copy_(dev(mean_free(x.cpu().numpy(), blocks, h0)))
+
+
+def test_set_topology_clears_the_history(gpu):
+    blocks, bpd, h0, bc = CASES["uniform_walls_3x2x2"]
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
+    S = PoissonSolver(bpd, h0, bc, bmean=0, tol_rel=0.0, tol_rhs=1e-10,
+                      extrapolate=True).set_topology(blocks)
+    p0 = mean_free(random_field(len(blocks), 51), blocks, h0)
+    d = mean_free(random_field(len(blocks), 52), blocks, h0)
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    weights = []
+    for t in range(4):
+        if t == 2:
+            S.set_topology(blocks)
+        rep = S.solve_device(dev(O.spmv(p0 + t * d)), x)
+        assert rep["status"] == 0
+        weights.append(rep["guess_weight_time"])
+    assert weights[0] == 0.0 and weights[2] == 0.0  # t = 2: history cleared
+    assert weights[1] != 0.0  # t = 1 extrapolates from the zero guess of t = 0
+    assert abs(weights[3] - 1.0) < 1e-6
+
+
+def test_extrapolation_off_keeps_the_reference_contract(gpu):
+    """Without cfg.extrapolate the given x is the guess (no history kept)."""
+    blocks, bpd, h0, bc = CASES["uniform_walls_3x2x2"]
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
+    S = PoissonSolver(bpd, h0, bc, bmean=0, tol_rel=0.0, tol_rhs=1e-10).set_topology(blocks)
+    p0 = mean_free(random_field(len(blocks), 61), blocks, h0)
+    d = mean_free(random_field(len(blocks), 62), blocks, h0)
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    for t in range(3):
+        b = O.spmv(p0 + t * d)
+        r_in = np.linalg.norm(b - O.spmv(x.cpu().numpy()))
+        rep = S.solve_device(dev(b), x)
+        assert rep["guess_weight_time"] == 0.0
+        assert abs(rep["err_init"] - r_in) <= 1e-12 * r_in
