@@ -19,10 +19,13 @@ same inputs:
     1e-7 are counted, not hidden);
   * the chosen form: FP16 split-precision MFMA GEMM with FP32 accumulation,
     FP32 Krylov vectors, FP64 iterative refinement, from the previous
-    pressure (`prev_guess`) and, the headline `value`, from the learned
-    warm start: a 3-layer CNN trained (untimed setup, the library's HIP
-    training kernels) on the (rhs, p_prev, p) triples of the --train steps
-    before the timed ones, with the previous pressure as the fallback.
+    pressure (`prev_guess`); from the learned warm start (`learned_only`): a
+    3-layer CNN trained (untimed setup, the library's HIP training kernels)
+    on the (rhs, p_prev, p) triples of the --train steps before the timed
+    ones, with the previous pressure as the fallback; and, the headline
+    `value`, from the learned guess combined with the linear extrapolation
+    in time from the previous two pressures, weighted by FP64 least squares
+    on their residuals (fallback: the previous pressure).
 The line carries the roofline of the headline form's dominant kernel (HIP
 events on the library's stream) and a CPU baseline (the oracle's restatement
 of the reference CPU solver on a bounded sample of the same workload).
@@ -117,16 +120,19 @@ def in_solve_profile(S, enable=None):
     return {"K1": us[0], "K3": us[1], "K5": us[2], "samples": n.value}
 
 
-def time_form(w, form, bmean, seq, warmup, max_iter, weights=None, profile=False, tol=1e-7):
-    """Solve every step of the sequence with `form`, each from the previous
-    step's reference pressure (or, given net weights, from the learned guess
-    with that pressure as the fallback); time the steps after the first
-    `warmup`. With `profile`, the library also times K1, K3 and K5 with HIP
-    events on its stream during the timed solves (one iteration in eight)."""
+def time_form(w, form, bmean, seq, warmup, max_iter, weights=None, profile=False, tol=1e-7,
+              extrapolate=False):
+    """Solve every step of the sequence with `form`, each handed the previous
+    step's reference pressure (given net weights, the learned guess with that
+    pressure as the fallback; with `extrapolate`, also the extrapolation in
+    time from the pressure handed to the solve before); time the steps after
+    the first `warmup`. With `profile`, the library also times K1, K3 and K5
+    with HIP events on its stream during the timed solves (one iteration in
+    eight)."""
     import torch
     from cup3d_amd.poisson import solver_for
     S = solver_for(w, tol_rel=0.0, tol_rhs=tol, precision=form, bmean=bmean, max_iter=max_iter,
-                   guess="given" if weights is None else "cnn")
+                   guess="given" if weights is None else "cnn", extrapolate=extrapolate)
     if weights is not None:
         S.cnn_set_weights(weights)
     if profile:
@@ -314,22 +320,39 @@ def main():
     print(f"{args.form}: {ms:.2f} ms per step", file=sys.stderr, flush=True)
     sm, sm64 = summary(xs, reps), summary(xs64, reps64)
     prev_guess = None
+    learned_only = None
     if learned:
-        # the same form from the learned guess (fallback: previous pressure)
         Sp, xsp, repsp, msp = S, xs, reps, ms
-        S, xs, reps, ms = time_form(w, args.form, args.bmean, seq, args.warmup, 3000, weights,
-                                    profile=True, tol=args.tol)
-        print(f"{args.form} + learned: {ms:.2f} ms per step", file=sys.stderr, flush=True)
-        smp, sm = sm, summary(xs, reps)
+        smp = sm
         prev_guess = {"solves_per_s": round(1e3 / msp, 3), "ms_per_step": round(msp, 3),
                       "solves": smp}
+
+        def gains(reps_, ms_):
+            return {
+                "initial_resid_ratio_median": float(np.median(
+                    [r["guess_resid"] / rp["err_init"] for r, rp in zip(reps_, repsp)])),
+                "iterations_saved_pct": round(100.0 * (1.0 - sum(r["iterations"] for r in reps_) /
+                                                       sum(r["iterations"] for r in repsp)), 2),
+                "speedup_vs_prev_guess": round(msp / ms_, 3)}
+
+        # the learned guess alone (fallback: previous pressure)
+        _, xsl, repsl, msl = time_form(w, args.form, args.bmean, seq, args.warmup, 3000,
+                                       weights, tol=args.tol)
+        print(f"{args.form} + learned: {msl:.2f} ms per step", file=sys.stderr, flush=True)
+        learned_only = {"solves_per_s": round(1e3 / msl, 3), "ms_per_step": round(msl, 3),
+                        "net_guess_used": f"{sum(r['guess_used'] == 2 for r in repsl)}/{len(repsl)}",
+                        "solves": summary(xsl, repsl), **gains(repsl, msl)}
+        # headline: learned guess and extrapolation in time, least-squares weights
+        S, xs, reps, ms = time_form(w, args.form, args.bmean, seq, args.warmup, 3000, weights,
+                                    profile=True, tol=args.tol, extrapolate=True)
+        print(f"{args.form} + learned + extrapolation: {ms:.2f} ms per step", file=sys.stderr,
+              flush=True)
+        sm = summary(xs, reps)
         learn.update({
-            "net_guess_used": f"{sum(r['guess_used'] == 2 for r in reps)}/{len(reps)}",
-            "initial_resid_ratio_median": float(np.median(
-                [r["guess_resid"] / rp["err_init"] for r, rp in zip(reps, repsp)])),
-            "iterations_saved_pct": round(100.0 * (1.0 - sum(r["iterations"] for r in reps) /
-                                                   sum(r["iterations"] for r in repsp)), 2),
-            "speedup_vs_prev_guess": round(msp / ms, 3)})
+            "net_weight_median": float(np.median([r["guess_weight_net"] for r in reps])),
+            "time_weight_median": float(np.median([r["guess_weight_time"] for r in reps])),
+            "net_guess_used": f"{sum(r['guess_weight_net'] != 0 for r in reps)}/{len(reps)}",
+            **gains(reps, ms)})
     if not all(r["status"] == 0 and r["true_rel_resid"] <= args.tol for r in reps):
         raise SystemExit(f"{args.form} did not reach the tolerance on every step: {sm}")
     rep, rhs = reps[-1], timed[-1]["rhs"]
@@ -373,7 +396,8 @@ def main():
         "config": {"workload": w.name, "form": args.form, "cells": w.ncells,
                    "blocks": int(len(w.blocks)), "tolerance": f"||b-Ax||/||b|| <= {args.tol:g}",
                    "steps": "consecutive time steps t = warmup+1 .. warmup+steps",
-                   "warm_start": ("learned CNN guess, previous-step pressure as fallback"
+                   "warm_start": ("learned CNN guess + linear extrapolation in time, FP64 "
+                                  "least-squares weights; previous-step pressure as fallback"
                                   if learned else "previous-step pressure"),
                    "bMeanConstraint": args.bmean,
                    "parallelism": "single GPU"},
@@ -403,6 +427,7 @@ def main():
     if learned:
         out["learned"] = learn
         out["prev_guess"] = prev_guess
+        out["learned_only"] = learned_only
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.bmean, rhs.cpu().numpy(),
                                            p_last.cpu().numpy(),

@@ -23,7 +23,7 @@ struct MixedWorkspace {
   int nb = 0;
   int timing_precision = 1;
   KrylovVectors<float> v32{};
-  // in-solve kernel timing: HIP events around K1, K3 and K5 of the first
+  // in-solve kernel timing: HIP events around K1, K3 and K5 of the middle
   // iteration of every polled batch, summed while prof is set
   bool prof = false;
   hipEvent_t pev[6] = {};

@@ -569,10 +569,13 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
     int progress_iter = 0;
     int guard = 0;
     while (true) {
+      // timed iteration of the batch: one in the middle of it, away from the
+      // idle GPU of the host poll; kept only if the batch got past it (the
+      // kernels of iterations after convergence exit at once)
+      const int tm_it = poll / 2;
+      const int iter0 = h_st->iter;
       for (int it = 0; it < poll; ++it) {
-        // the first iteration of a batch always runs (the batch starts only
-        // while the state is RUNNING), so its kernels are the ones timed
-        const bool tm = M.prof && it == 0;
+        const bool tm = M.prof && it == tm_it;
         if (tm) MCHECK(hipEventRecord(M.pev[0], s));
         launch_precond_mixed(M, s, cfg.precision, PREC_P, g32, nullptr, nullptr, st);
         if (tm) MCHECK(hipEventRecord(M.pev[1], s));
@@ -589,7 +592,7 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
       MCHECK(hipGetLastError());
       MCHECK(hipMemcpyAsync(h_st, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
       MCHECK(hipStreamSynchronize(s));
-      if (M.prof) {
+      if (M.prof && h_st->iter - iter0 > tm_it) {
         for (int k = 0; k < 3; ++k) {
           float ms = 0.f;
           MCHECK(hipEventElapsedTime(&ms, M.pev[2 * k], M.pev[2 * k + 1]));

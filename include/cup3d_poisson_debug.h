@@ -46,8 +46,9 @@ int cup3d_debug_time_kernel(cup3d_poisson_t h, int which, int reps, double* avg_
 
 /* In-solve kernel timing of the mixed-precision form. While enabled, every
  * solve on the handle records HIP events on its stream around K1 (p update +
- * GEMM), K3 (s update + GEMM) and K5 (x/r update) of the first iteration of
- * each polled batch of iterations (one batch = 8 iterations by default).
+ * GEMM), K3 (s update + GEMM) and K5 (x/r update) of the middle iteration
+ * of each polled batch of iterations (one batch = 8 iterations by default),
+ * counted when the batch ran past it.
  * cup3d_debug_profile(h, 1) enables and clears, (h, 0) disables;
  * cup3d_debug_profile_read returns the average durations in microseconds
  * (avg_us[3]: K1, K3, K5) and the number of samples, and clears them.

@@ -177,8 +177,8 @@ def test_in_solve_profile_leaves_solve_unchanged(gpu):
     assert L.cup3d_debug_profile(S.h, 0) == 0
     assert rep0["status"] == rep1["status"] == 0
     assert rep0["iterations"] == rep1["iterations"] and torch.equal(x0, x1)
-    # one sample per polled batch of 8 iterations
-    assert rep1["iterations"] // 8 <= n.value <= rep1["iterations"]
+    # one sample per polled batch of 8 iterations that got past its middle
+    assert 1 <= rep1["iterations"] // 16 <= n.value <= rep1["iterations"] // 8 + 1
     assert all(0.0 < us[k] < 1e4 for k in range(3))
     S64 = PoissonSolver(bpd, h0, bc, bmean=0).set_topology(blocks)
     assert L.cup3d_debug_profile(S64.h, 1) < 0
