@@ -83,53 +83,75 @@ constexpr int GT = GW * 64;    // threads per GEMM workgroup
 //           tiles) x (1 or 2 passes) MFMAs; the P_inv fragments of the next
 //           k-step are prefetched from L2 while the current ones are used.
 //  phase 3: FP32 epilogue, scaled by h_b^-1 and the two scales.
-template <int MODE, typename H, int NBW, bool SPLIT>
-__global__ __launch_bounds__(GT) void k_precond_h(int nb, const H* __restrict__ Ppk,
-                                                  float p_unscale,
-                                                  const float* __restrict__ invh,
-                                                  KrylovVectors<float> v,
-                                                  const float* __restrict__ pin,
-                                                  float* __restrict__ pout, SolverState* st) {
-  typedef typename HalfVec<H>::type HV;
-  typedef H H4 __attribute__((ext_vector_type(4)));
-  constexpr int NS = SPLIT ? 2 : 1;
-  constexpr int NPB = NBW / GW;  // blocks per wave in phase 1
-  constexpr int F4 = 2 * NPB;    // float4 per lane per vector (a block is 2 x 64 float4)
-  static_assert(NBW % GW == 0 && NBW % 16 == 0, "block group must fill the waves and tiles");
-  __shared__ __attribute__((aligned(16))) H As[NS][NBW * HLD];
-  __shared__ float sc[NBW];
-  if (MODE != PREC_PLAIN && st->status != ST_RUNNING) return;
-  const int j0 = xcd_remap(blockIdx.x, gridDim.x) * NBW;
-  const float beta = MODE == PREC_P ? (float)st->beta : 0.f;
-  const float omega = MODE == PREC_P ? (float)st->omega : 0.f;
-  const float alpha = MODE == PREC_S ? (float)st->alpha : 0.f;
-  const int pzero = MODE == PREC_P ? st->p_zero : 0;
-  const int restart = MODE == PREC_P ? st->restart_pending : 0;
-  float* __restrict__ out = MODE == PREC_PLAIN ? pout : (MODE == PREC_S ? v.z2 : v.z1);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
 
-  // phase 1. Element u of a lane: block jw + u/2, cells 256 (u%2) + 4 l .. +3.
-  const int jw = w * NPB;
+// Scalars of the fused vector update, read once per workgroup.
+struct PrecScalars {
+  float beta, omega, alpha;
+  int pzero, restart;
+};
+
+template <int MODE>
+__device__ __forceinline__ PrecScalars prec_scalars(const SolverState* st) {
+  PrecScalars c;
+  c.beta = MODE == PREC_P ? (float)st->beta : 0.f;
+  c.omega = MODE == PREC_P ? (float)st->omega : 0.f;
+  c.alpha = MODE == PREC_S ? (float)st->alpha : 0.f;
+  c.pzero = MODE == PREC_P ? st->p_zero : 0;
+  c.restart = MODE == PREC_P ? st->restart_pending : 0;
+  return c;
+}
+
+// Phase 1 for one wave of NWP filling waves: blocks [w NBW/NWP, (w+1) NBW/NWP)
+// of the group starting at block j0. Element u of a lane: block jw + u/2,
+// cells 256 (u%2) + 4 l .. +3. precond_load issues every load of the fused
+// vector update; precond_convert (after them) stores the updated vector,
+// finds the per-block power-of-two scale and casts hi/lo into LDS
+// (A0 = hi, A1 = lo).
+template <int NBW, int NWP>
+struct FillRegs {
+  static constexpr int NPB = NBW / NWP;  // blocks per wave
+  static constexpr int F4 = 2 * NPB;     // float4 per lane per vector (a block is 2 x 64 float4)
+  static_assert(NBW % NWP == 0, "block group must fill the waves");
   float4 ra[F4], rb[F4], rc[F4];
+};
+
+template <int MODE, int NBW, int NWP>
+__device__ __forceinline__ void precond_load(int nb, int j0, int w, int l,
+                                             const KrylovVectors<float>& v,
+                                             const float* __restrict__ pin,
+                                             FillRegs<NBW, NWP>& R) {
+  constexpr int F4 = FillRegs<NBW, NWP>::F4;
+  const int jw = w * FillRegs<NBW, NWP>::NPB;
 #pragma unroll
   for (int u = 0; u < F4; ++u) {
     const int b = j0 + jw + u / 2;
     const size_t i4 = ((size_t)b * BLEN + 256 * (u & 1) + 4 * l) / 4;
     if (b < nb) {
       if (MODE == PREC_P) {
-        ra[u] = reinterpret_cast<const float4*>(v.r)[i4];
-        rb[u] = reinterpret_cast<const float4*>(v.nu)[i4];
-        rc[u] = reinterpret_cast<const float4*>(v.p)[i4];
+        R.ra[u] = reinterpret_cast<const float4*>(v.r)[i4];
+        R.rb[u] = reinterpret_cast<const float4*>(v.nu)[i4];
+        R.rc[u] = reinterpret_cast<const float4*>(v.p)[i4];
       } else if (MODE == PREC_S) {
-        ra[u] = reinterpret_cast<const float4*>(v.r)[i4];
-        rb[u] = reinterpret_cast<const float4*>(v.nu)[i4];
+        R.ra[u] = reinterpret_cast<const float4*>(v.r)[i4];
+        R.rb[u] = reinterpret_cast<const float4*>(v.nu)[i4];
       } else {
-        ra[u] = reinterpret_cast<const float4*>(pin)[i4];
+        R.ra[u] = reinterpret_cast<const float4*>(pin)[i4];
       }
     } else {
-      ra[u] = rb[u] = rc[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      R.ra[u] = R.rb[u] = R.rc[u] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
+}
+
+template <int MODE, typename H, int NBW, bool SPLIT, int NWP>
+__device__ __forceinline__ void precond_convert(int nb, int j0, int w, int l, float p_unscale,
+                                                const float* __restrict__ invh,
+                                                const KrylovVectors<float>& v,
+                                                const PrecScalars& c, FillRegs<NBW, NWP>& R,
+                                                H* A0, H* A1, float* sc) {
+  typedef H H4 __attribute__((ext_vector_type(4)));
+  constexpr int NPB = FillRegs<NBW, NWP>::NPB, F4 = FillRegs<NBW, NWP>::F4;
+  const int jw = w * NPB;
   float amax[NPB];
 #pragma unroll
   for (int jj = 0; jj < NPB; ++jj) amax[jj] = 0.f;
@@ -137,26 +159,26 @@ __global__ __launch_bounds__(GT) void k_precond_h(int nb, const H* __restrict__ 
   for (int u = 0; u < F4; ++u) {
     const int b = j0 + jw + u / 2;
     const size_t i4 = ((size_t)b * BLEN + 256 * (u & 1) + 4 * l) / 4;
-    float4 val = ra[u];
+    float4 val = R.ra[u];
     if (MODE == PREC_P) {
-      if (!pzero) {
-        val.x = fmaf(beta, fmaf(-omega, rb[u].x, rc[u].x), ra[u].x);
-        val.y = fmaf(beta, fmaf(-omega, rb[u].y, rc[u].y), ra[u].y);
-        val.z = fmaf(beta, fmaf(-omega, rb[u].z, rc[u].z), ra[u].z);
-        val.w = fmaf(beta, fmaf(-omega, rb[u].w, rc[u].w), ra[u].w);
+      if (!c.pzero) {
+        val.x = fmaf(c.beta, fmaf(-c.omega, R.rb[u].x, R.rc[u].x), R.ra[u].x);
+        val.y = fmaf(c.beta, fmaf(-c.omega, R.rb[u].y, R.rc[u].y), R.ra[u].y);
+        val.z = fmaf(c.beta, fmaf(-c.omega, R.rb[u].z, R.rc[u].z), R.ra[u].z);
+        val.w = fmaf(c.beta, fmaf(-c.omega, R.rb[u].w, R.rc[u].w), R.ra[u].w);
       }
       if (b < nb) {
         reinterpret_cast<float4*>(v.p)[i4] = val;
-        if (restart) reinterpret_cast<float4*>(v.rhat)[i4] = ra[u];
+        if (c.restart) reinterpret_cast<float4*>(v.rhat)[i4] = R.ra[u];
       }
     } else if (MODE == PREC_S) {
-      val.x = fmaf(-alpha, rb[u].x, ra[u].x);
-      val.y = fmaf(-alpha, rb[u].y, ra[u].y);
-      val.z = fmaf(-alpha, rb[u].z, ra[u].z);
-      val.w = fmaf(-alpha, rb[u].w, ra[u].w);
+      val.x = fmaf(-c.alpha, R.rb[u].x, R.ra[u].x);
+      val.y = fmaf(-c.alpha, R.rb[u].y, R.ra[u].y);
+      val.z = fmaf(-c.alpha, R.rb[u].z, R.ra[u].z);
+      val.w = fmaf(-c.alpha, R.rb[u].w, R.ra[u].w);
       if (b < nb) reinterpret_cast<float4*>(v.r)[i4] = val;
     }
-    ra[u] = val;
+    R.ra[u] = val;
     amax[u / 2] = fmaxf(amax[u / 2], fmaxf(fmaxf(fabsf(val.x), fabsf(val.y)),
                                            fmaxf(fabsf(val.z), fabsf(val.w))));
   }
@@ -170,25 +192,36 @@ __global__ __launch_bounds__(GT) void k_precond_h(int nb, const H* __restrict__ 
     int ex = 0;
     if (amax[jj] > 0.f) frexpf(amax[jj], &ex);
     const float s = amax[jj] > 0.f ? ldexpf(1.0f, -ex) : 1.0f;
-    const float x[4] = {ra[u].x * s, ra[u].y * s, ra[u].z * s, ra[u].w * s};
+    const float x[4] = {R.ra[u].x * s, R.ra[u].y * s, R.ra[u].z * s, R.ra[u].w * s};
     H4 hi, lo;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      hi[c] = (H)x[c];
-      lo[c] = (H)(x[c] - (float)hi[c]);
+    for (int k = 0; k < 4; ++k) {
+      hi[k] = (H)x[k];
+      lo[k] = (H)(x[k] - (float)hi[k]);
     }
     const int o = (jw + jj) * HLD + 256 * (u & 1) + 4 * l;
-    *reinterpret_cast<H4*>(&As[0][o]) = hi;
-    if (SPLIT) *reinterpret_cast<H4*>(&As[NS - 1][o]) = lo;
+    *reinterpret_cast<H4*>(&A0[o]) = hi;
+    if (SPLIT) *reinterpret_cast<H4*>(&A1[o]) = lo;
     if ((u & 1) == 0 && l == 0) {
       const int b = j0 + jw + jj;
       sc[jw + jj] = b < nb ? invh[b] * p_unscale / s : 0.f;
     }
   }
-  __syncthreads();
+}
 
-  // phase 2
+// Phases 2 and 3 for GEMM wave w (of GW): output cells [64 w, 64 w + 64) of
+// all NBW blocks of the group staged in A0/A1; 16 k-steps of 32 cells, each
+// 4 (cell tiles) x NBW/16 (block tiles) x (1 or 2 passes) MFMAs, the P_inv
+// fragments of the next k-step prefetched from L2 while the current ones are
+// used; FP32 epilogue scaled by h_b^-1 and the two scales.
+template <typename H, int NBW, bool SPLIT>
+__device__ __forceinline__ void precond_gemm(int nb, int j0, int w, int l,
+                                             const H* __restrict__ Ppk, const H* A0,
+                                             const H* A1, const float* sc,
+                                             float* __restrict__ out) {
+  typedef typename HalfVec<H>::type HV;
   constexpr int JT = NBW / 16;
+  static_assert(NBW % 16 == 0, "block group must fill the tiles");
   const int lr = l & 15, g = l >> 4;
   float4_t acc[4][JT];
 #pragma unroll
@@ -211,8 +244,8 @@ __global__ __launch_bounds__(GT) void k_precond_h(int nb, const H* __restrict__ 
 #pragma unroll
     for (int jt = 0; jt < JT; ++jt) {
       const int o = (jt * 16 + lr) * HLD + kt * 32 + 8 * g;
-      a[jt] = *reinterpret_cast<const HV*>(&As[0][o]);
-      if (SPLIT) al[jt] = *reinterpret_cast<const HV*>(&As[NS - 1][o]);
+      a[jt] = *reinterpret_cast<const HV*>(&A0[o]);
+      if (SPLIT) al[jt] = *reinterpret_cast<const HV*>(&A1[o]);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t)
@@ -222,8 +255,6 @@ __global__ __launch_bounds__(GT) void k_precond_h(int nb, const H* __restrict__ 
         if (SPLIT) acc[t][jt] = mfma16(al[jt], bq[cur][t], acc[t][jt]);
       }
   }
-
-  // phase 3
 #pragma unroll
   for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
@@ -237,6 +268,31 @@ __global__ __launch_bounds__(GT) void k_precond_h(int nb, const H* __restrict__ 
           out[(size_t)b * BLEN + (4 * w + t) * 16 + lr] = acc[t][jt][q] * s;
       }
     }
+}
+
+// One workgroup applies M^-1 to NBW consecutive blocks: every wave fills
+// (phase 1), then every wave multiplies (phases 2 and 3).
+template <int MODE, typename H, int NBW, bool SPLIT>
+__global__ __launch_bounds__(GT) void k_precond_h(int nb, const H* __restrict__ Ppk,
+                                                  float p_unscale,
+                                                  const float* __restrict__ invh,
+                                                  KrylovVectors<float> v,
+                                                  const float* __restrict__ pin,
+                                                  float* __restrict__ pout, SolverState* st) {
+  constexpr int NS = SPLIT ? 2 : 1;
+  __shared__ __attribute__((aligned(16))) H As[NS][NBW * HLD];
+  __shared__ float sc[NBW];
+  if (MODE != PREC_PLAIN && st->status != ST_RUNNING) return;
+  const PrecScalars c = prec_scalars<MODE>(st);
+  const int j0 = xcd_remap(blockIdx.x, gridDim.x) * NBW;
+  float* __restrict__ out = MODE == PREC_PLAIN ? pout : (MODE == PREC_S ? v.z2 : v.z1);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  FillRegs<NBW, GW> R;
+  precond_load<MODE, NBW, GW>(nb, j0, w, l, v, pin, R);
+  precond_convert<MODE, H, NBW, SPLIT, GW>(nb, j0, w, l, p_unscale, invh, v, c, R, As[0],
+                                           As[NS - 1], sc);
+  __syncthreads();
+  precond_gemm<H, NBW, SPLIT>(nb, j0, w, l, Ppk, As[0], As[NS - 1], sc, out);
 }
 
 // Inner-solve start: r = rhat = (float)(r64 * scale), x = 0, ||r||^2, and the
