@@ -100,7 +100,7 @@ struct HostTopology {
   std::vector<int8_t> kind;         // nb*6
   std::vector<double> h, invh, h3;
   std::vector<int> irow_ptr, irow_cell, irow_ent, ent_col;  // CSR form of the interface rows
-  std::vector<double> ent_val;                              // (debug matrix, introspection)
+  std::vector<double> ent_val;                              // (debug matrix only)
   std::vector<int> fnb;             // nb*24 coarse/fine partners per face
   std::vector<int8_t> bpar;         // nb index parities
   long long bmean_row = -1;
@@ -116,6 +116,7 @@ struct TopologyInput {
   int bmean;
   int nb;
   const int* blocks;  // nb x 4 (level, ix, iy, iz)
+  bool rows = false;  // also assemble the explicit interface rows (debug matrix only)
 };
 
 // topology.cpp

@@ -276,7 +276,7 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
     g.fnb = H->d_fnb.as<int>();
     g.bpar = H->d_bpar.as<int8_t>();
     g.bmean = H->cfg.bmean;
-    g.has_iface = T.irow_cell.empty() ? 0 : 1;
+    g.has_iface = T.n_coarse_faces + T.n_fine_faces > 0 ? 1 : 0;
     g.bmean_row = (H->cfg.bmean == 1 || H->cfg.bmean == 3) ? origin_row : -1;
   };
   fill(H->g64, H->d_h.p, H->d_invh.p);
@@ -697,11 +697,8 @@ int cup3d_debug_time_kernel(cup3d_poisson_t H, int which, int reps, double* avg_
   return 0;
 }
 
-int64_t cup3d_poisson_num_interface_rows(cup3d_poisson_t H) {
-  return H ? (int64_t)H->topo.irow_cell.size() : -1;
-}
-int64_t cup3d_poisson_num_interface_entries(cup3d_poisson_t H) {
-  return H ? (int64_t)H->topo.ent_col.size() : -1;
+int64_t cup3d_poisson_num_coarse_fine_faces(cup3d_poisson_t H) {
+  return H ? (int64_t)H->topo.n_coarse_faces + H->topo.n_fine_faces : -1;
 }
 
 void cup3d_poisson_destroy(cup3d_poisson_t H) {

@@ -165,10 +165,12 @@ bool build_topology(const TopologyInput& in, HostTopology& T, std::string& err) 
   }
   if (in.bmean == 1 && T.origin_block >= 0) T.bmean_row = (long long)T.origin_block * BLEN;
 
-  // Interface rows: one merged row per cell that touches a coarse-fine face.
+  // Interface rows (the debug matrix only; the solver applies the
+  // coarse-fine faces matrix-free): one merged row per cell that touches a
+  // coarse-fine face.
   T.irow_ptr.assign(nb + 1, 0);
   T.irow_ent.assign(1, 0);
-  for (int b = 0; b < nb; ++b) {
+  for (int b = 0; b < nb && in.rows; ++b) {
     const int* B = in.blocks + 4 * b;
     bool any = false;
     for (int f = 0; f < 6; ++f) any |= T.kind[b * 6 + f] >= FACE_COARSE;
@@ -350,6 +352,7 @@ extern "C" int64_t cup3d_debug_operator_csr(const cup3d_poisson_config* cfg, int
   in.bmean = cfg->bmean;
   in.nb = (int)nblocks;
   in.blocks = bl.data();
+  in.rows = true;
   HostTopology T;
   std::string err;
   if (!build_topology(in, T, err)) return -2;

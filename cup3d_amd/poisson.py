@@ -56,7 +56,7 @@ EXPORTS = [
     "cup3d_poisson_cnn_predict", "cup3d_poisson_cnn_add_sample",
     "cup3d_poisson_cnn_clear_samples", "cup3d_poisson_cnn_num_samples", "cup3d_poisson_cnn_train",
     "cup3d_poisson_cnn_eval",
-    "cup3d_poisson_num_interface_rows", "cup3d_poisson_num_interface_entries",
+    "cup3d_poisson_num_coarse_fine_faces",
     "cup3d_poisson_destroy", "cup3d_poisson_last_error",
 ]
 
@@ -84,10 +84,8 @@ def lib():
                                           ctypes.c_float, ctypes.c_uint64,
                                           ctypes.POINTER(ctypes.c_double)]
     L.cup3d_poisson_cnn_eval.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
-    L.cup3d_poisson_num_interface_rows.argtypes = [_vp]
-    L.cup3d_poisson_num_interface_rows.restype = ctypes.c_int64
-    L.cup3d_poisson_num_interface_entries.argtypes = [_vp]
-    L.cup3d_poisson_num_interface_entries.restype = ctypes.c_int64
+    L.cup3d_poisson_num_coarse_fine_faces.argtypes = [_vp]
+    L.cup3d_poisson_num_coarse_fine_faces.restype = ctypes.c_int64
     L.cup3d_poisson_destroy.argtypes = [_vp]
     L.cup3d_poisson_destroy.restype = None
     L.cup3d_poisson_last_error.argtypes = [_vp]
@@ -155,11 +153,8 @@ class PoissonSolver:
     def n(self):
         return self.nb * 512
 
-    def interface_rows(self):
-        return self.L.cup3d_poisson_num_interface_rows(self.h)
-
-    def interface_entries(self):
-        return self.L.cup3d_poisson_num_interface_entries(self.h)
+    def coarse_fine_faces(self):
+        return self.L.cup3d_poisson_num_coarse_fine_faces(self.h)
 
     def solve_device(self, rhs, x):
         """Solve in place on device tensors; x holds the guess. Returns the report dict."""

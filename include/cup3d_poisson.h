@@ -187,9 +187,9 @@ int cup3d_poisson_cnn_train(cup3d_poisson_t h, int steps, int batch, float lr, f
  * (FP32 forward, res_weight of the last training), for validation. */
 int cup3d_poisson_cnn_eval(cup3d_poisson_t h, int k, double* loss);
 
-/* Introspection for tests and reports. */
-int64_t cup3d_poisson_num_interface_rows(cup3d_poisson_t h);
-int64_t cup3d_poisson_num_interface_entries(cup3d_poisson_t h);
+/* Introspection for tests and reports: block faces with a coarser or finer
+ * neighbour (each counted from both sides). */
+int64_t cup3d_poisson_num_coarse_fine_faces(cup3d_poisson_t h);
 
 void cup3d_poisson_destroy(cup3d_poisson_t h);
 /* Last error message of this handle (or of the last failed create if h is NULL). */

@@ -186,6 +186,30 @@ def test_host_staging_follows_topology_change(gpu):
         assert np.array_equal(x.cpu().numpy(), xh)
 
 
+@pytest.mark.parametrize("name", ["amr2_periodic_offcentre", "amr3_walls", "school4_small"])
+def test_coarse_fine_face_count(gpu, name):
+    """The face tables find every coarse-fine face: counted here from the block
+    list alone (a face whose neighbour across it is not a leaf of the same
+    level but exists one level up or down), both sides counted."""
+    blocks, bpd, h0, bc = CASES[name]
+    S = PoissonSolver(bpd, h0, bc, bmean=0).set_topology(blocks)
+    leaves = {tuple(b) for b in np.asarray(blocks).tolist()}
+    count = 0
+    for (l, i, j, k) in leaves:
+        for d in range(3):
+            n = bpd[d] << l
+            for s in (-1, 1):
+                c = [i, j, k]
+                c[d] += s
+                if not 0 <= c[d] < n:
+                    if bc[d] != 1:
+                        continue
+                    c[d] %= n
+                if (l, *c) not in leaves:
+                    count += 1
+    assert S.coarse_fine_faces() == count > 0
+
+
 def test_deterministic(gpu):
     S, O, blocks, h0 = pair("amr3_walls", 1, tol_rel=1e-7)
     b = dev(compatible_rhs(blocks, h0, 7))
