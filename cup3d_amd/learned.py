@@ -9,10 +9,12 @@ recurrence stagnates above 1e-7 on some steps of this sequence); the first
 library's own HIP kernels learn (FP32 forward/backward, Adam). The net is then judged on the remaining steps
 and on a sequence from a fresh seed by what it saves: the number of
 iterations, for the FP64 and the mixed-precision solve, against starting from
-the previous pressure. The solve always finishes to the same tolerance and
-falls back to the previous pressure whenever the net's guess has the larger
-residual, so a worse net costs one inference and two residuals, never
-accuracy.
+the previous pressure; also, from the second judged step on, the linear
+extrapolation in time from the previous two pressures alone and combined
+with the net (least-squares weights on their FP64 residuals). The solve
+always finishes to the same tolerance and falls back to the previous
+pressure whenever the net's guess has the larger residual, so a worse net
+costs one inference and two residuals, never accuracy.
 
   python -m cup3d_amd.learned [--steps 400] [--train 300] [--fresh 40]
                               [--train-steps 20000] [--batch 256] [--res-weight 100]
@@ -90,6 +92,12 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, bat
             base = solver_for(w, precision=form, guess="given", **kf)
             net = solver_for(w, precision=form, guess="cnn", **kf)
             net.cnn_set_weights(weights)
+            # with the extrapolation in time from the previous two pressures:
+            # alone, and together with the net (least-squares weights)
+            ext = solver_for(w, precision=form, guess="given", extrapolate=True, **kf)
+            both = solver_for(w, precision=form, guess="cnn", extrapolate=True, **kf)
+            both.cnn_set_weights(weights)
+            it_e, it_x, w_net, fail_e, fail_x = [], [], [], 0, 0
             it_b, it_n, used, ratio, err, raw, fail_b, fail_n = [], [], 0, [], [], [], 0, 0
             xp = torch.empty(w.ncells, dtype=torch.float64, device=dev)
             ax = torch.empty_like(xp)
@@ -104,6 +112,16 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, bat
                 rb = base.solve_device(rhs_[t], xb)
                 xn = sols_[t - 1].clone()
                 rn = net.solve_device(rhs_[t], xn)
+                xe = sols_[t - 1].clone()
+                re_ = ext.solve_device(rhs_[t], xe)
+                xx = sols_[t - 1].clone()
+                rx = both.solve_device(rhs_[t], xx)
+                if t > ts[0]:  # the first step has no history yet
+                    it_e.append(re_["iterations"])
+                    it_x.append(rx["iterations"])
+                    w_net.append(rx["guess_weight_net"])
+                    fail_e += int(re_["status"] != 0)
+                    fail_x += int(rx["status"] != 0)
                 # a solve that stops at max_iter counts with its iterations
                 fail_b += int(rb["status"] != 0)
                 fail_n += int(rn["status"] != 0)
@@ -130,6 +148,16 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, bat
                 "initial_resid_ratio_median": float(np.median(ratio)),
                 "net_resid_ratio_median": float(np.median(raw)),
                 "err_vs_fp64_max": float(np.max(err)),
+                # steps after the first: extrapolation alone and with the net
+                "iterations_prev_guess_from_second": float(np.mean(it_b[1:])),
+                "iterations_extrap": float(np.mean(it_e)),
+                "iterations_net_extrap": float(np.mean(it_x)),
+                "iterations_saved_extrap_pct": round(100.0 * (1 - np.sum(it_e) / np.sum(it_b[1:])), 2),
+                "iterations_saved_net_extrap_pct": round(
+                    100.0 * (1 - np.sum(it_x) / np.sum(it_b[1:])), 2),
+                "net_weight_with_extrap_median": float(np.median(w_net)),
+                "not_converged_extrap": fail_e,
+                "not_converged_net_extrap": fail_x,
             }
         return out
 
