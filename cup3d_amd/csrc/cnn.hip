@@ -76,7 +76,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_cnn_rms(long long n, const doubl
                                                         const double* __restrict__ h,
                                                         double* part) {
   double v[1] = {0.0};
-  const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
+  const long long base = (long long)blockIdx.x * (4 * BLEN);
   for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
     const long long i = base + e;
     if (i < n) {
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_cnn_dmean(long long n, const dou
                                                           const double* __restrict__ h,
                                                           double* part) {
   double v[2] = {0.0, 0.0};
-  const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
+  const long long base = (long long)blockIdx.x * (4 * BLEN);
   for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
     const long long i = base + e;
     if (i < n) {

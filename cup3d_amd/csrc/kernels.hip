@@ -58,7 +58,7 @@ __global__ __launch_bounds__(PC_T) void k_precond_f64(int nb, const double* __re
   constexpr int JT = PC_NB / 16;  // block tiles
   __shared__ double Vs[PC_NB * PC_LD];
   if (MODE != PREC_PLAIN && st->status != ST_RUNNING) return;
-  const int j0 = xcd_remap(blockIdx.x, gridDim.x) * PC_NB;
+  const int j0 = (int)blockIdx.x * PC_NB;
   const double beta = MODE == PREC_P ? st->beta : 0.0;
   const double omega = MODE == PREC_P ? st->omega : 0.0;
   const double alpha = MODE == PREC_S ? st->alpha : 0.0;
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_guess_dots(long long n,
                                                            const double* __restrict__ r2,
                                                            double* part) {
   double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  const long long base = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN);
+  const long long base = (long long)blockIdx.x * (4 * BLEN);
   for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
     const long long i = base + e;
     if (i < n) {

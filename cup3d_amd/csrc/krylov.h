@@ -103,7 +103,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_getvec(long long n, long long ze
                                                        SolverState* st, double* part,
                                                        double tol_b) {
   double v[1] = {0.0};
-  const long long i0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN) + 8 * threadIdx.x;
+  const long long i0 = (long long)blockIdx.x * (4 * BLEN) + 8 * threadIdx.x;
   if (i0 < n) {
     double bv[8], xv[8];
     load8(rhs + i0, bv);
@@ -603,7 +603,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_update(long long n, KrylovVector
   const T alpha = (T)st->alpha, omega = (T)st->omega;
   double s[2] = {0.0, 0.0};
   // each thread owns 8 consecutive elements of this workgroup's 2048
-  const long long i0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN) + 8 * threadIdx.x;
+  const long long i0 = (long long)blockIdx.x * (4 * BLEN) + 8 * threadIdx.x;
   if (i0 < n) {
     T xv[8], z1[8], z2[8], t[8], r[8], rh[8];
     load8(v.x[xc] + i0, xv);

@@ -284,7 +284,7 @@ __global__ __launch_bounds__(GT) void k_precond_h(int nb, const H* __restrict__ 
   __shared__ float sc[NBW];
   if (MODE != PREC_PLAIN && st->status != ST_RUNNING) return;
   const PrecScalars c = prec_scalars<MODE>(st);
-  const int j0 = xcd_remap(blockIdx.x, gridDim.x) * NBW;
+  const int j0 = (int)blockIdx.x * NBW;
   float* __restrict__ out = MODE == PREC_PLAIN ? pout : (MODE == PREC_S ? v.z2 : v.z1);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   FillRegs<NBW, GW> R;
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_inner_init(long long n, const do
                                                            double scale, KrylovVectors<float> v,
                                                            SolverState* st, double* part) {
   double s[1] = {0.0};
-  const long long i0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN) + 8 * threadIdx.x;
+  const long long i0 = (long long)blockIdx.x * (4 * BLEN) + 8 * threadIdx.x;
   if (i0 < n) {
     double rd[8];
     float r[8], z[8];
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_replace(long long n, const doubl
                                                         double inv_scale, KrylovVectors<float> v,
                                                         double* part) {
   double s[2] = {0.0, 0.0};
-  const long long i0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * (4 * BLEN) + 8 * threadIdx.x;
+  const long long i0 = (long long)blockIdx.x * (4 * BLEN) + 8 * threadIdx.x;
   if (i0 < n) {
     double rd[8];
     float r[8], rh[8];
