@@ -174,9 +174,13 @@ def kernel_times(S, which_list, reps=20):
 
 
 def copy_ceiling(dev, nbytes=1 << 30, reps=10):
-    """Device-to-device copy rate (read + write bytes / s), the practical HBM
-    ceiling next to the 8 TB/s datasheet peak."""
+    """Device-to-device copy rate (read + write bytes / s) over 1 GiB, the
+    practical HBM ceiling next to the 8 TB/s datasheet peak: the faster of
+    torch's copy and the library's nontemporal streaming copy
+    (cup3d_debug_copy_rate). Returns (ceiling, torch rate, streaming rate)
+    in GB/s."""
     import torch
+    from cup3d_amd.poisson import lib
     a = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
     b = torch.empty_like(a)
     b.copy_(a)
@@ -186,7 +190,14 @@ def copy_ceiling(dev, nbytes=1 << 30, reps=10):
         b.copy_(a)
     e1.record()
     e1.synchronize()
-    return round(2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+    t = round(2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+    del a, b
+    f = lib().cup3d_debug_copy_rate
+    f.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int]
+    f.restype = ctypes.c_double
+    sr = f(dev.index or 0, nbytes, reps)
+    st = round(sr / 1e9, 1) if sr > 0 else None
+    return max(t, st or 0.0), t, st
 
 
 def host_solve_ms(S, rhs, p_prev):
@@ -366,7 +377,7 @@ def main():
     kern = kernel_times(S, [2, 3, 4])
     dom = max(kern, key=lambda k: kern[k]["avg_us"] * CALLS_PER_ITER[k])
     kd = kern[dom]
-    ceiling = copy_ceiling(dev)
+    ceiling, copy_torch, copy_stream = copy_ceiling(dev)
     prof = getattr(S, "prof", None)
     if prof and dom == 3 and prof["samples"] > 0:
         # K1 timed inside the timed solves (HIP events on the library's stream)
@@ -422,7 +433,8 @@ def main():
         "mesh_change_ms": mesh_ms,
         "host_to_host": {"ms_per_solve": host_ms, "status": host_status,
                          "note": "cup3d_poisson_solve from host arrays, PCIe copies included"},
-        "hbm_copy_GB/s": ceiling,
+        "hbm_copy_GB/s": {"ceiling": ceiling, "torch_copy": copy_torch,
+                          "streaming_copy": copy_stream},
         "kernels": {KERNELS[k]: {kk: round(vv, 3) for kk, vv in v.items()}
                     for k, v in kern.items()},
         "kernels_in_solve_us": ({k: round(v, 2) for k, v in prof.items() if k != "samples"}

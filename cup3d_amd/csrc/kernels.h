@@ -24,6 +24,9 @@ void launch_update64(hipStream_t s, long long n, const KrylovVectors<double>& v,
 void launch_copy_opt64(hipStream_t s, long long n, const KrylovVectors<double>& v,
                        const SolverState* st, double* out);
 
+// b = a over n4 float4 (the copy-rate probe of the diagnostics).
+void launch_stream_copy(hipStream_t s, long long n4, const void* a, void* b);
+
 // Warm start from several guesses (device vectors of n doubles):
 // out = 2 x - hist (linear extrapolation in time from the last two steps);
 void launch_extrapolate(hipStream_t s, long long n, const double* x, const double* hist,

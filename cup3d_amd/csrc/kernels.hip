@@ -285,6 +285,22 @@ __global__ __launch_bounds__(WG_THREADS) void k_guess_blend(long long n,
   }
 }
 
+// Streaming copy for the bandwidth ceiling report (nontemporal float4 loads
+// and stores, grid-stride over a fixed grid).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(WG_THREADS) void k_stream_copy(long long n4,
+                                                            const f32x4* __restrict__ a,
+                                                            f32x4* __restrict__ b) {
+  for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n4;
+       i += (long long)gridDim.x * WG_THREADS)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(&a[i]), &b[i]);
+}
+
+void launch_stream_copy(hipStream_t s, long long n4, const void* a, void* b) {
+  k_stream_copy<<<1024, WG_THREADS, 0, s>>>(n4, static_cast<const f32x4*>(a),
+                                            static_cast<f32x4*>(b));
+}
+
 void launch_extrapolate(hipStream_t s, long long n, const double* x, const double* hist,
                         double* out) {
   const int nwg = (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 8192);

@@ -715,6 +715,31 @@ void cup3d_poisson_destroy(cup3d_poisson_t H) {
   delete H;
 }
 
+double cup3d_debug_copy_rate(int device, int64_t bytes, int reps) {
+  if (bytes <= 0 || bytes % 16 || reps <= 0) return -1.0;
+  if (hipSetDevice(device) != hipSuccess) return -1.0;
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double rate = -1.0;
+  if (hipMalloc(&a, bytes) == hipSuccess && hipMalloc(&b, bytes) == hipSuccess &&
+      hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+      hipMemset(a, 0, bytes) == hipSuccess) {
+    launch_stream_copy(nullptr, bytes / 16, a, b);  // warm
+    float ms = 0.f;
+    if (hipEventRecord(e0, nullptr) == hipSuccess) {
+      for (int r = 0; r < reps; ++r) launch_stream_copy(nullptr, bytes / 16, a, b);
+      if (hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+          hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0.f)
+        rate = 2.0 * (double)bytes * reps / (ms * 1e-3);
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rate;
+}
+
 int cup3d_debug_profile(cup3d_poisson_t H, int enable) {
   if (!H) return -1;
   if (H->cfg.precision == CUP3D_PREC_FP64) return fail(H, "in-solve profiling covers the mixed form");

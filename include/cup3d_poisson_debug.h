@@ -54,6 +54,11 @@ int cup3d_debug_time_kernel(cup3d_poisson_t h, int which, int reps, double* avg_
  * (avg_us[3]: K1, K3, K5) and the number of samples, and clears them.
  * The FP64 form returns < 0. */
 int cup3d_debug_profile(cup3d_poisson_t h, int enable);
+/* Device-to-device streaming copy rate in bytes/s (read + written) of a
+ * nontemporal float4 copy between two buffers of `bytes` on `device`,
+ * averaged over `reps` copies: the practical HBM ceiling reported next to
+ * the datasheet peak. Returns < 0 on error. */
+double cup3d_debug_copy_rate(int device, int64_t bytes, int reps);
 int cup3d_debug_profile_read(cup3d_poisson_t h, double* avg_us, long long* samples);
 
 /* Full-batch FP32 gradient (n_weights floats) and loss of the learned warm
