@@ -236,8 +236,9 @@ def cpu_baseline(w, bmean, rhs, x0, gpu_iters, iters):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this run has a single MI355X; "
-              "the multi-GPU sharding is described in DESIGN.md, not launched.", file=sys.stderr)
+        print("bench.py measures one GPU (one MI355X per process); the solver has no "
+              "multi-GPU path: how it would shard over GPUs is a design note, not code.",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     from cup3d_amd.poisson import solver_for
