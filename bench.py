@@ -275,7 +275,7 @@ def main():
         b = torch.from_numpy(w.rhs(step=t).reshape(-1)).to(dev)
         x = p_prev.clone()
         r = R.solve_device(b, x)
-        x -= (hw * x).sum() / hw.sum()
+        R.remove_mean(x)  # the caller's projection, on the device
         if t > 0:
             seq.append({"rhs": b, "p_prev": p_prev, "x_ref": x})
             ref_res.append(r["true_rel_resid"])

@@ -24,6 +24,10 @@ void launch_update64(hipStream_t s, long long n, const KrylovVectors<double>& v,
 void launch_copy_opt64(hipStream_t s, long long n, const KrylovVectors<double>& v,
                        const SolverState* st, double* out);
 
+// x -= the volume-weighted mean of x (h3 per block, inv_vol = 1 / sum 512 h^3);
+// the weighted sum is left in wsum[0].
+void launch_remove_mean(hipStream_t s, long long n, double* x, const double* h3, double inv_vol,
+                        double* part, double* wsum);
 // b = a over n4 float4 (the copy-rate probe of the diagnostics).
 void launch_stream_copy(hipStream_t s, long long n4, const void* a, void* b);
 

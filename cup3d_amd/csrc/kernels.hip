@@ -301,6 +301,50 @@ void launch_stream_copy(hipStream_t s, long long n4, const void* a, void* b) {
                                             static_cast<f32x4*>(b));
 }
 
+// Caller-side projection after a solve (PressureProjection.cpp:404-431):
+// x -= sum_b h_b^3 sum x / sum_b 512 h_b^3, the volume-weighted mean, with the
+// sum in a fixed order.
+__global__ __launch_bounds__(WG_THREADS) void k_wsum(long long n, const double* __restrict__ x,
+                                                     const double* __restrict__ h3,
+                                                     double* part) {
+  double v[1] = {0.0};
+  const long long i0 = (long long)blockIdx.x * (4 * BLEN) + 8 * threadIdx.x;
+  if (i0 < n) {
+    double xv[8];
+    load8(x + i0, xv);
+    double s = 0.0;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) s += xv[e];
+    v[0] = h3[i0 / BLEN] * s;
+  }
+  grid_store_partial<1>(v, part);
+}
+
+__global__ __launch_bounds__(WG_THREADS) void k_wsum_fin(const double* part, int nparts,
+                                                         double* out) {
+  double tot[1];
+  grid_sum_partials<1>(part, nparts, tot);
+  if (threadIdx.x == 0) out[0] = tot[0];
+}
+
+__global__ __launch_bounds__(WG_THREADS) void k_sub_mean(long long n, double* __restrict__ x,
+                                                         const double* __restrict__ wsum,
+                                                         double inv_vol) {
+  const double m = wsum[0] * inv_vol;
+  for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n;
+       i += (long long)gridDim.x * WG_THREADS)
+    x[i] -= m;
+}
+
+void launch_remove_mean(hipStream_t s, long long n, double* x, const double* h3, double inv_vol,
+                        double* part, double* wsum) {
+  const int nwg = (int)((n + 4 * BLEN - 1) / (4 * BLEN));
+  k_wsum<<<nwg, WG_THREADS, 0, s>>>(n, x, h3, part);
+  k_wsum_fin<<<1, WG_THREADS, 0, s>>>(part, nwg, wsum);
+  const int nsub = (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 8192);
+  k_sub_mean<<<nsub, WG_THREADS, 0, s>>>(n, x, wsum, inv_vol);
+}
+
 void launch_extrapolate(hipStream_t s, long long n, const double* x, const double* hist,
                         double* out) {
   const int nwg = (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 8192);

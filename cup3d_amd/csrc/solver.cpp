@@ -487,6 +487,25 @@ int cup3d_poisson_solve_device(cup3d_poisson_t H, const double* d_rhs, double* d
   return st;
 }
 
+int cup3d_poisson_remove_mean(cup3d_poisson_t H, double* d_x, double* mean) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  if (!d_x) return fail(H, "null vector");
+  if ((uintptr_t)d_x & 15) return fail(H, "device vectors must be 16-byte aligned");
+  HIPCHECK(hipSetDevice(H->device));
+  double vol = 0.0;
+  for (double h3 : H->topo.h3) vol += 512.0 * h3;
+  const double inv_vol = 1.0 / vol;
+  launch_remove_mean(H->stream, H->n, d_x, H->d_h3.as<double>(), inv_vol,
+                     H->d_part.as<double>(), H->d_gdots.as<double>());
+  HIPCHECK(hipGetLastError());
+  double wsum = 0.0;
+  HIPCHECK(hipMemcpyAsync(&wsum, H->d_gdots.p, sizeof(double), hipMemcpyDeviceToHost, H->stream));
+  HIPCHECK(hipStreamSynchronize(H->stream));
+  if (mean) *mean = wsum * inv_vol;  // the value the kernel subtracted
+  return 0;
+}
+
 int cup3d_poisson_solve(cup3d_poisson_t H, const double* const* rhs_blocks,
                         double* const* x_blocks, const double* rhs_contig, double* x_contig,
                         cup3d_solve_report* rep) {

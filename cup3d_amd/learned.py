@@ -62,7 +62,7 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, bat
             rep = ref.solve_device(b, x)
             if rep["status"] != 0:
                 raise RuntimeError(f"step {t}: solve did not converge: {rep}")
-            x -= (hw * x).sum() / hw.sum()
+            ref.remove_mean(x)
             rhs.append(b)
             sols.append(x)
             its.append(rep["iterations"])

@@ -56,7 +56,7 @@ EXPORTS = [
     "cup3d_poisson_cnn_predict", "cup3d_poisson_cnn_add_sample",
     "cup3d_poisson_cnn_clear_samples", "cup3d_poisson_cnn_num_samples", "cup3d_poisson_cnn_train",
     "cup3d_poisson_cnn_eval",
-    "cup3d_poisson_num_coarse_fine_faces",
+    "cup3d_poisson_num_coarse_fine_faces", "cup3d_poisson_remove_mean",
     "cup3d_poisson_destroy", "cup3d_poisson_last_error",
 ]
 
@@ -84,6 +84,7 @@ def lib():
                                           ctypes.c_float, ctypes.c_uint64,
                                           ctypes.POINTER(ctypes.c_double)]
     L.cup3d_poisson_cnn_eval.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.cup3d_poisson_remove_mean.argtypes = [_vp, _vp, ctypes.POINTER(ctypes.c_double)]
     L.cup3d_poisson_num_coarse_fine_faces.argtypes = [_vp]
     L.cup3d_poisson_num_coarse_fine_faces.restype = ctypes.c_int64
     L.cup3d_poisson_destroy.argtypes = [_vp]
@@ -152,6 +153,13 @@ class PoissonSolver:
     @property
     def n(self):
         return self.nb * 512
+
+    def remove_mean(self, x):
+        """x -= its volume-weighted mean, on the device; returns the mean."""
+        m = ctypes.c_double()
+        if self.L.cup3d_poisson_remove_mean(self.h, _ptr(x), ctypes.byref(m)) != 0:
+            raise self._err("remove_mean")
+        return m.value
 
     def coarse_fine_faces(self):
         return self.L.cup3d_poisson_num_coarse_fine_faces(self.h)

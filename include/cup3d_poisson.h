@@ -146,6 +146,12 @@ int cup3d_poisson_solve(cup3d_poisson_t h, const double* const* rhs_blocks,
 int cup3d_poisson_solve_device(cup3d_poisson_t h, const double* d_rhs, double* d_x,
                                cup3d_solve_report* rep);
 
+/* The caller's projection after a solve (PressureProjection.cpp:404-431) on
+ * the device: x -= sum_b h_b^3 sum_cells x / sum_b 512 h_b^3 (the
+ * volume-weighted mean, summed in a fixed order). The mean removed is
+ * returned in *mean when mean is not NULL. */
+int cup3d_poisson_remove_mean(cup3d_poisson_t h, double* d_x, double* mean);
+
 /* y = A x with the solver's operator and its bMean handling: the bMean = 1
  * row, the bMean = 2 term (sum h^3 x) h^3, the bMean = 3 row (device
  * pointers). */

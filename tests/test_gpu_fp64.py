@@ -210,6 +210,24 @@ def test_coarse_fine_face_count(gpu, name):
     assert S.coarse_fine_faces() == count > 0
 
 
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "amr3_walls", "school4_small"])
+def test_remove_mean(gpu, name):
+    """The caller's projection (PressureProjection.cpp:404-431) on the device:
+    the volume-weighted mean it returns is numpy's, and afterwards the field
+    has zero weighted mean and differs from the input by that constant."""
+    blocks, bpd, h0, bc = CASES[name]
+    S = PoissonSolver(bpd, h0, bc, bmean=0).set_topology(blocks)
+    h3 = np.repeat((h0 / 2.0 ** np.asarray(blocks)[:, 0]) ** 3, 512)
+    x0 = random_field(len(blocks), 71).reshape(-1) + 3.0
+    x = dev(x0)
+    m = S.remove_mean(x)
+    me = (h3 @ x0) / h3.sum()
+    assert abs(m - me) <= 1e-14 * abs(me)
+    xr = x.cpu().numpy()
+    assert np.abs(xr - (x0 - m)).max() == 0.0
+    assert abs(h3 @ xr) <= 1e-14 * (h3 @ np.abs(xr))
+
+
 def test_deterministic(gpu):
     S, O, blocks, h0 = pair("amr3_walls", 1, tol_rel=1e-7)
     b = dev(compatible_rhs(blocks, h0, 7))
