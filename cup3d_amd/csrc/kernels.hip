@@ -217,9 +217,7 @@ void launch_precond64(hipStream_t s, int mode, int nb, const double* Ppk, const 
 
 void launch_update64(hipStream_t s, long long n, const KrylovVectors<double>& v, SolverState* st,
                      double* part) {
-  const int nwg = (int)((n + 4 * BLEN - 1) / (4 * BLEN));
-  k_update<double><<<nwg, WG_THREADS, 0, s>>>(n, v, st, part);
-  k_update_fin<<<1, WG_THREADS, 0, s>>>(st, part, nwg);
+  launch_k5<double>(s, n, v, st, part);
 }
 
 void launch_copy_opt64(hipStream_t s, long long n, const KrylovVectors<double>& v,

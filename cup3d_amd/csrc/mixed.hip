@@ -641,9 +641,7 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
         if (tm) MCHECK(hipEventRecord(M.pev[3], s));
         spmv32<SPMV_DOT2>(s, g32, v.z2, v.t, v.r, st, part);
         if (tm) MCHECK(hipEventRecord(M.pev[4], s));
-        k_update<float><<<grid_vec(n), WG_THREADS, 0, s>>>(n, v, st, part);
-        if (tm) MCHECK(hipEventRecord(M.pev[5], s));
-        k_update_fin<<<1, WG_THREADS, 0, s>>>(st, part, grid_vec(n));
+        launch_k5<float>(s, n, v, st, part, tm ? M.pev[5] : nullptr);
       }
       MCHECK(hipGetLastError());
       MCHECK(hipMemcpyAsync(h_st, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
@@ -765,8 +763,7 @@ int mixed_time_kernel(MixedWorkspace& M, hipStream_t s, int which, int reps,
       case 0: spmv32<SPMV_PLAIN>(s, g32, v.z1, v.nu, nullptr, st, part); break;
       case 1: launch_precond_mixed(M, s, precision, PREC_PLAIN, g32, v.p, v.z1, st); break;
       case 2:
-        k_update<float><<<grid_vec(M.n), WG_THREADS, 0, s>>>(M.n, v, st, part);
-        k_update_fin<<<1, WG_THREADS, 0, s>>>(st, part, grid_vec(M.n));
+        launch_k5<float>(s, M.n, v, st, part);
         break;
       case 3: launch_precond_mixed(M, s, precision, PREC_P, g32, nullptr, nullptr, st); break;
       default: spmv32<SPMV_DOT1>(s, g32, v.z1, v.nu, v.rhat, st, part); break;
