@@ -72,7 +72,7 @@ TRAFFIC = {
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=8)
+    p.add_argument("--steps", type=int, default=16)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--form", default="mixed_fp16", choices=["fp64", "mixed_fp16", "mixed_bf16"])
     p.add_argument("--workload", default="c5", choices=["c1", "c2", "c3", "c5"],
