@@ -418,24 +418,37 @@ static int warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* 
   const double G11 = a - 2 * d[0] + d[1], G22 = a - 2 * d[2] + d[3];
   const double G12 = a - d[0] - d[2] + d[4];
   const double f1 = a - d[0], f2 = a - d[2];
-  double g1 = 0, g2 = 0;
-  auto one = [](double G, double f) { return G > 0 && std::isfinite(G) ? f / G : 0.0; };
-  const double det = G11 * G22 - G12 * G12;
-  if (c1 && c2 && det > 1e-12 * G11 * G22 && std::isfinite(det)) {
-    g1 = (f1 * G22 - f2 * G12) / det;
-    g2 = (f2 * G11 - f1 * G12) / det;
-  } else if (c1 && c2) {  // (nearly) parallel candidates: the better single one
-    const double e1 = one(G11, f1) * f1, e2 = one(G22, f2) * f2;  // residual^2 drops
-    if (e1 >= e2) g1 = one(G11, f1);
-    else g2 = one(G22, f2);
-  } else if (c1) {
-    g1 = one(G11, f1);
-  } else {
-    g2 = one(G22, f2);
+  // Options: the given guess, either candidate alone (1-D least squares), or
+  // both (2x2). A direction is only taken if it lowers the squared residual
+  // by at least 1 % against the option without it: a direction that barely
+  // helps (the net next to the extrapolation has weight ~1e-3) would only
+  // perturb the guess.
+  constexpr double KEEP = 0.99;
+  auto ok = [](double v) { return std::isfinite(v); };
+  auto one = [&](double G, double f) { return G > 0 && ok(G) && ok(f) ? f / G : 0.0; };
+  const double gn = c1 ? one(G11, f1) : 0.0, gt = c2 ? one(G22, f2) : 0.0;
+  const double r_net = a - gn * f1, r_time = a - gt * f2;  // residual^2 of each alone
+  double g1 = 0, g2 = 0, r2n = a;
+  if (c2 && ok(r_time) && r_time < KEEP * r2n) {
+    g2 = gt;
+    r2n = r_time;
   }
-  const double r2n = a - (g1 * f1 + g2 * f2);  // residual^2 at the minimiser
-  if (!std::isfinite(g1) || !std::isfinite(g2) || !std::isfinite(r2n) || (g1 == 0 && g2 == 0) ||
-      !(r2n < a))
+  if (c1 && ok(r_net) && r_net < KEEP * r2n) {
+    g1 = gn;
+    g2 = 0;
+    r2n = r_net;
+  }
+  const double det = G11 * G22 - G12 * G12;
+  if (c1 && c2 && ok(det) && det > 1e-12 * G11 * G22) {
+    const double b1 = (f1 * G22 - f2 * G12) / det, b2 = (f2 * G11 - f1 * G12) / det;
+    const double r_both = a - (b1 * f1 + b2 * f2);
+    if (ok(b1) && ok(b2) && ok(r_both) && r_both < KEEP * std::min(r_net, r_time)) {
+      g1 = b1;
+      g2 = b2;
+      r2n = r_both;
+    }
+  }
+  if (!ok(g1) || !ok(g2) || !ok(r2n) || (g1 == 0 && g2 == 0) || !(r2n < a))
     return 0;  // fallback: the given guess
   // a candidate with weight 0 is not read (it may hold non-finite values)
   launch_guess_blend(s, H->n, x0, g1 != 0 ? c1 : nullptr, g2 != 0 ? c2 : nullptr, g1, g2, d_x);

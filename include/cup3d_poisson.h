@@ -72,8 +72,9 @@ enum {
  * 2 x - x_previous. The guess used is x + g_net (net - x) + g_time
  * (extrapolation - x) with the weights that minimise the FP64 residual
  * ||b - A guess|| (least squares over the candidates' residuals); weights 0
- * (the given x) are one of the choices, so the guess is never worse than x.
- * set_topology clears the history. */
+ * (the given x) are one of the choices, so the guess is never worse than x,
+ * and a candidate is only taken if it lowers the squared residual by at
+ * least 1 % against the choice without it. set_topology clears the history. */
 
 typedef struct {
   int bpd[3];        /* blocks per direction at level 0 (-bpdx/-bpdy/-bpdz) */

@@ -59,8 +59,9 @@ def test_extrapolation_exact_on_linear_sequence(gpu, name, prec):
 def test_extrapolation_weight_is_the_least_squares_step(gpu, name):
     """On a sequence that is not linear in time the weight is the minimiser of
     |b - A (x + g (x - x_prev))| and the reported guess residual its value,
-    both recomputed on the CPU with the oracle's operator; the guess is never
-    worse than the given one."""
+    both recomputed on the CPU with the oracle's operator (or 0 and the given
+    guess's residual when the step lowers the squared residual by less than
+    1 %); the guess is never worse than the given one."""
     blocks, bpd, h0, bc = CASES[name]
     O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
     S = PoissonSolver(bpd, h0, bc, bmean=0, tol_rel=0.0, tol_rhs=1e-9, max_iter=3000,
@@ -69,6 +70,7 @@ def test_extrapolation_weight_is_the_least_squares_step(gpu, name):
     base = [mean_free(random_field(len(blocks), 40 + k), blocks, h0) for k in range(3)]
     x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
     hist = None
+    taken = 0
     for t in range(4):
         c = [np.cos(0.7 * t + k) for k in range(3)] + rng.uniform(-0.05, 0.05, 3)
         b = O.spmv(sum(ck * bk for ck, bk in zip(c, base)))
@@ -80,11 +82,15 @@ def test_extrapolation_weight_is_the_least_squares_step(gpu, name):
             u = O.spmv(xin - hist)
             g = (r0 @ u) / (u @ u)
             resid = np.linalg.norm(r0 - g * u)
+            if resid ** 2 >= 0.99 * (r0 @ r0):  # less than 1 %: the given guess is kept
+                g, resid = 0.0, np.linalg.norm(r0)
             assert abs(rep["guess_weight_time"] - g) <= 1e-9 * max(1.0, abs(g)), (rep, g)
             assert abs(rep["guess_resid"] - resid) <= 1e-9 * resid
             assert rep["guess_resid"] <= np.linalg.norm(r0)
+            taken += g != 0.0
         hist = xin
         x.copy_(dev(mean_free(x.cpu().numpy(), blocks, h0)))
+    assert taken >= 2  # the test has teeth: the extrapolation was used
 
 
 def test_set_topology_clears_the_history(gpu):
@@ -102,8 +108,9 @@ def test_set_topology_clears_the_history(gpu):
         rep = S.solve_device(dev(O.spmv(p0 + t * d)), x)
         assert rep["status"] == 0
         weights.append(rep["guess_weight_time"])
-    assert weights[0] == 0.0 and weights[2] == 0.0  # t = 2: history cleared
-    assert weights[1] != 0.0  # t = 1 extrapolates from the zero guess of t = 0
+    # t = 0: no history; t = 2: history cleared (t = 1 extrapolates from the
+    # zero guess of t = 0, a direction that does not reach the 1 % gain here)
+    assert weights[0] == 0.0 and weights[2] == 0.0
     assert abs(weights[3] - 1.0) < 1e-6
 
 
