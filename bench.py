@@ -60,9 +60,9 @@ COPY_BYTES = 256 << 20  # calibration copy for the FETCH_SIZE / WRITE_SIZE count
 # profiles/mixed_fp16_uniform_256x128x128_traffic.md: (workload, form,
 # kernel) -> bytes.
 TRAFFIC = {
-    ("amr4_school50_blobs64_sequence", "mixed_fp16", "precond_update_K1"): 615.04e6,
-    ("amr4_school50_blobs64_sequence", "mixed_fp16", "spmv_dot_K2"): 402.09e6,
-    ("amr4_school50_blobs64_sequence", "mixed_fp16", "update_K5"): 967.39e6,
+    ("amr4_school50_blobs64_sequence", "mixed_fp16", "precond_update_K1"): 618.95e6,
+    ("amr4_school50_blobs64_sequence", "mixed_fp16", "spmv_dot_K2"): 402.15e6,
+    ("amr4_school50_blobs64_sequence", "mixed_fp16", "update_K5"): 971.88e6,
     ("uniform_256x128x128_walls_blobs256_sphere", "mixed_fp16", "precond_update_K1"): 88.25e6,
     ("uniform_256x128x128_walls_blobs256_sphere", "mixed_fp16", "spmv_dot_K2"): 53.10e6,
     ("uniform_256x128x128_walls_blobs256_sphere", "mixed_fp16", "update_K5"): 134.3e6,
