@@ -230,10 +230,11 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
   if (nblocks <= 0 || !blocks) return fail(H, "empty block list");
   if (nblocks > (1LL << 22)) return fail(H, "too many blocks (limit 4194304)");
   HIPCHECK(hipSetDevice(H->device));
-  H->blocks.resize(nblocks * 4);
+  // validate into locals: a rejected block list leaves the handle as it was
+  std::vector<int> bl(nblocks * 4);
   for (int64_t b = 0; b < nblocks; ++b) {
-    H->blocks[4 * b] = blocks[b].level;
-    for (int d = 0; d < 3; ++d) H->blocks[4 * b + 1 + d] = blocks[b].index[d];
+    bl[4 * b] = blocks[b].level;
+    for (int d = 0; d < 3; ++d) bl[4 * b + 1 + d] = blocks[b].index[d];
   }
   TopologyInput in;
   for (int d = 0; d < 3; ++d) {
@@ -244,9 +245,21 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
   in.h0 = H->cfg.h0;
   in.bmean = H->cfg.bmean;
   in.nb = (int)nblocks;
-  in.blocks = H->blocks.data();
+  in.blocks = bl.data();
   std::string e;
-  if (!build_topology(in, H->topo, e)) return fail(H, e);
+  HostTopology Tn;
+  if (!build_topology(in, Tn, e)) return fail(H, e);
+  H->blocks.swap(bl);
+  H->topo = std::move(Tn);
+  // from here on a failure (device allocation or upload) leaves the handle
+  // without a topology: solves are refused until set_topology succeeds
+  struct Commit {
+    cup3d_poisson* h;
+    bool ok = false;
+    ~Commit() {
+      if (!ok) h->nb = 0;
+    }
+  } commit{H};
   HostTopology& T = H->topo;
   H->nb = (int)nblocks;
   H->n = (long long)nblocks * BLEN;
@@ -307,6 +320,7 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
   if (cnn_set_topology(H->cnn, H->nb, H->d_nbr.as<int>(), H->d_h.as<double>(), H->err) != 0)
     return -1;
   HIPCHECK(hipDeviceSynchronize());
+  commit.ok = true;
   return 0;
 }
 

@@ -52,6 +52,31 @@ def test_create_rejects_bad_config():
     assert b"bc" in L.cup3d_poisson_last_error(None)
 
 
+@pytest.mark.parametrize("field,value,message", [
+    ("bmean", 4, b"bmean"), ("max_iter", 0, b"max_iter"), ("precision", 3, b"precision"),
+    ("guess", 3, b"guess"), ("extrapolate", 2, b"extrapolate"), ("h0", 0.0, b"h0")])
+def test_create_rejects_each_bad_field(field, value, message):
+    """Every configuration field is checked before any device is touched."""
+    L = P.lib()
+    c = P.Config()
+    for d in range(3):
+        c.bpd[d] = 2
+        c.bc[d] = 2
+    c.h0, c.max_iter = 0.1, 10
+    setattr(c, field, value)
+    h = ctypes.c_void_p()
+    assert L.cup3d_poisson_create(ctypes.byref(c), ctypes.byref(h)) == -1
+    assert message in L.cup3d_poisson_last_error(None)
+
+
+def test_null_handle_calls_fail_cleanly():
+    L = P.lib()
+    assert L.cup3d_poisson_solve_device(None, None, None, None) < 0
+    assert L.cup3d_poisson_remove_mean(None, None, None) < 0
+    assert L.cup3d_poisson_num_coarse_fine_faces(None) < 0
+    L.cup3d_poisson_destroy(None)
+
+
 def debug_csr(blocks, bpd, h0, bc, bmean):
     L = load_library("libcup3d_poisson.so")
     f = L.cup3d_debug_operator_csr
@@ -95,6 +120,16 @@ def test_topology_operator_matches_oracle_assembly(name, bmean):
     D.eliminate_zeros()
     scale = np.abs(B.data).max()
     assert (abs(D).max() if D.nnz else 0.0) <= 1e-14 * scale
+
+
+@pytest.mark.parametrize("blocks", [
+    [[0, 0, 0, 0], [0, 0, 0, 0]],   # duplicate leaf
+    [[0, 2, 0, 0], [0, 1, 0, 0]],   # index outside the 2x1x1 domain
+    [[-1, 0, 0, 0], [0, 1, 0, 0]],  # invalid level
+])
+def test_topology_rejects_bad_block_lists(blocks):
+    with pytest.raises(AssertionError):
+        debug_csr(np.array(blocks, np.int32), (2, 1, 1), 1 / 16, (2, 2, 2), 0)
 
 
 def test_topology_rejects_unbalanced_grid():

@@ -228,6 +228,33 @@ def test_remove_mean(gpu, name):
     assert abs(h3 @ xr) <= 1e-14 * (h3 @ np.abs(xr))
 
 
+def test_host_checked_errors(gpu):
+    """Bad calls are refused on the host with a message, before any kernel:
+    solving before set_topology, null or misaligned vectors, a block list
+    that is not 2:1 balanced (which leaves the previous topology in place);
+    the handle stays usable afterwards."""
+    from cup3d_amd.poisson import lib
+    L = lib()
+    blocks, bpd, h0, bc = CASES["uniform_walls_3x2x2"]
+    S = PoissonSolver(bpd, h0, bc, bmean=0, tol_rel=1e-8)
+    n = len(blocks) * 512
+    x = torch.zeros(n + 2, dtype=torch.float64, device=gpu)
+    b = torch.ones(n + 2, dtype=torch.float64, device=gpu)
+    assert L.cup3d_poisson_solve_device(S.h, b.data_ptr(), x.data_ptr(), None) < 0
+    assert b"set_topology" in L.cup3d_poisson_last_error(S.h)
+    S.set_topology(blocks)
+    assert L.cup3d_poisson_solve_device(S.h, None, x.data_ptr(), None) < 0
+    assert L.cup3d_poisson_solve_device(S.h, b.data_ptr() + 8, x.data_ptr(), None) < 0
+    assert b"aligned" in L.cup3d_poisson_last_error(S.h)
+    with pytest.raises(RuntimeError):
+        S.set_topology(np.array([[2, 0, 0, 0], [0, 1, 0, 0]], np.int32))
+    assert b"balance" in L.cup3d_poisson_last_error(S.h)
+    # the rejected block list left the previous topology in place
+    rhs = compatible_rhs(blocks, h0, 81)
+    xs = torch.zeros(n, dtype=torch.float64, device=gpu)
+    assert S.solve_device(dev(rhs), xs)["status"] == 0
+
+
 def test_deterministic(gpu):
     S, O, blocks, h0 = pair("amr3_walls", 1, tol_rel=1e-7)
     b = dev(compatible_rhs(blocks, h0, 7))
