@@ -20,6 +20,15 @@ __device__ __forceinline__ int xcd_remap(int i, int G) {
   return x * q + (x < r ? x : r) + j;
 }
 
+// Orders this wavefront's LDS stores before its later LDS loads of the same
+// words (the LDS executes one wavefront's operations in issue order; the fence
+// keeps the compiler from moving accesses across this point). Enough where a
+// wavefront reads back only what it wrote itself.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -213,8 +213,7 @@ __device__ __forceinline__ void iface_face(const DevGrid<T>& g, const T* __restr
     // stage the coarse block's face layer, [t2][t1]
     const int cb = nb0;
     pl[lane] = x[(size_t)cb * BLEN + (minus ? 7 : 0) * sd + u * st1 + v * st2];
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();
     const T h = g.h[b];
     const int c1 = (u >> 1) + (((par >> t1) & 1) ? BS / 2 : 0);
     const int c2 = (v >> 1) + (((par >> t2) & 1) ? BS / 2 : 0);
@@ -258,8 +257,7 @@ __device__ __forceinline__ void iface_face(const DevGrid<T>& g, const T* __restr
     acc = hf * (sc * (T)(1.0 / 3.0) + sf * (T)0.2 - (T)(8.0 / 15.0) * tsum);
   }
   ys[cl] += acc;
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
+  wave_sync();
 }
 
 // ---------------------------------------------------------------------------
@@ -270,9 +268,11 @@ __device__ __forceinline__ void iface_face(const DevGrid<T>& g, const T* __restr
 // One wavefront per block: lane = (ix, iy), each lane walks its z column in
 // registers; the block is staged in LDS for the x/y neighbours. Every global
 // load (the column, the epilogue operand, the six face halos of same-level
-// neighbours) is issued before the single barrier, so a wave waits for memory
+// neighbours) is issued before the LDS staging, so a wave waits for memory
 // once. Grids with coarse-fine faces (IFACE) stage y in LDS once more to add
-// the interface rows.
+// the interface rows. A wavefront reads only its own LDS slices, so it
+// synchronises with itself alone (wave_sync) and never waits for the other
+// three; the workgroup barrier is the partial-sum reduction at the end.
 // ---------------------------------------------------------------------------
 template <typename T, int MODE, int BM, bool IFACE>
 __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __restrict__ x,
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
     }
     if (IFACE && lane < 31) fmeta[w][lane] = meta;
   }
-  __syncthreads();
+  wave_sync();
   T yv[BS];
   double msum = 0.0;
   if (valid) {
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
 #pragma unroll
       for (int iz = 0; iz < BS; ++iz) ys[w][iz * 64 + lane] = yv[iz];
     }
-    __syncthreads();
+    wave_sync();
     if (valid) {
       const int par = fmeta[w][30];
       const int fq = (ix >= BS / 2 ? 1 : 0) + (iy >= BS / 2 ? 2 : 0);
@@ -375,7 +375,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
                         fmeta[w][4 * f + fq]);
       }
     }
-    __syncthreads();
+    wave_sync();
     if (valid) {
 #pragma unroll
       for (int iz = 0; iz < BS; ++iz) yv[iz] = ys[w][iz * 64 + lane];
