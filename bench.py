@@ -270,7 +270,7 @@ def main():
     ntrain = args.train if learned else 0
     nsteps = ntrain + args.warmup + args.steps
     p_prev = torch.zeros(w.ncells, dtype=torch.float64, device=dev)
-    seq, ref_res = [], []
+    seq, ref_res, ref_ok = [], [], 0
     for t in range(nsteps + 1):
         b = torch.from_numpy(w.rhs(step=t).reshape(-1)).to(dev)
         x = p_prev.clone()
@@ -279,6 +279,7 @@ def main():
         if t > 0:
             seq.append({"rhs": b, "p_prev": p_prev, "x_ref": x})
             ref_res.append(r["true_rel_resid"])
+            ref_ok += r["status"] == 0
         p_prev = x
         print(f"setup: reference solve of step {t}/{nsteps} done", file=sys.stderr, flush=True)
     del R
@@ -422,6 +423,7 @@ def main():
         "cell_iterations_per_s": round(w.ncells * sm["iterations_mean"] / (ms * 1e-3), 1),
         "solves": sm,
         "reference": {"form": "mixed_fp16 refined to ||b-Ax|| <= 1e-9 ||b|| (FP64 residual)",
+                      "converged": f"{ref_ok}/{len(ref_res)}",
                       "true_rel_resid_max": max(ref_res)},
         "fp64": {"solves_per_s": round(1e3 / ms64, 3), "ms_per_step": round(ms64, 3),
                  "max_iter": 1000, "solves": sm64,

@@ -611,7 +611,11 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
 #ifndef CUP3D_STALL_ITERS
 #define CUP3D_STALL_ITERS 50
 #endif
-  constexpr int STALL_ITERS = CUP3D_STALL_ITERS;
+  // The window doubles after every stall restart that did not halve the FP64
+  // residual: a recurrence that needs longer than the window for its first
+  // halving (BiCGSTAB's residual often rises before it falls) would otherwise
+  // repeat the same unproductive restart until max_refine is spent.
+  int stall_iters = CUP3D_STALL_ITERS;
   while (!done && steps <= cfg.max_refine) {
     const double scale = rnorm;
     const double inner_rel = std::max(cfg.inner_tol, 0.5 * tol / scale);
@@ -665,7 +669,7 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
       }
       // a recurrence that has not halved its best residual for STALL_ITERS
       // iterations is restarted from the FP64 residual of its best iterate
-      const bool stalled = ist == ST_RUNNING && h_st->iter - progress_iter > STALL_ITERS;
+      const bool stalled = ist == ST_RUNNING && h_st->iter - progress_iter > stall_iters;
       if (stalled) ist = ST_MAXITER;
       else if (ist == ST_RUNNING && h_st->err > REPLACE_DROP * last) continue;
       if (ist == ST_NONFINITE) {
@@ -704,6 +708,8 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
       } else if (rnorm <= tol) {
         status = ST_CONVERGED;
         done = true;
+      } else if (stalled && rnorm > 0.5 * scale) {
+        stall_iters *= 2;
       }
       break;
     }

@@ -94,7 +94,10 @@ typedef struct {
   /* mixed precision only: floor of the FP32 Krylov tolerance relative to the
      residual it starts from (it targets max(inner_tol, 0.5 tol / |r|);
      0 = 1e-7), and the cap on restarts of the FP32 Krylov process (each
-     breakdown or iteration cap starts a new one from the FP64 residual) */
+     breakdown, iteration cap or stall starts a new one from the FP64
+     residual; a stall is 50 iterations without halving the best residual,
+     a window that doubles after every stall restart that did not halve the
+     FP64 residual) */
   double inner_tol;
   int max_refine;
   int poll_every;    /* iterations between host polls of the device status (0 = default) */
