@@ -818,14 +818,14 @@ int cnn_gradient(CnnState& C, hipStream_t s, int k, float res_weight, float* gra
 
 void cnn_free(CnnState& C) {
   cnn_clear_samples(C);
-  hipFree(C.d_w);
-  hipFree(C.d_adam);
-  hipFree(C.d_wpk);
-  hipFree(C.d_scal);
-  if (C.d_part) hipFree(C.d_part);
-  if (C.d_pred) hipFree(C.d_pred);
-  if (C.d_scratch) hipFree(C.d_scratch);
-  if (C.d_items) hipFree(C.d_items);
+  (void)hipFree(C.d_w);
+  (void)hipFree(C.d_adam);
+  (void)hipFree(C.d_wpk);
+  (void)hipFree(C.d_scal);
+  if (C.d_part) (void)hipFree(C.d_part);
+  if (C.d_pred) (void)hipFree(C.d_pred);
+  if (C.d_scratch) (void)hipFree(C.d_scratch);
+  if (C.d_items) (void)hipFree(C.d_items);
   C = CnnState();
 }
 

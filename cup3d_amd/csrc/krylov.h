@@ -123,8 +123,8 @@ __global__ __launch_bounds__(WG_THREADS) void k_getvec(long long n, long long ze
   grid_store_partial<1>(v, part);
 }
 
-static __global__ __launch_bounds__(WG_THREADS) void k_getvec_fin(SolverState* st, const double* part,
-                                                           int nparts, double tol_b) {
+[[maybe_unused]] static __global__ __launch_bounds__(WG_THREADS) void k_getvec_fin(
+    SolverState* st, const double* part, int nparts, double tol_b) {
   __shared__ SolverState S;
   double tot[1];
   fin_load<1>(st, S, part, nparts, tot);
