@@ -71,9 +71,13 @@ __device__ __forceinline__ void grid_store_partial(double (&v)[K], double* __res
 template <int K>
 __device__ __forceinline__ void grid_sum_partials(const double* __restrict__ part, int nparts,
                                                   double (&tot)[K]) {
-  // Eight partials per thread are loaded before any is added, so the loads of
-  // up to 8 * blockDim.x partials cost one memory round trip, not eight.
-  constexpr int U = 8;
+  // U partials per thread are loaded before any is added, so the loads of up
+  // to U * blockDim.x partials cost one memory round trip, not U (the
+  // producers' partials sit in the L2 of every XCD or in HBM). U * K = 60
+  // loads stay within the 63 a wavefront can have outstanding. Each thread
+  // adds its partials in index order whatever U is, so U does not change
+  // the result.
+  constexpr int U = 60 / K;
   __shared__ double red[K][4];
   double acc[K];
 #pragma unroll
