@@ -128,3 +128,33 @@ def fft_solution_periodic(rhs, blocks, bpd, h0, row):
     xh[0, 0, 0] = 0.0
     x = np.real(np.fft.ifftn(xh))
     return global_to_blocks(x, blocks).reshape(-1)
+
+
+def index_encoded_case(blocks, bpd, h0, bc):
+    """Known answer for the block-neighbour tables on a uniform level-0 grid
+    (the idea of the reference's tests/cubismup3d/TestBoundaries.cpp:19-82:
+    an index-encoded field whose shifted values are known exactly).
+
+    x = i + Nx j + Nx Ny k on global cell (i, j, k). The 7-point Laplacian of
+    a linear field is 0 away from the domain faces, so h (sum of the six
+    neighbours - 6 x) is nonzero only on the faces: a periodic wrap brings
+    the value from the far side, +/- (stride of the direction) x (cells in
+    it); a wall or freespace ghost equals the centre, +/- stride. All values
+    are integers times a power of two: the answer is exact in FP64 and FP32.
+    Returns (x, A x) in block layout, flat."""
+    blocks = np.asarray(blocks)
+    assert (blocks[:, 0] == 0).all()
+    n = [8 * b for b in bpd]
+    stride = [1, n[0], n[0] * n[1]]
+    r = np.arange(8)
+    iz, iy, ix = np.meshgrid(r, r, r, indexing="ij")
+    g = [blocks[:, 1, None, None, None] * 8 + ix[None],
+         blocks[:, 2, None, None, None] * 8 + iy[None],
+         blocks[:, 3, None, None, None] * 8 + iz[None]]
+    x = sum(stride[d] * g[d] for d in range(3)).astype(np.float64)
+    y = np.zeros_like(x)
+    for d in range(3):
+        jump = stride[d] * (n[d] if bc[d] == PER else 1)
+        y += jump * (g[d] == 0) - jump * (g[d] == n[d] - 1)
+    return x.reshape(-1), (h0 * y).reshape(-1)
+

@@ -14,8 +14,8 @@ import torch
 
 from cup3d_amd import workloads as W
 from cup3d_amd.poisson import PoissonSolver, solver_for
-from helpers import (WALL, fft_solution_periodic, grid_cases, random_field, school4_small,
-                     tiny_amr, uniform)
+from helpers import (PER, WALL, fft_solution_periodic, grid_cases, index_encoded_case,
+                     random_field, school4_small, tiny_amr, uniform)
 
 pytestmark = pytest.mark.gpu
 
@@ -51,6 +51,27 @@ def test_operator_matches_oracle(gpu, name, bmean):
     yo = O.spmv(x)
     assert torch.isfinite(y).all()
     assert np.abs(y.cpu().numpy() - yo).max() <= 1e-13 * np.abs(yo).max()
+
+
+@pytest.mark.parametrize("bpd,bc", [((2, 3, 2), (PER, PER, PER)), ((3, 2, 2), (WALL,) * 3),
+                                    ((2, 2, 3), (PER, WALL, 0))])
+def test_operator_index_encoded_field(gpu, bpd, bc):
+    """The GPU neighbour tables and block order against a closed form, not the
+    oracle: A x for x = i + Nx j + Nx Ny k is exact (integers times a power of
+    two) in the FP64 operator and in the inner solve's FP32 operator."""
+    import ctypes
+    from cup3d_amd.poisson import lib
+    blocks, bpd, h0, bc = uniform(bpd, bc, h0=1.0 / 32)
+    x, y = index_encoded_case(blocks, bpd, h0, bc)
+    S = PoissonSolver(bpd, h0, bc, bmean=0, precision="mixed_fp16").set_topology(blocks)
+    y64 = S.apply_operator(dev(x), torch.zeros(len(x), dtype=torch.float64, device=gpu))
+    assert np.array_equal(y64.cpu().numpy(), y)
+    f = lib().cup3d_debug_apply_operator32
+    f.argtypes = [ctypes.c_void_p] * 3
+    xd, y32 = dev(x), torch.zeros(len(x), dtype=torch.float64, device=gpu)
+    assert f(S.h, xd.data_ptr(), y32.data_ptr()) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(y32.cpu().numpy(), y)
 
 
 @pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "amr3_walls", "amr2_periodic_offcentre"])

@@ -11,7 +11,7 @@ import pytest
 import scipy.sparse as sp
 
 from helpers import (PER, WALL, amr_sphere, boundary_cells, cell_centres, grid_cases,
-                     random_field, uniform)
+                     index_encoded_case, random_field, uniform)
 
 
 def make(case, bmean=0):
@@ -91,6 +91,21 @@ def test_uniform_matches_kronecker_bruteforce(bc):
     A_ref = G.tocsr()[perm][:, perm]
     A = O.scipy_matrix()
     assert abs(A - A_ref).max() < 1e-15
+
+
+INDEX_CASES = [((2, 3, 2), (PER, PER, PER)), ((3, 2, 2), (WALL, WALL, WALL)),
+               ((2, 2, 3), (PER, WALL, 0))]
+
+
+@pytest.mark.parametrize("bpd,bc", INDEX_CASES)
+def test_index_encoded_field_known_answer(bpd, bc):
+    """Neighbour tables, block ordering and periodic wrap: A applied to the
+    index-encoded field x = i + Nx j + Nx Ny k equals the closed form of
+    helpers.index_encoded_case exactly (nonzero only on the domain faces)."""
+    case = uniform(bpd, bc, h0=1.0 / 32)
+    x, y = index_encoded_case(*case)
+    assert np.array_equal(make(case).spmv(x), y)
+    assert np.count_nonzero(y) > 0
 
 
 @pytest.mark.parametrize("name", list(grid_cases()))
