@@ -3,7 +3,8 @@
   libcup3d_poisson.so   HIP kernels for gfx950 + C++ host driver + C ABI
                         (include/cup3d_poisson.h), built with hipcc
   libcup3d_workloads.so synthetic grids and right-hand sides (g++, OpenMP)
-  libcup3d_adapter.so   C++ PoissonSolverBase adapter over the C ABI (g++)
+  example_projection    a caller of the header-only C++ PoissonSolverBase
+                        adapter (cup3d_amd/host) on the standalone stand-in types
 
 Run: python -m cup3d_amd.build
 """
@@ -70,27 +71,17 @@ def build_workloads():
     return out
 
 
-def build_adapter():
-    os.makedirs(LIB, exist_ok=True)
-    out = os.path.join(LIB, "libcup3d_adapter.so")
-    _run([CXX, "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-I", INCLUDE, "-I", HOST,
-          os.path.join(HOST, "PoissonSolverHIP.cpp"), "-o", out + ".tmp",
+def build_example():
+    out = os.path.join(LIB, "example_projection")
+    _run([CXX, "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, "-I", HOST,
+          os.path.join(HOST, "example_projection.cpp"), "-o", out + ".tmp",
           "-L", LIB, "-lcup3d_poisson", "-Wl,-rpath,$ORIGIN"])
     os.replace(out + ".tmp", out)
     return out
 
 
-def build_example():
-    out = os.path.join(LIB, "example_projection")
-    _run([CXX, "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, "-I", HOST,
-          os.path.join(HOST, "example_projection.cpp"), "-o", out + ".tmp",
-          "-L", LIB, "-lcup3d_adapter", "-lcup3d_poisson", "-Wl,-rpath,$ORIGIN"])
-    os.replace(out + ".tmp", out)
-    return out
-
-
 def build_all():
-    return [build_workloads(), build_poisson(), build_adapter(), build_example()]
+    return [build_workloads(), build_poisson(), build_example()]
 
 
 if __name__ == "__main__":

@@ -35,6 +35,8 @@ void launch_stream_copy(hipStream_t s, long long n4, const void* a, void* b);
 // out = 2 x - hist (linear extrapolation in time from the last two steps);
 void launch_extrapolate(hipStream_t s, long long n, const double* x, const double* hist,
                         double* out);
+// x[row] = 0 (nothing for row < 0): the bMean = 1 / 3 cell that getVec zeroes;
+void launch_zero_cell(hipStream_t s, double* x, long long row);
 // dots[0..4] = (r0, r1), (r1, r1), (r0, r2), (r2, r2), (r1, r2) in a fixed
 // order (r1 or r2 may be null: their dots are 0);
 void launch_guess_dots(hipStream_t s, long long n, const double* r0, const double* r1,

@@ -349,6 +349,12 @@ void launch_extrapolate(hipStream_t s, long long n, const double* x, const doubl
   k_extrapolate<<<nwg, WG_THREADS, 0, s>>>(n, x, hist, out);
 }
 
+__global__ void k_zero_cell(double* __restrict__ x, long long row) { x[row] = 0.0; }
+
+void launch_zero_cell(hipStream_t s, double* x, long long row) {
+  if (row >= 0) k_zero_cell<<<1, 1, 0, s>>>(x, row);
+}
+
 void launch_guess_dots(hipStream_t s, long long n, const double* r0, const double* r1,
                        const double* r2, double* part, double* dots) {
   const int nwg = (int)((n + 4 * BLEN - 1) / (4 * BLEN));

@@ -557,15 +557,18 @@ __global__ __launch_bounds__(WG_THREADS) void k_bmean2_apply(DevGrid<T> g, T* __
 }
 
 // Launch for a grid: picks the bMean and interface variants.
+// An event, if given, is recorded right after the main SpMV kernel.
 template <typename T, int MODE, int BM>
 inline void spmv_launch_iface(hipStream_t s, const DevGrid<T>& g, const T* x, T* y, const T* a,
-                              T* c, T* c2, SolverState* st, double* part) {
+                              T* c, T* c2, SolverState* st, double* part,
+                              hipEvent_t mid = nullptr) {
   int nwg = (g.nb + 3) / 4;
   if (g.has_iface) {
     k_spmv<T, MODE, BM, true><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
   } else {
     k_spmv<T, MODE, BM, false><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
   }
+  if (mid) (void)hipEventRecord(mid, s);
   if (BM == 2) {
     k_bmean2_sum<MODE><<<1, WG_THREADS, 0, s>>>(st, part, nwg);
     // nb * 64 threads of 8 cells: (nb + 3) / 4 workgroups again, so the
@@ -577,11 +580,11 @@ inline void spmv_launch_iface(hipStream_t s, const DevGrid<T>& g, const T* x, T*
 }
 template <typename T, int MODE>
 inline void spmv_launch(hipStream_t s, const DevGrid<T>& g, const T* x, T* y, const T* a, T* c,
-                        T* c2, SolverState* st, double* part) {
-  if (g.bmean == 1) spmv_launch_iface<T, MODE, 1>(s, g, x, y, a, c, c2, st, part);
-  else if (g.bmean == 2) spmv_launch_iface<T, MODE, 2>(s, g, x, y, a, c, c2, st, part);
-  else if (g.bmean == 3) spmv_launch_iface<T, MODE, 3>(s, g, x, y, a, c, c2, st, part);
-  else spmv_launch_iface<T, MODE, 0>(s, g, x, y, a, c, c2, st, part);
+                        T* c2, SolverState* st, double* part, hipEvent_t mid = nullptr) {
+  if (g.bmean == 1) spmv_launch_iface<T, MODE, 1>(s, g, x, y, a, c, c2, st, part, mid);
+  else if (g.bmean == 2) spmv_launch_iface<T, MODE, 2>(s, g, x, y, a, c, c2, st, part, mid);
+  else if (g.bmean == 3) spmv_launch_iface<T, MODE, 3>(s, g, x, y, a, c, c2, st, part, mid);
+  else spmv_launch_iface<T, MODE, 0>(s, g, x, y, a, c, c2, st, part, mid);
 }
 
 // ---------------------------------------------------------------------------

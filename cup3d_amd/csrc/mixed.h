@@ -23,11 +23,12 @@ struct MixedWorkspace {
   int nb = 0;
   int timing_precision = 1;
   KrylovVectors<float> v32{};
-  // in-solve kernel timing: HIP events around K1, K3 and K5 of the middle
-  // iteration of every polled batch, summed while prof is set
+  // in-solve kernel timing: HIP events around the main kernels of K1 .. K5
+  // (the finalize kernels excluded) of the middle iteration of every polled
+  // batch, summed while prof is set
   bool prof = false;
-  hipEvent_t pev[6] = {};
-  double prof_ms[3] = {0.0, 0.0, 0.0};
+  hipEvent_t pev[10] = {};
+  double prof_ms[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   long long prof_n = 0;
 };
 

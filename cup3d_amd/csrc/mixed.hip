@@ -476,8 +476,8 @@ void launch_precond_mixed(const MixedWorkspace& M, hipStream_t s, int precision,
 
 template <int MODE>
 void spmv32(hipStream_t s, const DevGrid<float>& g, const float* x, float* y, const float* a,
-            SolverState* st, double* part) {
-  spmv_launch<float, MODE>(s, g, x, y, a, nullptr, nullptr, st, part);
+            SolverState* st, double* part, hipEvent_t mid = nullptr) {
+  spmv_launch<float, MODE>(s, g, x, y, a, nullptr, nullptr, st, part, mid);
 }
 
 int grid_1d(long long n) { return (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 8192); }
@@ -635,23 +635,26 @@ int mixed_solve(MixedWorkspace& M, hipStream_t s, const cup3d_poisson_config& cf
       const int tm_it = poll / 2;
       const int iter0 = h_st->iter;
       for (int it = 0; it < poll; ++it) {
+        // event pairs (2k, 2k+1) bracket the main kernel of K(k+1)
         const bool tm = M.prof && it == tm_it;
         if (tm) MCHECK(hipEventRecord(M.pev[0], s));
         launch_precond_mixed(M, s, cfg.precision, PREC_P, g32, nullptr, nullptr, st);
         if (tm) MCHECK(hipEventRecord(M.pev[1], s));
-        spmv32<SPMV_DOT1>(s, g32, v.z1, v.nu, v.rhat, st, part);
         if (tm) MCHECK(hipEventRecord(M.pev[2], s));
-        launch_precond_mixed(M, s, cfg.precision, PREC_S, g32, nullptr, nullptr, st);
-        if (tm) MCHECK(hipEventRecord(M.pev[3], s));
-        spmv32<SPMV_DOT2>(s, g32, v.z2, v.t, v.r, st, part);
+        spmv32<SPMV_DOT1>(s, g32, v.z1, v.nu, v.rhat, st, part, tm ? M.pev[3] : nullptr);
         if (tm) MCHECK(hipEventRecord(M.pev[4], s));
-        launch_k5<float>(s, n, v, st, part, tm ? M.pev[5] : nullptr);
+        launch_precond_mixed(M, s, cfg.precision, PREC_S, g32, nullptr, nullptr, st);
+        if (tm) MCHECK(hipEventRecord(M.pev[5], s));
+        if (tm) MCHECK(hipEventRecord(M.pev[6], s));
+        spmv32<SPMV_DOT2>(s, g32, v.z2, v.t, v.r, st, part, tm ? M.pev[7] : nullptr);
+        if (tm) MCHECK(hipEventRecord(M.pev[8], s));
+        launch_k5<float>(s, n, v, st, part, tm ? M.pev[9] : nullptr);
       }
       MCHECK(hipGetLastError());
       MCHECK(hipMemcpyAsync(h_st, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
       MCHECK(hipStreamSynchronize(s));
       if (M.prof && h_st->iter - iter0 > tm_it) {
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 5; ++k) {
           float ms = 0.f;
           MCHECK(hipEventElapsedTime(&ms, M.pev[2 * k], M.pev[2 * k + 1]));
           M.prof_ms[k] += ms;

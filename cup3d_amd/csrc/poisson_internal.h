@@ -54,7 +54,6 @@ struct alignas(64) SolverState {
   int p_zero;           // K1 uses p = r (p = nu = 0)
   int xcur, xopt;       // rotating iterate buffers (0..2)
   int pad[2];
-  unsigned int counters[16];  // last-workgroup tickets, one per kernel site
 };
 
 // Workspace passed to the kernels (all device pointers).

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <string>
 #include <vector>
@@ -111,6 +112,17 @@ namespace {
 int fail(cup3d_poisson* H, const std::string& m) {
   H->err = m;
   return -1;
+}
+
+// Device vectors handed to an entry point: non-null and 16-byte aligned (the
+// kernels load and store them as 16-byte vectors). Checked on the host before
+// any launch.
+int check_vectors(cup3d_poisson* H, std::initializer_list<const void*> ptrs) {
+  for (const void* p : ptrs) {
+    if (!p) return fail(H, "null vector");
+    if ((uintptr_t)p & 15) return fail(H, "device vectors must be 16-byte aligned");
+  }
+  return 0;
 }
 
 SolverState make_state(const cup3d_poisson_config& c) {
@@ -404,11 +416,18 @@ static int warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* 
   *resid = r_prev;
   const KrylovVectors<double>& v = H->v64;
   const double* x0 = v.x[0];
+  // With bMean 1 or 3 getVec zeroes the origin cell of the guess; every
+  // candidate gets the same zero before its residual is taken, so the
+  // weights minimise the residual the solve actually starts from.
+  const long long zero_row = (H->cfg.bmean == 1 || H->cfg.bmean == 3) && H->topo.origin_block >= 0
+                                 ? (long long)H->topo.origin_block * BLEN
+                                 : -1;
   double *c1 = nullptr, *c2 = nullptr, *r1 = nullptr, *r2 = nullptr;
   if (H->cfg.extrapolate && H->hist_valid) {
     c2 = v.z2;
     r2 = v.p;
     launch_extrapolate(s, H->n, x0, H->d_hist.as<double>(), c2);
+    launch_zero_cell(s, c2, zero_row);
     if (resid_into(H, c2, r2) != 0) return -1;
   }
   if (H->cfg.extrapolate) {  // this step's pressure is the next solve's history
@@ -420,6 +439,7 @@ static int warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* 
     c1 = H->cnn.d_pred;
     r1 = v.t;
     if (cnn_predict(H->cnn, s, v.nu, x0, c1, H->err) != 0) return -1;
+    launch_zero_cell(s, c1, zero_row);
     if (resid_into(H, c1, r1) != 0) return -1;
   }
   if (!c1 && !c2) return 0;
@@ -478,8 +498,7 @@ int cup3d_poisson_solve_device(cup3d_poisson_t H, const double* d_rhs, double* d
                                cup3d_solve_report* rep) {
   if (!H) return -1;
   if (H->nb <= 0) return fail(H, "set_topology must be called before solve");
-  if (!d_rhs || !d_x) return fail(H, "null vector");
-  if (((uintptr_t)d_rhs | (uintptr_t)d_x) & 15) return fail(H, "device vectors must be 16-byte aligned");
+  if (check_vectors(H, {d_rhs, d_x}) != 0) return -1;
   HIPCHECK(hipSetDevice(H->device));
   cup3d_solve_report local;
   std::memset(&local, 0, sizeof(local));
@@ -517,8 +536,7 @@ int cup3d_poisson_solve_device(cup3d_poisson_t H, const double* d_rhs, double* d
 int cup3d_poisson_remove_mean(cup3d_poisson_t H, double* d_x, double* mean) {
   if (!H) return -1;
   if (H->nb <= 0) return fail(H, "set_topology must be called first");
-  if (!d_x) return fail(H, "null vector");
-  if ((uintptr_t)d_x & 15) return fail(H, "device vectors must be 16-byte aligned");
+  if (check_vectors(H, {d_x}) != 0) return -1;
   HIPCHECK(hipSetDevice(H->device));
   double vol = 0.0;
   for (double h3 : H->topo.h3) vol += 512.0 * h3;
@@ -580,6 +598,7 @@ int cup3d_poisson_solve(cup3d_poisson_t H, const double* const* rhs_blocks,
 int cup3d_poisson_apply_operator(cup3d_poisson_t H, const double* d_x, double* d_y) {
   if (!H) return -1;
   if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  if (check_vectors(H, {d_x, d_y}) != 0) return -1;
   HIPCHECK(hipSetDevice(H->device));
   SolverState init = make_state(H->cfg);
   *H->h_state = init;
@@ -595,6 +614,7 @@ int cup3d_poisson_apply_operator(cup3d_poisson_t H, const double* d_x, double* d
 int cup3d_poisson_apply_preconditioner(cup3d_poisson_t H, const double* d_p, double* d_z) {
   if (!H) return -1;
   if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  if (check_vectors(H, {d_p, d_z}) != 0) return -1;
   HIPCHECK(hipSetDevice(H->device));
   if (H->cfg.precision == CUP3D_PREC_FP64) {
     KrylovVectors<double> v = H->v64;
@@ -629,7 +649,7 @@ int cup3d_poisson_cnn_predict(cup3d_poisson_t H, const double* d_rhs, const doub
                               double* d_out) {
   if (!H) return -1;
   if (H->nb <= 0) return fail(H, "set_topology must be called first");
-  if (!d_rhs || !d_prev || !d_out) return fail(H, "null vector");
+  if (check_vectors(H, {d_rhs, d_prev, d_out}) != 0) return -1;
   HIPCHECK(hipSetDevice(H->device));
   if (increment_residual(H, d_rhs, d_prev) < 0) return -1;
   return cnn_predict(H->cnn, H->stream, H->v64.nu, H->v64.x[0], d_out, H->err);
@@ -639,7 +659,7 @@ int cup3d_poisson_cnn_add_sample(cup3d_poisson_t H, const double* d_rhs, const d
                                  const double* d_p) {
   if (!H) return -1;
   if (H->nb <= 0) return fail(H, "set_topology must be called first");
-  if (!d_rhs || !d_prev || !d_p) return fail(H, "null vector");
+  if (check_vectors(H, {d_rhs, d_prev, d_p}) != 0) return -1;
   HIPCHECK(hipSetDevice(H->device));
   if (increment_residual(H, d_rhs, d_prev) < 0) return -1;
   return cnn_add_sample(H->cnn, H->stream, H->v64.nu, H->v64.x[0], d_p, H->err);
@@ -677,8 +697,9 @@ int cup3d_debug_cnn_gradient(cup3d_poisson_t H, int k, float res_weight, float* 
 }
 
 int cup3d_debug_apply_operator32(cup3d_poisson_t H, const double* d_x, double* d_y) {
-  if (!H || !d_x || !d_y) return -1;
+  if (!H) return -1;
   if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  if (check_vectors(H, {d_x, d_y}) != 0) return -1;
   HIPCHECK(hipSetDevice(H->device));
   if (mixed_apply_operator32(H->mixed, H->stream, H->g32, H->d_state.as<SolverState>(),
                              H->d_part.as<double>(), d_x, d_y, H->err) != 0)
@@ -802,7 +823,7 @@ int cup3d_debug_profile(cup3d_poisson_t H, int enable) {
 int cup3d_debug_profile_read(cup3d_poisson_t H, double* avg_us, long long* samples) {
   if (!H || !avg_us || !samples) return -1;
   MixedWorkspace& M = H->mixed;
-  for (int k = 0; k < 3; ++k) avg_us[k] = M.prof_n ? 1e3 * M.prof_ms[k] / M.prof_n : 0.0;
+  for (int k = 0; k < 5; ++k) avg_us[k] = M.prof_n ? 1e3 * M.prof_ms[k] / M.prof_n : 0.0;
   *samples = M.prof_n;
   for (double& m : M.prof_ms) m = 0.0;
   M.prof_n = 0;

@@ -11,11 +11,9 @@
 #include <string>
 #include <vector>
 
-#include "PoissonSolverBase.h"
-#include "PoissonSolverHIP.h"
-#include "SimulationData.h"
+#include "cup3d_poisson_standalone.hpp"
 
-using namespace cubismup3d;
+using namespace cup3d_hip::standalone;
 
 int main(int argc, char** argv) {
   const std::string solver = argc > 1 ? argv[1] : "hip_iterative";
@@ -27,13 +25,13 @@ int main(int argc, char** argv) {
   sim.bMeanConstraint = 1;
   sim.BCx_flag = sim.BCy_flag = sim.BCz_flag = periodic;
   sim.bpdx = sim.bpdy = sim.bpdz = bp;
-  sim.h0 = 1.0 / (8 * bp);
+  sim.hmax = 1.0 / (8 * bp);
   ScalarGrid lhs, pres;
   sim.lhs = &lhs;
   sim.pres = &pres;
   const int nb = bp * bp * bp;
   std::vector<ScalarBlock> rb(nb), pb(nb);
-  const double h = sim.h0, k[3] = {2 * M_PI * 1, 2 * M_PI * 2, 2 * M_PI * 3};
+  const double h = sim.hmax, k[3] = {2 * M_PI * 1, 2 * M_PI * 2, 2 * M_PI * 3};
   double lam = 0;
   for (int d = 0; d < 3; ++d) lam += 2 * std::cos(k[d] * h) - 2;
   int id = 0;

@@ -37,7 +37,7 @@ import numpy as np
 
 def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, batch=256,
         lr=1e-3, res_weight=100.0, seed=1, tol=1e-7, log=sys.stderr, workload="c4",
-        fp64_max_iter=3000):
+        fp64_max_iter=3000, forms=("fp64", "mixed_fp16")):
     import torch
 
     from . import workloads as W
@@ -87,7 +87,7 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, bat
 
     def judge(rhs_, sols_, ts):
         out = {}
-        for form in ("fp64", "mixed_fp16"):
+        for form in forms:
             kf = dict(kw, max_iter=fp64_max_iter) if form == "fp64" else kw
             base = solver_for(w, precision=form, guess="given", **kf)
             net = solver_for(w, precision=form, guess="cnn", **kf)

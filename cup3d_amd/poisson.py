@@ -73,6 +73,7 @@ def lib():
     L.cup3d_poisson_solve_device.argtypes = [_vp, _vp, _vp, ctypes.POINTER(Report)]
     L.cup3d_poisson_apply_operator.argtypes = [_vp, _vp, _vp]
     L.cup3d_poisson_apply_preconditioner.argtypes = [_vp, _vp, _vp]
+    L.cup3d_debug_apply_operator32.argtypes = [_vp, _vp, _vp]
     L.cup3d_cnn_num_weights.argtypes = []
     L.cup3d_poisson_cnn_set_weights.argtypes = [_vp, _vp, ctypes.c_int]
     L.cup3d_poisson_cnn_get_weights.argtypes = [_vp, _vp, ctypes.c_int]

@@ -4,7 +4,7 @@
  *
  * Drop-in boundary. The reference selects its solver with
  *   std::shared_ptr<PoissonSolverBase> makePoissonSolver(SimulationData&)
- *     (source/poisson/PoissonSolverBase.cpp:30-52)
+ *     (source/poisson/PoissonSolverBase.cpp:10-32)
  * and calls the argument-less
  *   virtual void PoissonSolverBase::solve()
  *     (source/poisson/PoissonSolverBase.h:10-17)
@@ -14,7 +14,7 @@
  * (PoissonSolverExp.cpp:375-418, 452-472). The functions below carry exactly
  * that data across a plain C boundary: no C++ types, no exceptions, plain
  * pointers and sizes. A C++ adapter implementing PoissonSolverBase on top of
- * them is cup3d_amd/host/PoissonSolverHIP.{h,cpp}.
+ * them is cup3d_amd/host/cup3d_poisson_adapter.hpp.
  *
  * Layout. A block is 8x8x8 doubles stored [iz][iy][ix] (Cubism ScalarBlock,
  * source/Definitions.h:524-531). Blocks are numbered by the caller (Cubism's

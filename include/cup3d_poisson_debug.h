@@ -45,13 +45,15 @@ int cup3d_debug_time_kernel(cup3d_poisson_t h, int which, int reps, double* avg_
                             double* bytes, double* flops);
 
 /* In-solve kernel timing of the mixed-precision form. While enabled, every
- * solve on the handle records HIP events on its stream around K1 (p update +
- * GEMM), K3 (s update + GEMM) and K5 (x/r update) of the middle iteration
- * of each polled batch of iterations (one batch = 8 iterations by default),
- * counted when the batch ran past it.
+ * solve on the handle records HIP events on its stream around the main kernel
+ * of K1 (p update + GEMM), K2 (SpMV + (rhat, nu)), K3 (s update + GEMM), K4
+ * (SpMV + (t, s), (t, t)) and K5 (x/r update), their one-workgroup finalize
+ * kernels excluded, in the middle iteration of each polled batch of
+ * iterations (one batch = 8 iterations by default), counted when the batch
+ * ran past it.
  * cup3d_debug_profile(h, 1) enables and clears, (h, 0) disables;
  * cup3d_debug_profile_read returns the average durations in microseconds
- * (avg_us[3]: K1, K3, K5) and the number of samples, and clears them.
+ * (avg_us[5]: K1 .. K5) and the number of samples, and clears them.
  * The FP64 form returns < 0. */
 int cup3d_debug_profile(cup3d_poisson_t h, int enable);
 /* Device-to-device streaming copy rate in bytes/s (read + written) of a

@@ -5,6 +5,7 @@ PoissonSolverExp::getMat (two independent implementations)."""
 import ctypes
 import glob
 import os
+import subprocess
 import re
 
 import numpy as np
@@ -139,9 +140,22 @@ def test_topology_rejects_unbalanced_grid():
         debug_csr(blocks, (2, 1, 1), 1 / 16, (2, 2, 2), 0)
 
 
-def test_adapter_library_links():
-    """The C++ adapter resolves every C ABI symbol it uses."""
-    L = ctypes.CDLL(os.path.join(ROOT, "cup3d_amd", "lib", "libcup3d_adapter.so"))
-    assert L is not None
+def test_adapter_binds_into_reference_factory(tmp_path):
+    """The header-only adapter compiles inside a translation unit that has
+    declared the reference's PoissonSolverBase, SimulationData, BCflag and
+    BlockInfo first (shaped like source/poisson/PoissonSolverBase.cpp), and
+    the result links against libcup3d_poisson.so with no undefined symbol."""
+    out = tmp_path / "ref_tu.so"
+    r = subprocess.run(
+        ["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fPIC", "-shared",
+         "-Wl,--no-undefined", "-I", os.path.join(ROOT, "include"),
+         "-I", os.path.join(ROOT, "cup3d_amd", "host"),
+         os.path.join(ROOT, "tests", "adapter_tu", "reference_factory.cpp"), "-o", str(out),
+         "-L", os.path.join(ROOT, "cup3d_amd", "lib"), "-lcup3d_poisson",
+         "-Wl,-rpath," + os.path.join(ROOT, "cup3d_amd", "lib")],
+        capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    L = ctypes.CDLL(str(out))
+    assert L.reference_style_projection is not None
     exe = os.path.join(ROOT, "cup3d_amd", "lib", "example_projection")
     assert os.access(exe, os.X_OK)

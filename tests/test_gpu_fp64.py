@@ -267,6 +267,19 @@ def test_host_checked_errors(gpu):
     assert L.cup3d_poisson_solve_device(S.h, None, x.data_ptr(), None) < 0
     assert L.cup3d_poisson_solve_device(S.h, b.data_ptr() + 8, x.data_ptr(), None) < 0
     assert b"aligned" in L.cup3d_poisson_last_error(S.h)
+    # every entry point that hands device vectors to a kernel checks them first
+    for fn in (L.cup3d_poisson_apply_operator, L.cup3d_poisson_apply_preconditioner,
+               L.cup3d_debug_apply_operator32):
+        assert fn(S.h, None, x.data_ptr()) < 0
+        assert b"null" in L.cup3d_poisson_last_error(S.h)
+        assert fn(S.h, b.data_ptr(), x.data_ptr() + 8) < 0
+        assert b"aligned" in L.cup3d_poisson_last_error(S.h)
+    for fn in (L.cup3d_poisson_cnn_predict, L.cup3d_poisson_cnn_add_sample):
+        assert fn(S.h, b.data_ptr(), None, x.data_ptr()) < 0
+        assert b"null" in L.cup3d_poisson_last_error(S.h)
+        assert fn(S.h, b.data_ptr() + 8, x.data_ptr(), x.data_ptr()) < 0
+        assert b"aligned" in L.cup3d_poisson_last_error(S.h)
+    assert L.cup3d_poisson_remove_mean(S.h, None, None) < 0
     with pytest.raises(RuntimeError):
         S.set_topology(np.array([[2, 0, 0, 0], [0, 1, 0, 0]], np.int32))
     assert b"balance" in L.cup3d_poisson_last_error(S.h)
