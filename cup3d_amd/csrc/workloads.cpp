@@ -495,6 +495,24 @@ void cw_rhs_blobs(int nb, const int* blocks, double h0, const int* bpd, const in
   blob_rhs(S, nb, blocks, h0, out);
 }
 
+// The blob parameters cw_rhs_blobs draws for (ext, bc, nblobs, sigma range,
+// seed), 10 doubles per blob: centre[3] (at step 0), direction[3], sigma,
+// amplitude[3]. For checking the rhs against the analytic divergence.
+void cw_blob_params(const double* ext, const int* bc, int nblobs, double smin, double smax,
+                    uint64_t seed, double* out) {
+  const Scene S = make_blob_scene(ext, bc, nblobs, smin, smax, seed);
+  for (int i = 0; i < nblobs; ++i) {
+    const Blob& b = S.blobs[i];
+    double* o = out + 10 * (size_t)i;
+    for (int a = 0; a < 3; ++a) {
+      o[a] = b.c[a];
+      o[3 + a] = b.d[a];
+      o[7 + a] = b.A[a];
+    }
+    o[6] = b.sigma;
+  }
+}
+
 // chi (1 inside obstacles, 0 outside) at cell centres, for diagnostics.
 void cw_chi(int nb, const int* blocks, double h0, const int* bpd, const double* sphere_c,
             double sr, int n_ell, const double* ell_axes, uint64_t ell_seed, double* out) {

@@ -50,6 +50,9 @@ def _lib():
                                      ctypes.c_uint64, ctypes.c_int, ctypes.c_double, _f64p,
                                      ctypes.c_double, ctypes.c_int, _f64p, ctypes.c_uint64,
                                      _f64p]
+        lib.cw_blob_params.restype = None
+        lib.cw_blob_params.argtypes = [_f64p, _i32p, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_uint64, _f64p]
         lib.cw_chi.restype = None
         lib.cw_chi.argtypes = [ctypes.c_int, _i32p, ctypes.c_double, _i32p, _f64p,
                                ctypes.c_double, ctypes.c_int, _f64p, ctypes.c_uint64, _f64p]
@@ -120,6 +123,18 @@ def rhs_blobs(blocks, h0, bpd, bc, nblobs, sigma, seed=SEED, step=0, speed=0.005
                         _p(bc, ctypes.c_int), nblobs, smin, smax, seed, step, speed,
                         _p(sc, ctypes.c_double), sr, n_ell, _p(ax, ctypes.c_double), eseed,
                         _p(out, ctypes.c_double))
+    return out
+
+
+def blob_params(bpd, h0, bc, nblobs, sigma, seed=SEED):
+    """The blobs rhs_blobs draws: (nblobs, 10) rows of centre[3] at step 0,
+    unit direction[3], sigma, amplitude[3]."""
+    ext = np.ascontiguousarray([8 * bpd[a] * h0 for a in range(3)], np.float64)
+    bc = np.ascontiguousarray(bc, np.int32)
+    smin, smax = (sigma, sigma) if np.isscalar(sigma) else sigma
+    out = np.zeros((nblobs, 10), np.float64)
+    _lib().cw_blob_params(_p(ext, ctypes.c_double), _p(bc, ctypes.c_int), nblobs, smin, smax,
+                          seed, _p(out, ctypes.c_double))
     return out
 
 

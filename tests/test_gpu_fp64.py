@@ -84,7 +84,7 @@ def test_preconditioner_matches_oracle(gpu, name):
 
 
 @pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "uniform_periodic_2x3x2", "amr2_walls",
-                                  "amr3_walls", "tiny_amr_periodic"])
+                                  "amr3_walls", "tiny_amr_periodic", "school4_small"])
 def test_solve_bmean0_same_iterations(gpu, name):
     S, O, blocks, h0 = pair(name, 0, tol_rel=1e-7)
     b = compatible_rhs(blocks, h0, 3)
@@ -100,15 +100,18 @@ def test_solve_bmean0_same_iterations(gpu, name):
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_solve_bmean1_first_iterations(gpu, name):
-    """bMean = 1 amplifies rounding as iterations accumulate (the oracle shows
-    it against itself under 1e-15 rhs perturbations); the first iterates agree."""
-    S, O, blocks, h0 = pair(name, 1, tol_rel=0.0, max_iter=6)
+    """bMean = 1 amplifies rounding as iterations accumulate (the oracle
+    against itself under 1e-15 rhs perturbations stays within 1e-11 up to
+    iteration 12 and drifts past 1e-8 by iteration 20,
+    test_oracle_solver.py::test_bmean1_recurrence_amplifies_rounding); the
+    first 12 iterates agree."""
+    S, O, blocks, h0 = pair(name, 1, tol_rel=0.0, max_iter=12)
     b = compatible_rhs(blocks, h0, 4)
     x0 = random_field(len(blocks), 5).reshape(-1) * 1e-3
     x = dev(x0)
     rep = S.solve_device(dev(b), x)
-    xo, info = O.bicgstab(b, x0=x0, tol_rel=0.0, max_iter=6)
-    assert rep["status"] == info["status"] == 1 and rep["iterations"] == info["iters"] == 6
+    xo, info = O.bicgstab(b, x0=x0, tol_rel=0.0, max_iter=12)
+    assert rep["status"] == info["status"] == 1 and rep["iterations"] == info["iters"] == 12
     xg = x.cpu().numpy()
     assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= 1e-10
     assert abs(rep["err_final"] - info["err_final"]) <= 1e-10 * info["err_final"]

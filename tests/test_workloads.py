@@ -1,0 +1,116 @@
+"""The synthetic right-hand sides every benchmark and test input rests on,
+checked against what the mathematics fixes:
+
+* the Fourier rhs is a sum of at most 16 cosine modes with integer
+  wavenumbers in {1..4}^3, volume-weighted by h^3, with zero mean; its
+  discrete inverse through the periodic Laplacian's eigenvalues is a solution
+  of the oracle's operator;
+* the blob rhs is h^3 times the divergence of the Gaussian-blob velocity,
+  taken as the central difference of PressureProjection.cpp:114-141
+  (b = h^2/(2 dt) sum (u+ - u-), dt = 1), so it approaches the analytic
+  divergence at O(h^2) under refinement, blob centres moving with the step;
+* penalisation zeroes the velocity inside the obstacle: a cell whose six
+  neighbour centres lie inside the sphere has b equal to the mean shift only,
+  and cells away from it are unchanged up to that shift.
+"""
+import numpy as np
+import pyoracle
+import pytest
+
+from cup3d_amd import workloads as W
+from helpers import WALL, blocks_to_global, cell_centres, global_to_blocks, uniform
+
+
+def test_fourier_rhs_spectrum():
+    w = W.c1_periodic_32()
+    n = 32
+    for seed in (W.SEED, W.SEED + 7):
+        b = w.rhs(seed=seed)
+        h = w.h0
+        g = blocks_to_global(b, w.blocks, w.bpd) / h ** 3
+        assert abs(g.sum()) <= 1e-12 * np.abs(g).sum()
+        F = np.fft.fftn(g)  # axes z, y, x
+        peaks = np.argwhere(np.abs(F) > 1e-9 * np.abs(F).max())
+        ks = set()
+        for kz, ky, kx in peaks:
+            k = np.array([kx, ky, kz])
+            k = np.where(k > n // 2, k - n, k)
+            # a cosine mode cos(2 pi k.x + phi) sits at +k and -k only: all
+            # components of one sign, each of magnitude 1..4
+            assert (np.all(k >= 1) and np.all(k <= 4)) or (np.all(k <= -1) and np.all(k >= -4)), k
+            ks.add(tuple(np.abs(k)))
+        assert len(peaks) == 2 * len(ks)
+        assert 8 <= len(ks) <= 16
+
+
+def test_fourier_rhs_discrete_inverse():
+    """x = IFFT(b_hat / lambda(k)) with the eigenvalues lambda(k) = h sum_d
+    (2 cos(2 pi k_d / n) - 2) of the periodic operator solves the oracle's
+    system A x = b (bMean = 0): the rhs is compatible and purely modal."""
+    w = W.c1_periodic_32()
+    b = w.rhs()
+    g = blocks_to_global(b, w.blocks, w.bpd)
+    n = g.shape
+    lam = sum(np.reshape(2 * np.cos(2 * np.pi * np.arange(n[a]) / n[a]) - 2,
+                         [-1 if d == a else 1 for d in range(3)]) for a in range(3)) * w.h0
+    lam[0, 0, 0] = 1.0
+    xh = np.fft.fftn(g) / lam
+    assert abs(xh[0, 0, 0]) <= 1e-12 * np.abs(xh).max()
+    xh[0, 0, 0] = 0.0
+    x = global_to_blocks(np.real(np.fft.ifftn(xh)), w.blocks).reshape(-1)
+    O = pyoracle.Oracle(w.blocks, w.bpd, w.h0, w.bc, 0)
+    assert np.abs(O.spmv(x) - b.reshape(-1)).max() <= 1e-12 * np.abs(b).max()
+
+
+def analytic_divergence(P, X, step, speed):
+    """div u at points X [..., 3] of u = sum_b A_b exp(-|x - c_b|^2 / (2 s_b^2))."""
+    div = np.zeros(X.shape[:-1])
+    for row in P:
+        c = row[0:3] + step * speed * row[3:6]
+        s, A = row[6], row[7:10]
+        d = X - c
+        gss = np.exp(-(d * d).sum(-1) / (2 * s * s))
+        div += -gss * (d * A).sum(-1) / (s * s)
+    return div
+
+
+@pytest.mark.parametrize("step", [0, 3])
+def test_blob_rhs_is_divergence_at_second_order(step):
+    sigma, nblobs, speed = 0.09, 3, 0.005
+    errs = []
+    for bp in (4, 8):
+        blocks, bpd, h0, bc = uniform((bp, bp, bp), (WALL,) * 3)
+        P = W.blob_params(bpd, h0, bc, nblobs, sigma)
+        b = W.rhs_blobs(blocks, h0, bpd, bc, nblobs, sigma, step=step, speed=speed)
+        X, h = cell_centres(blocks, h0)
+        ana = analytic_divergence(P, X, step, speed)
+        # cells whose neighbours all lie inside the domain (wall ghosts are 0)
+        inner = np.all((X > h[0] * 1.01) & (X < 1 - h[0] * 1.01), axis=-1)
+        e = b / h[0] ** 3 - ana  # the removed mean is a constant shift
+        e = e[inner] - e[inner].mean()
+        errs.append(np.abs(e).max() / np.abs(ana[inner]).max())
+    assert errs[0] < 0.08  # (h / sigma)^2 / 6-sized at h = 1/32, sigma = 0.09
+    # halving h divides the central-difference error by 4
+    assert 3.3 < errs[0] / errs[1] < 4.7, errs
+
+
+def test_blob_rhs_penalised_sphere():
+    blocks, bpd, h0, bc = uniform((4, 4, 4), (WALL,) * 3)
+    centre, r = (0.5, 0.5, 0.5), 0.2
+    free = W.rhs_blobs(blocks, h0, bpd, bc, 16, (0.05, 0.1))
+    pen = W.rhs_blobs(blocks, h0, bpd, bc, 16, (0.05, 0.1), sphere=(centre, r))
+    X, h = cell_centres(blocks, h0)
+    dist = np.sqrt(((X - np.array(centre)) ** 2).sum(-1))
+    deep = dist + h[0] * 1.01 < r   # all six neighbour centres inside: u = 0 there
+    away = dist - h[0] * 1.01 > r   # all six outside: the same velocities as without it
+    assert deep.sum() > 100 and away.sum() > 10000
+    assert np.ptp(pen[deep]) <= 1e-15 * np.abs(free).max()
+    assert abs(pen.sum()) <= 1e-12 * np.abs(pen).sum()  # mean removed (uniform h)
+    d = (pen - free)[away]
+    assert np.ptp(d) <= 1e-14 * np.abs(free).max()
+    # chi agrees with the geometry the penalisation used
+    chi = W.chi(blocks, h0, bpd, sphere=(centre, r))
+    assert np.array_equal(chi > 0.5, dist <= r)
+    # the obstacle changes the rhs in a shell around its surface
+    shell = ~deep & ~away
+    assert np.abs((pen - free)[shell] - d.mean()).max() > 1e-3 * np.abs(free).max()
