@@ -511,7 +511,8 @@ def main():
     tol = args.tol
 
     def leg(name, form, **kw):
-        S, res = time_form(w, form, args.bmean, seq, args.warmup, rel, tol=kw.pop("tol", tol), **kw)
+        S, res = time_form(w, form, args.bmean, seq, args.warmup, rel, tol=kw.pop("tol", tol),
+                           log=name, **kw)
         print(f"{name}: {np.mean([r[0] for r in res]):.2f} ms per step", file=sys.stderr,
               flush=True)
         return S, res

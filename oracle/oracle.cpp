@@ -240,14 +240,14 @@ static void Dmixed(const Indexer& I, int b, int ix, int iy, int iz, CV o[4]) {
   }
 }
 
-// interpolate — PoissonSolverExp.cpp:112-155
-static void interpolate(const Grid& g, const Indexer& I, int bc, int ixc, int iyc, int izc,
-                        int bf, long long fclose, long long ffar, double sign, double sds1,
-                        double sds2, Row& row) {
-  const double h = g.h(bf);
-  row[fclose] += sign * h * 2. / 3.;
-  row[ffar] += -sign * h * 1. / 5.;
-  const double tf = sign * h * 8. / 15.;
+// interpolate — PoissonSolverExp.cpp:112-155, with the factor sign * h_fine
+// of the reference passed as `scale` (scale = 1 gives the ghost value's
+// weights themselves).
+static void interp_scaled(const Indexer& I, int bc, int ixc, int iyc, int izc, long long fclose,
+                          long long ffar, double scale, double sds1, double sds2, Row& row) {
+  row[fclose] += scale * 2. / 3.;
+  row[ffar] += -scale * 1. / 5.;
+  const double tf = scale * 8. / 15.;
   row[I.This(bc, ixc, iyc, izc)] += tf;
   CV D[3];
   D1(I, bc, ixc, iyc, izc, 1, D);
@@ -261,6 +261,11 @@ static void interpolate(const Grid& g, const Indexer& I, int bc, int ixc, int iy
   CV Dm[4];
   Dmixed(I, bc, ixc, iyc, izc, Dm);
   for (int i = 0; i < 4; i++) row[Dm[i].first] += sds1 * sds2 * tf * Dm[i].second;
+}
+static void interpolate(const Grid& g, const Indexer& I, int bc, int ixc, int iyc, int izc,
+                        int bf, long long fclose, long long ffar, double sign, double sds1,
+                        double sds2, Row& row) {
+  interp_scaled(I, bc, ixc, iyc, izc, fclose, ffar, sign * g.h(bf), sds1, sds2, row);
 }
 
 // makeFlux — PoissonSolverExp.cpp:157-219
@@ -299,6 +304,179 @@ static void makeFlux(const Grid& g, int b, int ix, int iy, int iz, int face, Row
     const long long ff = I.neiFine(fine, q, ix, iy, iz, 1);
     row[fc] += h;
     interpolate(g, I, b, ix, iy, iz, fine, fc, ff, -1., sgn[q][0], sgn[q][1], row);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The caller around the solve: KernelPressureRHS (PressureProjection.cpp:
+// 96-199) and KernelGradP with the velocity update (:244-323, 449-466),
+// including Cubism's flux correction at coarse-fine faces.
+//
+// Lab (ghost) values one cell outside a block, per face kind:
+//   same level      the neighbour's cell (periodic wrap included);
+//   coarser         the fine ghost of the operator's interface interpolation
+//                   2/3 f_close - 1/5 f_far + 8/15 T(f_coarse) (interpolate
+//                   above, scale 1): exact for quadratics (Cubism's BlockLab
+//                   interpolation is absent from the reference mount);
+//   domain boundary velocity (BlockLabBC, Definitions.h:284-435): the normal
+//                   component is mirrored, ghost = -u (wall and freespace);
+//                   pressure (BlockLabNeumann3D): ghost = p (zero gradient);
+//   finer           the coarse cell's face term is replaced by the sum of the
+//                   four fine cells' face terms, each built with its own fine
+//                   ghost (the flux correction: the coarse side carries no
+//                   flux of its own, then receives the fine fluxes).
+// A face term is s fac (f_nb + f_c) for side s = -1 / +1, so the interior
+// formula fac (f(x+h) - f(x-h)) is the sum of a cell's two face terms.
+// ---------------------------------------------------------------------------
+enum FaceClass { FC_INSIDE, FC_SAME, FC_COARSE, FC_FINE, FC_BOUNDARY };
+
+// Class of face `face` of cell (ix,iy,iz) of block b, as makeFlux branches
+// (PoissonSolverExp.cpp:157-219; boundaries as getMat, :286-296, 361).
+static int face_class(const Grid& g, int b, int ix, int iy, int iz, int face, int& nb) {
+  const int c[3] = {ix, iy, iz};
+  const int d = face / 2, s = (face % 2) ? 1 : -1;
+  if (c[d] + s >= 0 && c[d] + s < NX) return FC_INSIDE;
+  const Blk& B = g.blocks[b];
+  const bool boundary = s < 0 ? B.index[d] == 0 : B.index[d] == g.nbp(B.level, d) - 1;
+  if (boundary && g.bc[d] != BC_PERIODIC) return FC_BOUNDARY;
+  int nI[3];
+  g.neighbour(b, face, nI);
+  nb = g.find(B.level, nI);
+  if (nb >= 0) return FC_SAME;
+  const int pI[3] = {nI[0] / 2, nI[1] / 2, nI[2] / 2};
+  nb = B.level > 0 ? g.find(B.level - 1, pI) : -1;
+  if (nb >= 0) return FC_COARSE;
+  return FC_FINE;
+}
+
+// f at cell k of a field with `nc` components, component `comp`.
+static inline double fat(const double* f, int nc, int comp, long long k) {
+  return f[k * nc + comp];
+}
+static double row_apply(const Row& r, const double* f, int nc, int comp) {
+  double v = 0.;
+  for (const auto& kv : r) v += kv.second * fat(f, nc, comp, kv.first);
+  return v;
+}
+
+// Lab value across a same-level or coarser face of cell (b, ix, iy, iz).
+static double ghost(const Grid& g, int b, int ix, int iy, int iz, int face, int cls, int nb,
+                    const double* f, int nc, int comp) {
+  const Indexer I{g, face};
+  if (cls == FC_SAME) return fat(f, nc, comp, I.neiUnif(nb, ix, iy, iz));
+  Row r;  // FC_COARSE: makeFlux's CheckCoarser branch with scale 1
+  interp_scaled(I, nb, I.ic(b, ix, 0), I.ic(b, iy, 1), I.ic(b, iz, 2), I.This(b, ix, iy, iz),
+                I.neiInward(b, ix, iy, iz), 1., I.taylorSign(ix, iy, iz, 1),
+                I.taylorSign(ix, iy, iz, 2), r);
+  return row_apply(r, f, nc, comp);
+}
+
+// Sum over the four fine cells behind a finer face of coarse cell (b, ix,
+// iy, iz) of fac_f (G_q + f_q) - chi_q fac_f (Gs_q + fs_q), in the fine-cell
+// order of makeFlux's CheckFiner branch (:191-217); fs / chi may be null.
+static double fine_faces(const Grid& g, int b, int ix, int iy, int iz, int face, double fac_f,
+                         const double* f, const double* fs, const double* chi, int nc,
+                         int comp) {
+  const Indexer I{g, face};
+  int nI[3], cI[3];
+  g.neighbour(b, face, nI);
+  I.Zchild(nI, ix, iy, iz, cI);
+  const int fine = g.find(g.blocks[b].level + 1, cI);
+  if (fine < 0) throw std::runtime_error("finer face without a finer neighbour");
+  const double sgn[4][2] = {{-1., -1.}, {1., -1.}, {-1., 1.}, {1., 1.}};
+  double acc = 0.;
+  for (int q = 0; q < 4; ++q) {
+    const long long fc = I.neiFine(fine, q, ix, iy, iz, 0);
+    const long long ff = I.neiFine(fine, q, ix, iy, iz, 1);
+    Row r;
+    interp_scaled(I, b, ix, iy, iz, fc, ff, 1., sgn[q][0], sgn[q][1], r);
+    double t = (row_apply(r, f, nc, comp) + fat(f, nc, comp, fc));
+    if (fs && chi) t -= chi[fc] * (row_apply(r, fs, nc, comp) + fat(fs, nc, comp, fc));
+    acc += fac_f * t;
+  }
+  return acc;
+}
+
+// KernelPressureRHS (PressureProjection.cpp:109-199) with dt:
+//   b = fac (uE - uW + vN - vS + wB - wF) - chi fac (the same of udef),
+//   fac = h^2 / (2 dt), plus the flux corrections of finer faces.
+// vel, udef: [n][3]; chi: [n]; udef and chi may be null (no obstacle).
+static void pressure_rhs(const Grid& g, const double* vel, const double* udef,
+                         const double* chi, double dt, double* out) {
+  const int nb = g.nblocks();
+#pragma omp parallel for schedule(static)
+  for (int b = 0; b < nb; ++b) {
+    const double h = g.h(b), fac = 0.5 * h * h / dt;
+    const double hf = 0.5 * h, fac_f = 0.5 * hf * hf / dt;
+    const Indexer G{g, 0};
+    for (int iz = 0; iz < NZ; ++iz)
+      for (int iy = 0; iy < NY; ++iy)
+        for (int ix = 0; ix < NX; ++ix) {
+          const long long k = G.This(b, ix, iy, iz);
+          double U[6], Us[6], corr = 0.;
+          for (int face = 0; face < 6; ++face) {
+            const int d = face / 2, s = (face % 2) ? 1 : -1;
+            int nbk = -1;
+            const int cls = face_class(g, b, ix, iy, iz, face, nbk);
+            if (cls == FC_INSIDE) {
+              const int c[3] = {ix + (d == 0 ? s : 0), iy + (d == 1 ? s : 0), iz + (d == 2 ? s : 0)};
+              const long long kn = G.This(b, c[0], c[1], c[2]);
+              U[face] = fat(vel, 3, d, kn);
+              Us[face] = udef ? fat(udef, 3, d, kn) : 0.;
+            } else if (cls == FC_BOUNDARY || cls == FC_FINE) {
+              U[face] = -fat(vel, 3, d, k);
+              Us[face] = udef ? -fat(udef, 3, d, k) : 0.;
+              if (cls == FC_FINE)
+                corr += s * fine_faces(g, b, ix, iy, iz, face, fac_f, vel, udef, chi, 3, d);
+            } else {
+              U[face] = ghost(g, b, ix, iy, iz, face, cls, nbk, vel, 3, d);
+              Us[face] = udef ? ghost(g, b, ix, iy, iz, face, cls, nbk, udef, 3, d) : 0.;
+            }
+          }
+          double v = fac * (U[1] - U[0] + U[3] - U[2] + U[5] - U[4]);
+          if (udef && chi) v += -chi[k] * fac * (Us[1] - Us[0] + Us[3] - Us[2] + Us[5] - Us[4]);
+          out[k] = v + corr;
+        }
+  }
+}
+
+// KernelGradP (PressureProjection.cpp:244-323) and the velocity update
+// (:449-466): o = -dt h^2 / 2 (p(x+h) - p(x-h)) per component plus the flux
+// corrections of finer faces; vel += o / h^3. vel: [n][3].
+static void grad_p_update(const Grid& g, const double* p, double dt, double* vel) {
+  const int nb = g.nblocks();
+#pragma omp parallel for schedule(static)
+  for (int b = 0; b < nb; ++b) {
+    const double h = g.h(b), fac = -0.5 * dt * h * h;
+    const double hf = 0.5 * h, fac_f = -0.5 * dt * hf * hf;
+    const double inv = 1.0 / (h * h * h);
+    const Indexer G{g, 0};
+    for (int iz = 0; iz < NZ; ++iz)
+      for (int iy = 0; iy < NY; ++iy)
+        for (int ix = 0; ix < NX; ++ix) {
+          const long long k = G.This(b, ix, iy, iz);
+          double P[6], corr[3] = {0., 0., 0.};
+          for (int face = 0; face < 6; ++face) {
+            const int d = face / 2, s = (face % 2) ? 1 : -1;
+            int nbk = -1;
+            const int cls = face_class(g, b, ix, iy, iz, face, nbk);
+            if (cls == FC_INSIDE) {
+              const int c[3] = {ix + (d == 0 ? s : 0), iy + (d == 1 ? s : 0), iz + (d == 2 ? s : 0)};
+              P[face] = p[G.This(b, c[0], c[1], c[2])];
+            } else if (cls == FC_BOUNDARY) {
+              P[face] = p[k];
+            } else if (cls == FC_FINE) {
+              P[face] = -p[k];
+              corr[d] += s * fine_faces(g, b, ix, iy, iz, face, fac_f, p, nullptr, nullptr, 1, 0);
+            } else {
+              P[face] = ghost(g, b, ix, iy, iz, face, cls, nbk, p, 1, 0);
+            }
+          }
+          for (int d = 0; d < 3; ++d) {
+            const double o = fac * (P[2 * d + 1] - P[2 * d]) + corr[d];
+            vel[k * 3 + d] += inv * o;
+          }
+        }
   }
 }
 
@@ -970,6 +1148,16 @@ int orc_pipelined(void* o, const double* rhs, const double* x0, double* x, doubl
   errs[0] = r.err_init;
   errs[1] = r.err_opt;
   return r.status;
+}
+
+// KernelPressureRHS (vel, udef [n][3]; chi [n]; udef/chi may be null).
+void orc_pressure_rhs(void* o, const double* vel, const double* udef, const double* chi,
+                      double dt, double* out) {
+  pressure_rhs(((Oracle*)o)->g, vel, udef, chi, dt, out);
+}
+// KernelGradP + velocity update, vel [n][3] updated in place.
+void orc_grad_p_update(void* o, const double* p, double dt, double* vel) {
+  grad_p_update(((Oracle*)o)->g, p, dt, vel);
 }
 
 int orc_num_threads() {

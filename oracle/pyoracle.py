@@ -46,6 +46,9 @@ def _load(fast: bool = False):
     lib.orc_pipelined.restype = ctypes.c_int
     lib.orc_pipelined.argtypes = [ctypes.c_void_p, _f64p, _f64p, _f64p, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_int, _i32p, _f64p]
+    lib.orc_pressure_rhs.argtypes = [ctypes.c_void_p, _f64p, _f64p, _f64p, ctypes.c_double,
+                                     _f64p]
+    lib.orc_grad_p_update.argtypes = [ctypes.c_void_p, _f64p, ctypes.c_double, _f64p]
     lib.orc_last_error.restype = ctypes.c_char_p
     lib.orc_num_threads.restype = ctypes.c_int
     _libs[name] = lib
@@ -126,6 +129,22 @@ class Oracle:
 
     def lhs_cpu(self, x):
         return self._apply("orc_lhs_cpu", x)
+
+    def pressure_rhs(self, vel, dt, udef=None, chi=None):
+        """KernelPressureRHS: vel, udef [n, 3]; chi [n] (udef, chi: obstacle)."""
+        vel = np.ascontiguousarray(np.asarray(vel, np.float64).reshape(self.n, 3))
+        udef = None if udef is None else np.ascontiguousarray(
+            np.asarray(udef, np.float64).reshape(self.n, 3))
+        chi = None if chi is None else self._vec(chi)
+        out = np.zeros(self.n, np.float64)
+        self.lib.orc_pressure_rhs(self.obj, _p(vel), _p(udef), _p(chi), float(dt), _p(out))
+        return out
+
+    def grad_p_update(self, p, vel, dt):
+        """KernelGradP + velocity update: returns vel + o / h^3, [n, 3]."""
+        v = np.ascontiguousarray(np.asarray(vel, np.float64).reshape(self.n, 3)).copy()
+        self.lib.orc_grad_p_update(self.obj, _p(self._vec(p)), float(dt), _p(v))
+        return v
 
     def bicgstab(self, rhs, x0=None, tol_abs=0.0, tol_rel=1e-7, max_restarts=100,
                  max_iter=1000, history=False):
