@@ -24,6 +24,14 @@ void launch_update64(hipStream_t s, long long n, const KrylovVectors<double>& v,
 void launch_copy_opt64(hipStream_t s, long long n, const KrylovVectors<double>& v,
                        const SolverState* st, double* out);
 
+// The caller around the solve (projection.hip): the rhs of the pressure
+// equation from the velocity (and an obstacle's udef / chi, both or neither
+// null), and the pressure-gradient update of the velocity ([n][3] doubles).
+void launch_pressure_rhs(hipStream_t s, const DevGrid<double>& g, const double* vel,
+                         const double* udef, const double* chi, double dt, double* out);
+void launch_grad_p_update(hipStream_t s, const DevGrid<double>& g, const double* p, double dt,
+                          double* vel);
+
 // x -= the volume-weighted mean of x (h3 per block, inv_vol = 1 / sum 512 h^3);
 // the weighted sum is left in wsum[0].
 void launch_remove_mean(hipStream_t s, long long n, double* x, const double* h3, double inv_vol,

@@ -631,6 +631,60 @@ int cup3d_poisson_apply_preconditioner(cup3d_poisson_t H, const double* d_p, dou
   return 0;
 }
 
+int cup3d_projection_rhs(cup3d_poisson_t H, const double* d_vel, const double* d_udef,
+                         const double* d_chi, double dt, double* d_rhs) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  if (check_vectors(H, {d_vel, d_rhs}) != 0) return -1;
+  if ((d_udef == nullptr) != (d_chi == nullptr))
+    return fail(H, "udef and chi must be given together (or both null)");
+  if (d_udef && check_vectors(H, {d_udef, d_chi}) != 0) return -1;
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(H, "dt must be positive");
+  if (d_rhs == d_vel || d_rhs == d_udef || d_rhs == d_chi)
+    return fail(H, "rhs must not alias the inputs");
+  HIPCHECK(hipSetDevice(H->device));
+  launch_pressure_rhs(H->stream, H->g64, d_vel, d_udef, d_chi, dt, d_rhs);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(H->stream));
+  return 0;
+}
+
+int cup3d_projection_increment(cup3d_poisson_t H, double* d_rhs, const double* d_p_old) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  if (check_vectors(H, {d_rhs, d_p_old}) != 0) return -1;
+  HIPCHECK(hipSetDevice(H->device));
+  // the plain operator (no bMean row): KernelDivPressure
+  DevGrid<double> g = H->g64;
+  g.bmean = 0;
+  g.bmean_row = -1;
+  SolverState init = make_state(H->cfg);
+  *H->h_state = init;
+  HIPCHECK(hipMemcpyAsync(H->d_state.p, H->h_state, sizeof(SolverState), hipMemcpyHostToDevice,
+                          H->stream));
+  double* tmp = H->v64.nu;
+  launch_spmv64(H->stream, SPMV_RESID, g, d_p_old, nullptr, d_rhs, tmp, nullptr,
+                H->d_state.as<SolverState>(), H->d_part.as<double>());
+  HIPCHECK(hipMemcpyAsync(d_rhs, tmp, (size_t)H->n * sizeof(double), hipMemcpyDeviceToDevice,
+                          H->stream));
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(H->stream));
+  return 0;
+}
+
+int cup3d_projection_update_velocity(cup3d_poisson_t H, const double* d_p, double dt,
+                                     double* d_vel) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  if (check_vectors(H, {d_p, d_vel}) != 0) return -1;
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(H, "dt must be positive");
+  HIPCHECK(hipSetDevice(H->device));
+  launch_grad_p_update(H->stream, H->g64, d_p, dt, d_vel);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(H->stream));
+  return 0;
+}
+
 int cup3d_cnn_num_weights(void) { return cnn_num_weights(); }
 
 int cup3d_poisson_cnn_set_weights(cup3d_poisson_t H, const float* w, int nw) {

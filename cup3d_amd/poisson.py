@@ -57,6 +57,7 @@ EXPORTS = [
     "cup3d_poisson_cnn_clear_samples", "cup3d_poisson_cnn_num_samples", "cup3d_poisson_cnn_train",
     "cup3d_poisson_cnn_eval",
     "cup3d_poisson_num_coarse_fine_faces", "cup3d_poisson_remove_mean",
+    "cup3d_projection_rhs", "cup3d_projection_increment", "cup3d_projection_update_velocity",
     "cup3d_poisson_destroy", "cup3d_poisson_last_error",
 ]
 
@@ -74,6 +75,9 @@ def lib():
     L.cup3d_poisson_apply_operator.argtypes = [_vp, _vp, _vp]
     L.cup3d_poisson_apply_preconditioner.argtypes = [_vp, _vp, _vp]
     L.cup3d_debug_apply_operator32.argtypes = [_vp, _vp, _vp]
+    L.cup3d_projection_rhs.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_double, _vp]
+    L.cup3d_projection_increment.argtypes = [_vp, _vp, _vp]
+    L.cup3d_projection_update_velocity.argtypes = [_vp, _vp, ctypes.c_double, _vp]
     L.cup3d_cnn_num_weights.argtypes = []
     L.cup3d_poisson_cnn_set_weights.argtypes = [_vp, _vp, ctypes.c_int]
     L.cup3d_poisson_cnn_get_weights.argtypes = [_vp, _vp, ctypes.c_int]
@@ -210,6 +214,34 @@ class PoissonSolver:
         if self.L.cup3d_poisson_apply_preconditioner(self.h, _ptr(p), _ptr(z)) != 0:
             raise self._err("apply_preconditioner")
         return z
+
+    # the caller around the solve (PressureProjection::operator())
+    def projection_rhs(self, vel, dt, out, udef=None, chi=None):
+        """KernelPressureRHS: out = rhs of the pressure equation from vel
+        ([n, 3] float64 CUDA tensor) and, for an obstacle, udef and chi."""
+        import torch
+        torch.cuda.synchronize(self.device)
+        ud = None if udef is None else _ptr(udef)
+        ch = None if chi is None else _ptr(chi)
+        if self.L.cup3d_projection_rhs(self.h, _ptr(vel), ud, ch, float(dt), _ptr(out)) != 0:
+            raise self._err("projection_rhs")
+        return out
+
+    def projection_increment(self, rhs, p_old):
+        """rhs -= A p_old (the increment form of the solve), in place."""
+        import torch
+        torch.cuda.synchronize(self.device)
+        if self.L.cup3d_projection_increment(self.h, _ptr(rhs), _ptr(p_old)) != 0:
+            raise self._err("projection_increment")
+        return rhs
+
+    def projection_update_velocity(self, p, dt, vel):
+        """KernelGradP + velocity update: vel -= dt grad p, in place."""
+        import torch
+        torch.cuda.synchronize(self.device)
+        if self.L.cup3d_projection_update_velocity(self.h, _ptr(p), float(dt), _ptr(vel)) != 0:
+            raise self._err("projection_update_velocity")
+        return vel
 
     # learned warm start
     def cnn_num_weights(self):

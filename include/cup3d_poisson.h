@@ -156,6 +156,30 @@ int cup3d_poisson_solve_device(cup3d_poisson_t h, const double* d_rhs, double* d
  * returned in *mean when mean is not NULL. */
 int cup3d_poisson_remove_mean(cup3d_poisson_t h, double* d_x, double* mean);
 
+/* The caller around the solve, on the handle's grid and stream
+ * (PressureProjection::operator(), PressureProjection.cpp:333-466). Vector
+ * fields are Cubism VectorBlocks: [nblocks][8][8][8][3] doubles ([iz][iy][ix],
+ * then the component); scalars as the solver's vectors. At coarse-fine faces
+ * the finer side takes its ghosts from the solver's interface interpolation
+ * (exact for quadratics) and the coarser side receives the four fine face
+ * terms (Cubism's flux correction); at non-periodic domain faces the normal
+ * velocity is mirrored (BlockLabBC: walls and freespace) and the pressure
+ * has zero gradient (BlockLabNeumann3D). */
+/* KernelPressureRHS (PressureProjection.cpp:96-199): d_rhs = h^2/(2 dt) div(vel)
+ * - chi h^2/(2 dt) div(udef) by central differences; d_udef and d_chi are an
+ * obstacle's deformation velocity and indicator, both given or both NULL. */
+int cup3d_projection_rhs(cup3d_poisson_t h, const double* d_vel, const double* d_udef,
+                         const double* d_chi, double dt, double* d_rhs);
+/* The increment form of the solve (PressureProjection.cpp:340-362 with
+ * KernelDivPressure :16-94): d_rhs -= A d_p_old with the plain operator (no
+ * bMean row); the caller then solves from a zero guess and adds d_p_old back
+ * after removing the mean. */
+int cup3d_projection_increment(cup3d_poisson_t h, double* d_rhs, const double* d_p_old);
+/* KernelGradP and the velocity update (PressureProjection.cpp:244-323, 449-466):
+ * vel += -(dt h^2 / 2) (p(x+h) - p(x-h)) / h^3 per component, in place. */
+int cup3d_projection_update_velocity(cup3d_poisson_t h, const double* d_p, double dt,
+                                     double* d_vel);
+
 /* y = A x with the solver's operator and its bMean handling: the bMean = 1
  * row, the bMean = 2 term (sum h^3 x) h^3, the bMean = 3 row (device
  * pointers). */

@@ -1,0 +1,127 @@
+"""The caller around the solve on the GPU (cup3d_projection_*: KernelPressureRHS,
+the increment form and KernelGradP with the velocity update,
+PressureProjection.cpp:96-466) against the oracle's restatement, through the
+C ABI, on uniform, periodic-AMR, 3-level and 4-level grids. Bar: 1e-12 of the
+field's magnitude (FP64; the two sides sum the interface interpolation in
+different orders). A whole projection step (rhs, increment, solve, mean
+removal, p += p_old, velocity update) is checked end to end."""
+import numpy as np
+import pyoracle
+import pytest
+import torch
+
+from cup3d_amd.poisson import PoissonSolver
+from helpers import grid_cases, random_field, school4_small, tiny_amr
+
+pytestmark = pytest.mark.gpu
+
+CASES = dict(grid_cases())
+CASES["tiny_amr3_walls"] = tiny_amr(three_levels=True)
+CASES["tiny_amr_periodic"] = tiny_amr(periodic=True)
+CASES["school4_small"] = school4_small()
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+
+
+def pair(name, **kw):
+    blocks, bpd, h0, bc = CASES[name]
+    S = PoissonSolver(bpd, h0, bc, bmean=0, **kw).set_topology(blocks)
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
+    return S, O, blocks
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("obstacle", [False, True])
+def test_rhs_matches_oracle(gpu, name, obstacle):
+    S, O, blocks = pair(name)
+    n = len(blocks) * 512
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal((n, 3))
+    ud = rng.standard_normal((n, 3)) if obstacle else None
+    chi = (rng.random(n) < 0.3).astype(np.float64) if obstacle else None
+    dt = 0.01
+    want = O.pressure_rhs(u, dt, udef=ud, chi=chi)
+    out = torch.zeros(n, dtype=torch.float64, device=gpu)
+    S.projection_rhs(dev(u), dt, out, None if ud is None else dev(ud),
+                     None if chi is None else dev(chi))
+    got = out.cpu().numpy()
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_update_velocity_matches_oracle(gpu, name):
+    S, O, blocks = pair(name)
+    n = len(blocks) * 512
+    rng = np.random.default_rng(12)
+    p = rng.standard_normal(n)
+    v0 = rng.standard_normal((n, 3))
+    dt = 0.02
+    want = O.grad_p_update(p, v0, dt)
+    v = dev(v0)
+    S.projection_update_velocity(dev(p), dt, v)
+    dv_w, dv_g = want - v0, v.cpu().numpy() - v0
+    assert np.abs(dv_g - dv_w).max() <= 1e-12 * np.abs(dv_w).max()
+
+
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "amr3_walls", "school4_small"])
+def test_increment_matches_oracle(gpu, name):
+    S, O, blocks = pair(name)
+    n = len(blocks) * 512
+    b = random_field(len(blocks), 13).reshape(-1)
+    p = random_field(len(blocks), 14).reshape(-1)
+    want = b - O.spmv_raw(p)
+    rhs = dev(b)
+    S.projection_increment(rhs, dev(p))
+    assert np.abs(rhs.cpu().numpy() - want).max() <= 1e-13 * np.abs(want).max()
+
+
+def test_host_checks(gpu):
+    S, O, blocks = pair("uniform_walls_3x2x2")
+    n = len(blocks) * 512
+    u = torch.zeros((n, 3), dtype=torch.float64, device=gpu)
+    out = torch.zeros(n, dtype=torch.float64, device=gpu)
+    with pytest.raises(RuntimeError, match="together"):
+        S.projection_rhs(u, 0.1, out, udef=u)
+    with pytest.raises(RuntimeError, match="dt"):
+        S.projection_rhs(u, 0.0, out)
+    with pytest.raises(RuntimeError, match="dt"):
+        S.projection_update_velocity(out, -1.0, u)
+
+
+@pytest.mark.parametrize("name", ["amr3_walls", "school4_small"])
+def test_projection_step_matches_oracle(gpu, name):
+    """PressureProjection::operator() in the increment form, on the device:
+    rhs from the velocity, rhs -= A p_old, solve from zero, remove the
+    volume-weighted mean, p = phi + p_old, velocity update; the oracle runs
+    the same steps with its classical BiCGSTAB (same iteration count at
+    bMean = 0)."""
+    S, O, blocks = pair(name, tol_rel=1e-10)
+    n = len(blocks) * 512
+    rng = np.random.default_rng(15)
+    u = rng.standard_normal((n, 3))
+    p_old = random_field(len(blocks), 16).reshape(-1) * 1e-3
+    dt = 0.05
+    h3 = np.repeat((O.h) ** 3, 512)
+    # oracle
+    b = O.pressure_rhs(u, dt) - O.spmv_raw(p_old)
+    phi, info = O.bicgstab(b, tol_rel=1e-10)
+    phi -= (h3 @ phi) / h3.sum()
+    p_o = phi + p_old
+    v_o = O.grad_p_update(p_o, u, dt)
+    # device
+    v = dev(u)
+    rhs = torch.zeros(n, dtype=torch.float64, device=gpu)
+    S.projection_rhs(v, dt, rhs)
+    po = dev(p_old)
+    S.projection_increment(rhs, po)
+    x = torch.zeros(n, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(rhs, x)
+    assert rep["status"] == info["status"] == 0 and rep["iterations"] == info["iters"]
+    S.remove_mean(x)
+    x += po
+    S.projection_update_velocity(x, dt, v)
+    assert np.linalg.norm(x.cpu().numpy() - p_o) <= 1e-10 * np.linalg.norm(p_o)
+    dv_o = v_o - u
+    assert np.abs(v.cpu().numpy() - u - dv_o).max() <= 1e-9 * np.abs(dv_o).max()
