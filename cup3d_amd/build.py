@@ -33,8 +33,8 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I", INCLUDE, "-I", CSRC]
 
-POISSON_SOURCES = ["kernels.hip", "mixed.hip", "cnn.hip", "projection.hip", "topology.cpp",
-                   "solver.cpp"]
+POISSON_SOURCES = ["kernels.hip", "mixed.hip", "cnn.hip", "projection.hip", "topology_gpu.hip",
+                   "topology.cpp", "solver.cpp"]
 
 
 def _run(cmd):

@@ -42,6 +42,7 @@ struct CnnState {
   int* d_items = nullptr;
   int items_cap = 0;
   double* d_pred = nullptr;   // prediction buffer for the warm start
+  int cap_nb = 0;             // blocks d_part and d_pred hold
   int nb = 0;
   long long n = 0;
   const int* d_nbr = nullptr;

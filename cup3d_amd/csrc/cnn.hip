@@ -534,13 +534,16 @@ int cnn_set_topology(CnnState& C, int nb, const int* d_nbr, const double* d_h, s
   C.n = (long long)nb * BLEN;
   C.d_nbr = d_nbr;
   C.d_h = d_h;
+  if (nb <= C.cap_nb && C.d_part && C.d_pred) return 0;  // the buffers grow only
   if (C.d_part) (void)hipFree(C.d_part);
   if (C.d_pred) (void)hipFree(C.d_pred);
   C.d_part = nullptr;
   C.d_pred = nullptr;
+  C.cap_nb = 0;
   // partial sums: up to 2 per workgroup of 4 blocks
   CCHECK(hipMalloc(&C.d_part, (2 * ((nb + 3) / 4) + 16) * sizeof(double)));
   CCHECK(hipMalloc(&C.d_pred, C.n * sizeof(double)));
+  C.cap_nb = nb;
   return 0;
 }
 

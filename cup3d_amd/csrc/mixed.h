@@ -20,6 +20,7 @@ struct MixedWorkspace {
   void* d_vec32 = nullptr; // FP32 Krylov vectors
   void* d_res64 = nullptr; // FP64 refinement residual
   long long n = 0;
+  long long cap_n = 0;     // cells the vector buffers hold
   int nb = 0;
   int timing_precision = 1;
   KrylovVectors<float> v32{};

@@ -533,13 +533,18 @@ int mixed_init_pinv(MixedWorkspace& M, const std::vector<double>& P, std::string
 }
 
 int mixed_set_topology(MixedWorkspace& M, int nb, long long n, std::string& err) {
-  if (M.d_vec32) (void)hipFree(M.d_vec32);
-  if (M.d_res64) (void)hipFree(M.d_res64);
-  M.d_vec32 = M.d_res64 = nullptr;
+  // the buffers grow only: a mesh change that does not add cells keeps them
+  if (n > M.cap_n || !M.d_vec32 || !M.d_res64) {
+    if (M.d_vec32) (void)hipFree(M.d_vec32);
+    if (M.d_res64) (void)hipFree(M.d_res64);
+    M.d_vec32 = M.d_res64 = nullptr;
+    M.cap_n = 0;
+    MCHECK(hipMalloc(&M.d_vec32, (size_t)11 * n * sizeof(float)));
+    MCHECK(hipMalloc(&M.d_res64, (size_t)2 * n * sizeof(double)));
+    M.cap_n = n;
+  }
   M.nb = nb;
   M.n = n;
-  MCHECK(hipMalloc(&M.d_vec32, (size_t)11 * n * sizeof(float)));
-  MCHECK(hipMalloc(&M.d_res64, (size_t)2 * n * sizeof(double)));
   float* base = static_cast<float*>(M.d_vec32);
   KrylovVectors<float>& v = M.v32;
   for (int k = 0; k < 3; ++k) v.x[k] = base + k * n;

@@ -25,6 +25,7 @@
 #include "kernels.h"
 #include "mixed.h"
 #include "poisson_internal.h"
+#include "topology_gpu.h"
 
 using namespace cup3d;
 
@@ -46,6 +47,12 @@ struct DevBuf {
     bytes = b;
     return hipMalloc(&p, b ? b : 1);
   }
+  // at least b bytes; keeps the allocation when it is large enough (the
+  // contents are not preserved either way)
+  hipError_t reserve(size_t b) {
+    if (p && bytes >= b) return hipSuccess;
+    return alloc(b);
+  }
   template <typename T>
   T* as() const {
     return static_cast<T*>(p);
@@ -63,6 +70,11 @@ struct DevBuf {
 
 }  // namespace
 
+// Device topology tables of one generation (topology_gpu.hip).
+struct DevTopo {
+  DevBuf blocks, nbr, fkind, fnb, bpar, h, invh, h3, hf, invhf, keys, vals, info;
+};
+
 struct cup3d_poisson {
   cup3d_poisson_config cfg;
   int device = 0;
@@ -70,12 +82,16 @@ struct cup3d_poisson {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
 
-  // topology
+  // topology: the tables live on the device (built there by the topology
+  // kernels, two generations so that a rejected block list leaves the
+  // current one in place); the host keeps the block list, h^3 per block,
+  // the origin block and the face counts
   HostTopology topo;
   std::vector<int> blocks;  // nb x 4
   int nb = 0;
   long long n = 0;
-  DevBuf d_nbr, d_h, d_invh, d_h3, d_hf, d_invhf, d_fkind, d_fnb, d_bpar, d_blocks;
+  DevTopo tp[2];
+  int tcur = 0;
   DevGrid<double> g64;
   DevGrid<float> g32;
 
@@ -193,7 +209,7 @@ int cup3d_poisson_create(const cup3d_poisson_config* cfg, cup3d_poisson_t* out) 
   *out = nullptr;
   const cup3d_poisson_config& c = *cfg;
   for (int d = 0; d < 3; ++d) {
-    if (c.bpd[d] <= 0) { g_create_error = "bpd must be positive"; return -1; }
+    if (c.bpd[d] <= 0 || c.bpd[d] >= 65536) { g_create_error = "bpd must be in 1..65535"; return -1; }
     if (c.bc[d] < 0 || c.bc[d] > 2) { g_create_error = "bc must be 0 (freespace), 1 (periodic) or 2 (wall)"; return -1; }
   }
   if (!(c.h0 > 0)) { g_create_error = "h0 must be positive"; return -1; }
@@ -242,29 +258,77 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
   if (nblocks <= 0 || !blocks) return fail(H, "empty block list");
   if (nblocks > (1LL << 22)) return fail(H, "too many blocks (limit 4194304)");
   HIPCHECK(hipSetDevice(H->device));
-  // validate into locals: a rejected block list leaves the handle as it was
   std::vector<int> bl(nblocks * 4);
   for (int64_t b = 0; b < nblocks; ++b) {
     bl[4 * b] = blocks[b].level;
     for (int d = 0; d < 3; ++d) bl[4 * b + 1 + d] = blocks[b].index[d];
   }
-  TopologyInput in;
+  // build the tables on the device into the other generation: a rejected
+  // block list leaves the handle as it was
+  const size_t nb = (size_t)nblocks;
+  size_t cap = 1024;
+  while (cap < 2 * nb) cap <<= 1;
+  DevTopo& D = H->tp[1 - H->tcur];
+  if (D.blocks.reserve(nb * 4 * sizeof(int)) || D.nbr.reserve(nb * 6 * sizeof(int)) ||
+      D.fkind.reserve(nb * 6) || D.fnb.reserve(nb * 24 * sizeof(int)) || D.bpar.reserve(nb) ||
+      D.h.reserve(nb * sizeof(double)) || D.invh.reserve(nb * sizeof(double)) ||
+      D.h3.reserve(nb * sizeof(double)) || D.hf.reserve(nb * sizeof(float)) ||
+      D.invhf.reserve(nb * sizeof(float)) || D.keys.reserve(cap * sizeof(unsigned long long)) ||
+      D.vals.reserve(cap * sizeof(int)) || D.info.reserve(TOPO_INFO_WORDS * sizeof(int)))
+    return fail(H, "hipMalloc(topology tables) failed");
+  hipStream_t s = H->stream;
+  HIPCHECK(hipMemcpyAsync(D.blocks.p, bl.data(), nb * 4 * sizeof(int), hipMemcpyHostToDevice, s));
+  TopoParams tpar;
+  tpar.nb = (int)nb;
   for (int d = 0; d < 3; ++d) {
-    in.bpd[d] = H->cfg.bpd[d];
-    in.bc[d] = H->cfg.bc[d];
+    tpar.bpd[d] = H->cfg.bpd[d];
+    tpar.bc[d] = H->cfg.bc[d];
   }
-  in.level_max = H->cfg.level_max;
-  in.h0 = H->cfg.h0;
-  in.bmean = H->cfg.bmean;
-  in.nb = (int)nblocks;
-  in.blocks = bl.data();
-  std::string e;
+  tpar.h0 = H->cfg.h0;
+  tpar.blocks = D.blocks.as<int>();
+  const TopoTable ttab{D.keys.as<unsigned long long>(), D.vals.as<int>(), (unsigned long long)cap - 1};
+  const TopoOut tout{D.nbr.as<int>(),  D.fkind.as<int8_t>(), D.fnb.as<int>(),
+                     D.bpar.as<int8_t>(), D.h.as<double>(),  D.invh.as<double>(),
+                     D.h3.as<double>(), D.hf.as<float>(),  D.invhf.as<float>()};
+  launch_topology_build(s, tpar, ttab, tout, D.info.as<int>());
+  HIPCHECK(hipGetLastError());
+  int info[TOPO_INFO_WORDS];
+  HIPCHECK(hipMemcpyAsync(info, D.info.p, sizeof(info), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  // the first offending block in list order, with the message of the
+  // sequential checks (level/index, domain, index width, duplicate)
+  const int NONE = 0x7fffffff;
+  const int first = std::min(std::min(info[TI_INVALID], info[TI_OUTSIDE]),
+                             std::min(info[TI_TOO_FINE], info[TI_DUPLICATE]));
+  if (first != NONE) {
+    const std::string b = "block " + std::to_string(first);
+    if (first == info[TI_INVALID]) return fail(H, b + ": invalid level/index");
+    if (first == info[TI_OUTSIDE]) return fail(H, b + ": index outside the domain");
+    if (first == info[TI_TOO_FINE])
+      return fail(H, b + ": level too fine (more than 2^20 blocks per direction)");
+    return fail(H, "duplicate " + b);
+  }
+  if (info[TI_BALANCE] != NONE)
+    return fail(H, "block " + std::to_string(info[TI_BALANCE] / 6) + " face " +
+                       std::to_string(info[TI_BALANCE] % 6) +
+                       ": neighbour is neither same-level, coarser nor finer (2:1 balance violated)");
+  if (info[TI_FULL] != NONE) return fail(H, "topology hash table full");
   HostTopology Tn;
-  if (!build_topology(in, Tn, e)) return fail(H, e);
+  Tn.nb = (int)nb;
+  Tn.origin_block = info[TI_ORIGIN];
+  Tn.n_coarse_faces = info[TI_N_COARSE];
+  Tn.n_fine_faces = info[TI_N_FINE];
+  Tn.h3.resize(nb);
+  for (size_t b = 0; b < nb; ++b) {
+    const double h = H->cfg.h0 / double(1 << bl[4 * b]);
+    Tn.h3[b] = h * h * h;
+  }
+  if (H->cfg.bmean == 1 && Tn.origin_block >= 0) Tn.bmean_row = (long long)Tn.origin_block * BLEN;
+  H->tcur = 1 - H->tcur;
   H->blocks.swap(bl);
   H->topo = std::move(Tn);
-  // from here on a failure (device allocation or upload) leaves the handle
-  // without a topology: solves are refused until set_topology succeeds
+  // from here on a failure (device allocation) leaves the handle without a
+  // topology: solves are refused until set_topology succeeds
   struct Commit {
     cup3d_poisson* h;
     bool ok = false;
@@ -273,42 +337,31 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
     }
   } commit{H};
   HostTopology& T = H->topo;
+  const DevTopo& C = H->tp[H->tcur];
   H->nb = (int)nblocks;
   H->n = (long long)nblocks * BLEN;
-  std::vector<float> hf(T.h.begin(), T.h.end()), invhf(T.invh.begin(), T.invh.end());
-  if (upload(H, H->d_nbr, T.nbr.data(), T.nbr.size() * sizeof(int)) ||
-      upload(H, H->d_h, T.h.data(), T.h.size() * sizeof(double)) ||
-      upload(H, H->d_invh, T.invh.data(), T.invh.size() * sizeof(double)) ||
-      upload(H, H->d_h3, T.h3.data(), T.h3.size() * sizeof(double)) ||
-      upload(H, H->d_hf, hf.data(), hf.size() * sizeof(float)) ||
-      upload(H, H->d_invhf, invhf.data(), invhf.size() * sizeof(float)) ||
-      upload(H, H->d_fkind, T.kind.data(), T.kind.size() * sizeof(int8_t)) ||
-      upload(H, H->d_fnb, T.fnb.data(), T.fnb.size() * sizeof(int)) ||
-      upload(H, H->d_bpar, T.bpar.data(), T.bpar.size() * sizeof(int8_t)) ||
-      upload(H, H->d_blocks, H->blocks.data(), H->blocks.size() * sizeof(int)))
-    return -1;
   const long long origin_row = T.origin_block >= 0 ? (long long)T.origin_block * BLEN : -1;
   auto fill = [&](auto& g, const void* h, const void* invh) {
     using G = std::remove_reference_t<decltype(g)>;
     using TT = std::remove_const_t<std::remove_pointer_t<decltype(G::h)>>;
     g.nb = H->nb;
     g.n = H->n;
-    g.nbr = H->d_nbr.as<int>();
+    g.nbr = C.nbr.as<int>();
     g.h = static_cast<const TT*>(h);
     g.invh = static_cast<const TT*>(invh);
-    g.h3 = H->d_h3.as<double>();
-    g.fkind = H->d_fkind.as<int8_t>();
-    g.fnb = H->d_fnb.as<int>();
-    g.bpar = H->d_bpar.as<int8_t>();
+    g.h3 = C.h3.as<double>();
+    g.fkind = C.fkind.as<int8_t>();
+    g.fnb = C.fnb.as<int>();
+    g.bpar = C.bpar.as<int8_t>();
     g.bmean = H->cfg.bmean;
     g.has_iface = T.n_coarse_faces + T.n_fine_faces > 0 ? 1 : 0;
     g.bmean_row = (H->cfg.bmean == 1 || H->cfg.bmean == 3) ? origin_row : -1;
   };
-  fill(H->g64, H->d_h.p, H->d_invh.p);
-  fill(H->g32, H->d_hf.p, H->d_invhf.p);
+  fill(H->g64, C.h.p, C.invh.p);
+  fill(H->g32, C.hf.p, C.invhf.p);
   // FP64 Krylov vectors: x[3], r, rhat, p, nu, t, z1, z2, b
   const size_t n = (size_t)H->n;
-  if (H->d_vec64.alloc(11 * n * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(vectors) failed");
+  if (H->d_vec64.reserve(11 * n * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(vectors) failed");
   double* base = H->d_vec64.as<double>();
   KrylovVectors<double>& v = H->v64;
   for (int k = 0; k < 3; ++k) v.x[k] = base + k * n;
@@ -321,15 +374,15 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
   v.z2 = base + 9 * n;
   v.b = base + 10 * n;
   H->part_len = (size_t)((H->nb + 3) / 4 + 8) * 6;  // up to 5 partials per workgroup
-  if (H->d_part.alloc(H->part_len * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(partials) failed");
+  if (H->d_part.reserve(H->part_len * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(partials) failed");
   H->hist_valid = false;
-  if (H->cfg.extrapolate && H->d_hist.alloc(n * sizeof(double)) != hipSuccess)
+  if (H->cfg.extrapolate && H->d_hist.reserve(n * sizeof(double)) != hipSuccess)
     return fail(H, "hipMalloc(history) failed");
   if (!H->d_gdots.p && H->d_gdots.alloc(8 * sizeof(double)) != hipSuccess)
     return fail(H, "hipMalloc(guess dots) failed");
   HIPCHECK(hipMemset(H->d_state.p, 0, sizeof(SolverState)));
   if (mixed_set_topology(H->mixed, H->nb, H->n, H->err) != 0) return -1;
-  if (cnn_set_topology(H->cnn, H->nb, H->d_nbr.as<int>(), H->d_h.as<double>(), H->err) != 0)
+  if (cnn_set_topology(H->cnn, H->nb, C.nbr.as<int>(), C.h.as<double>(), H->err) != 0)
     return -1;
   HIPCHECK(hipDeviceSynchronize());
   commit.ok = true;
@@ -541,7 +594,7 @@ int cup3d_poisson_remove_mean(cup3d_poisson_t H, double* d_x, double* mean) {
   double vol = 0.0;
   for (double h3 : H->topo.h3) vol += 512.0 * h3;
   const double inv_vol = 1.0 / vol;
-  launch_remove_mean(H->stream, H->n, d_x, H->d_h3.as<double>(), inv_vol,
+  launch_remove_mean(H->stream, H->n, d_x, H->tp[H->tcur].h3.as<double>(), inv_vol,
                      H->d_part.as<double>(), H->d_gdots.as<double>());
   HIPCHECK(hipGetLastError());
   double wsum = 0.0;
@@ -748,6 +801,32 @@ int cup3d_debug_cnn_gradient(cup3d_poisson_t H, int k, float res_weight, float* 
   if (!H || !grad || !loss) return -1;
   HIPCHECK(hipSetDevice(H->device));
   return cnn_gradient(H->cnn, H->stream, k, res_weight, grad, loss, H->err);
+}
+
+int cup3d_debug_device_topology(cup3d_poisson_t H, int32_t* nbr, int8_t* kind, int32_t* fnb,
+                                int8_t* bpar, double* hh, double* invh, double* h3,
+                                int64_t* info) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  HIPCHECK(hipSetDevice(H->device));
+  const DevTopo& C = H->tp[H->tcur];
+  const size_t nb = (size_t)H->nb;
+  auto get = [&](void* dst, const DevBuf& src, size_t bytes) -> hipError_t {
+    return dst ? hipMemcpy(dst, src.p, bytes, hipMemcpyDeviceToHost) : hipSuccess;
+  };
+  HIPCHECK(get(nbr, C.nbr, nb * 6 * sizeof(int32_t)));
+  HIPCHECK(get(kind, C.fkind, nb * 6));
+  HIPCHECK(get(fnb, C.fnb, nb * 24 * sizeof(int32_t)));
+  HIPCHECK(get(bpar, C.bpar, nb));
+  HIPCHECK(get(hh, C.h, nb * sizeof(double)));
+  HIPCHECK(get(invh, C.invh, nb * sizeof(double)));
+  HIPCHECK(get(h3, C.h3, nb * sizeof(double)));
+  if (info) {
+    info[0] = H->topo.origin_block;
+    info[1] = H->topo.n_coarse_faces;
+    info[2] = H->topo.n_fine_faces;
+  }
+  return 0;
 }
 
 int cup3d_debug_apply_operator32(cup3d_poisson_t H, const double* d_x, double* d_y) {

@@ -23,6 +23,7 @@
 // factorisation reproduces all nine cases of Dmixed exactly.
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <map>
 #include <unordered_map>
@@ -92,18 +93,23 @@ bool build_topology(const TopologyInput& in, HostTopology& T, std::string& err) 
       err = "block " + std::to_string(b) + ": invalid level/index";
       return false;
     }
-    for (int d = 0; d < 3; ++d)
+    for (int d = 0; d < 3; ++d) {
       if (B[1 + d] >= (in.bpd[d] << B[0])) {
         err = "block " + std::to_string(b) + ": index outside the domain";
         return false;
       }
+      if ((in.bpd[d] << B[0]) > (1 << 20)) {  // the key holds 20-bit indices
+        err = "block " + std::to_string(b) + ": level too fine (more than 2^20 blocks per direction)";
+        return false;
+      }
+    }
     if (!leaf.emplace(key(B[0], B[1], B[2], B[3]), b).second) {
       err = "duplicate block " + std::to_string(b);
       return false;
     }
   }
   auto find = [&](int l, int i, int j, int k) {
-    if (l < 0) return -1;
+    if (l < 0 || l > 15 || i >= (1 << 20) || j >= (1 << 20) || k >= (1 << 20)) return -1;
     auto it = leaf.find(key(l, i, j, k));
     return it == leaf.end() ? -1 : it->second;
   };
@@ -402,4 +408,53 @@ extern "C" int64_t cup3d_debug_operator_csr(const cup3d_poisson_config* cfg, int
   }
   if (rowptr && cap > 0) rowptr[0] = 0;
   return nnz;
+}
+
+extern "C" int cup3d_debug_host_topology(const cup3d_poisson_config* cfg, int64_t nblocks,
+                                         const cup3d_block_desc* blocks, int32_t* nbr,
+                                         int8_t* kind, int32_t* fnb, int8_t* bpar, double* h,
+                                         double* invh, double* h3, int64_t* info, char* err,
+                                         int errlen) {
+  using namespace cup3d;
+  auto set_err = [&](const std::string& m) {
+    if (err && errlen > 0) {
+      const size_t k = std::min(m.size(), (size_t)errlen - 1);
+      std::memcpy(err, m.data(), k);
+      err[k] = 0;
+    }
+    return -1;
+  };
+  if (!cfg || !blocks || nblocks <= 0 || nblocks > (1LL << 22)) return set_err("bad arguments");
+  std::vector<int> bl(nblocks * 4);
+  for (int64_t b = 0; b < nblocks; ++b) {
+    bl[4 * b] = blocks[b].level;
+    for (int d = 0; d < 3; ++d) bl[4 * b + 1 + d] = blocks[b].index[d];
+  }
+  TopologyInput in;
+  for (int d = 0; d < 3; ++d) {
+    in.bpd[d] = cfg->bpd[d];
+    in.bc[d] = cfg->bc[d];
+  }
+  in.level_max = cfg->level_max;
+  in.h0 = cfg->h0;
+  in.bmean = cfg->bmean;
+  in.nb = (int)nblocks;
+  in.blocks = bl.data();
+  HostTopology T;
+  std::string e;
+  if (!build_topology(in, T, e)) return set_err(e);
+  const size_t nb = (size_t)nblocks;
+  if (nbr) std::memcpy(nbr, T.nbr.data(), nb * 6 * sizeof(int32_t));
+  if (kind) std::memcpy(kind, T.kind.data(), nb * 6);
+  if (fnb) std::memcpy(fnb, T.fnb.data(), nb * 24 * sizeof(int32_t));
+  if (bpar) std::memcpy(bpar, T.bpar.data(), nb);
+  if (h) std::memcpy(h, T.h.data(), nb * sizeof(double));
+  if (invh) std::memcpy(invh, T.invh.data(), nb * sizeof(double));
+  if (h3) std::memcpy(h3, T.h3.data(), nb * sizeof(double));
+  if (info) {
+    info[0] = T.origin_block;
+    info[1] = T.n_coarse_faces;
+    info[2] = T.n_fine_faces;
+  }
+  return 0;
 }

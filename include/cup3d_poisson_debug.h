@@ -69,6 +69,26 @@ int cup3d_debug_profile_read(cup3d_poisson_t h, double* avg_us, long long* sampl
 int cup3d_debug_cnn_gradient(cup3d_poisson_t h, int k, float res_weight, float* grad,
                              double* loss);
 
+/* Topology tables of a block list, as the host build of this library makes
+ * them (the build the explicit-matrix export above uses): same-level
+ * neighbours nbr[nb*6] (-1: none), face kinds kind[nb*6] (0 none, 1 same
+ * level, 2 coarser, 3 finer), partners fnb[nb*24] (coarser block in slot 0,
+ * or the four finer blocks by quadrant), index parities bpar[nb], h, 1/h and
+ * h^3 per block; info[3] = origin block, coarser faces, finer faces. Any
+ * output may be NULL. On a rejected block list returns < 0 and copies the
+ * message into err (errlen bytes, NUL-terminated). Needs no GPU. */
+int cup3d_debug_host_topology(const cup3d_poisson_config* cfg, int64_t nblocks,
+                              const cup3d_block_desc* blocks, int32_t* nbr, int8_t* kind,
+                              int32_t* fnb, int8_t* bpar, double* h, double* invh, double* h3,
+                              int64_t* info, char* err, int errlen);
+
+/* The same tables as the handle's device topology build made them for the
+ * current topology (copied to host arrays of the sizes above; any may be
+ * NULL), for comparing the two builds. Returns 0 or < 0. */
+int cup3d_debug_device_topology(cup3d_poisson_t h, int32_t* nbr, int8_t* kind, int32_t* fnb,
+                                int8_t* bpar, double* hh, double* invh, double* h3,
+                                int64_t* info);
+
 #ifdef __cplusplus
 }
 #endif

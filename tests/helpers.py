@@ -158,3 +158,64 @@ def index_encoded_case(blocks, bpd, h0, bc):
         y += jump * (g[d] == 0) - jump * (g[d] == n[d] - 1)
     return x.reshape(-1), (h0 * y).reshape(-1)
 
+
+def nbr_table(blocks, bpd, bc):
+    """Same-level face neighbours (x-, x+, y-, y+, z-, z+), -1 otherwise."""
+    idx = {tuple(b): i for i, b in enumerate(np.asarray(blocks).tolist())}
+    out = -np.ones((len(blocks), 6), np.int64)
+    for i, (l, a, b, c) in enumerate(np.asarray(blocks).tolist()):
+        I = [a, b, c]
+        for f in range(6):
+            d, s = f // 2, (1 if f % 2 else -1)
+            n = bpd[d] << l
+            J = list(I)
+            J[d] += s
+            if J[d] < 0 or J[d] >= n:
+                if bc[d] != 1:
+                    continue
+                J[d] %= n
+            out[i, f] = idx.get((l, *J), -1)
+    return out
+
+
+def face_tables(blocks, bpd, bc):
+    """Per block face (x-, x+, y-, y+, z-, z+): kind (0 none, 1 same level,
+    2 coarser, 3 finer), the same-level neighbour, and the partners (the
+    coarser block, or the four finer blocks by quadrant q = t1 half + 2 t2
+    half, tangents (y, z), (z, x), (x, y)); the index parities per block.
+    A dictionary walk of the block list, for checking the library's builds."""
+    blocks = np.asarray(blocks).tolist()
+    idx = {tuple(b): i for i, b in enumerate(blocks)}
+    nb = len(blocks)
+    kind = np.zeros((nb, 6), np.int8)
+    nbr = -np.ones((nb, 6), np.int32)
+    fnb = -np.ones((nb, 6, 4), np.int32)
+    bpar = np.zeros(nb, np.int8)
+    tang = {0: (1, 2), 1: (2, 0), 2: (0, 1)}
+    for i, (l, a, b, c) in enumerate(blocks):
+        bpar[i] = (a & 1) | ((b & 1) << 1) | ((c & 1) << 2)
+        for f in range(6):
+            d, s = f // 2, (1 if f % 2 else -1)
+            n = bpd[d] << l
+            J = [a, b, c]
+            J[d] += s
+            if J[d] < 0 or J[d] >= n:
+                if bc[d] != PER:
+                    continue
+                J[d] %= n
+            if (l, *J) in idx:
+                kind[i, f], nbr[i, f] = 1, idx[(l, *J)]
+                continue
+            par = (l - 1, J[0] // 2, J[1] // 2, J[2] // 2)
+            if l > 0 and par in idx:
+                kind[i, f], fnb[i, f, 0] = 2, idx[par]
+                continue
+            t1, t2 = tang[d]
+            for q in range(4):
+                C = [2 * J[0], 2 * J[1], 2 * J[2]]
+                C[d] += 1 if s < 0 else 0
+                C[t1] += q & 1
+                C[t2] += q >> 1
+                fnb[i, f, q] = idx[(l + 1, *C)]
+            kind[i, f] = 3
+    return kind, nbr, fnb.reshape(nb, 24), bpar

@@ -11,32 +11,13 @@ import torch
 
 from cup3d_amd import workloads as W
 from cup3d_amd.poisson import PoissonSolver, lib, solver_for
-from helpers import grid_cases, random_field
+from helpers import grid_cases, nbr_table, random_field
 
 pytestmark = pytest.mark.gpu
 
 
 def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a, np.float64).reshape(-1)).cuda()
-
-
-def nbr_table(blocks, bpd, bc):
-    """Same-level face neighbours (x-, x+, y-, y+, z-, z+), -1 otherwise."""
-    idx = {tuple(b): i for i, b in enumerate(np.asarray(blocks).tolist())}
-    out = -np.ones((len(blocks), 6), np.int64)
-    for i, (l, a, b, c) in enumerate(np.asarray(blocks).tolist()):
-        I = [a, b, c]
-        for f in range(6):
-            d, s = f // 2, (1 if f % 2 else -1)
-            n = bpd[d] << l
-            J = list(I)
-            J[d] += s
-            if J[d] < 0 or J[d] >= n:
-                if bc[d] != 1:
-                    continue
-                J[d] %= n
-            out[i, f] = idx.get((l, *J), -1)
-    return out
 
 
 def smooth_fields(blocks, h0, seed):
