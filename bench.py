@@ -43,6 +43,8 @@ sample of the same workload).
   python bench.py [--steps K] [--warmup W] [--form mixed_fp16|mixed_bf16|fp64]
                   [--workload c1|c2|c3|c4|c5] [--tol 1e-7]
   python bench.py --workload c4        the learned-warm-start protocol (C4)
+  python bench.py --workload diff2     implicit diffusion steps on the uniform grid
+                                       (diff3: the 3-level AMR sphere grid)
   python bench.py --cpu-full           complete CPU-baseline solves on C1, C2, C3
 """
 from __future__ import annotations
@@ -102,7 +104,8 @@ def parse():
     p.add_argument("--steps", type=int, default=16)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--form", default="mixed_fp16", choices=["fp64", "mixed_fp16", "mixed_bf16"])
-    p.add_argument("--workload", default="c5", choices=["c1", "c2", "c3", "c4", "c5"],
+    p.add_argument("--workload", default="c5",
+                   choices=["c1", "c2", "c3", "c4", "c5", "diff2", "diff3"],
                    help="c5: 4-level AMR school grid (default); c2: uniform 256x128x128 walls; "
                         "c3: 3-level AMR sphere; c1: 32^3 periodic; c4: the learned-warm-start "
                         "protocol on the uniform blob sequence")
@@ -126,6 +129,9 @@ def parse():
     p.add_argument("--c4-steps", type=int, default=400)
     p.add_argument("--c4-train", type=int, default=300)
     p.add_argument("--c4-fresh", type=int, default=40)
+    p.add_argument("--nu", type=float, default=1e-3,
+                   help="diffusion workloads: kinematic viscosity")
+    p.add_argument("--dt", type=float, default=0.01, help="diffusion workloads: time step")
     p.add_argument("--kernels-only", action="store_true",
                    help="launch the timed kernels a few times and a 256 MiB copy (for "
                         "hardware-counter runs), print their names, and exit")
@@ -454,6 +460,107 @@ def bench_c4(args):
     print(json.dumps(out))
 
 
+def diffusion_kernel_times(S, nu, dt, reps=20):
+    from cup3d_amd.poisson import lib
+    L = lib()
+    f = L.cup3d_debug_time_diffusion_kernel
+    f.argtypes = ([ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+                  + [ctypes.POINTER(ctypes.c_double)] * 3)
+    out = {}
+    for which, name in ((0, "helmholtz_spmv"), (1, "dst_precond_update_K1"),
+                        (2, "dst_precond")):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        if f(S.h, which, nu, dt, reps, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)) != 0:
+            raise RuntimeError(L.cup3d_poisson_last_error(S.h).decode())
+        sec = a.value * 1e-3
+        out[name] = {"avg_us": round(a.value * 1e3, 2), "alg_bytes_per_launch": b.value,
+                     "GB/s": round(b.value / sec / 1e9, 1),
+                     "frac": round(b.value / sec / 1e9 / HBM_PEAK, 4)}
+    return out
+
+
+def bench_diffusion(args):
+    """Implicit diffusion (AdvectionDiffusionImplicit::euler after its
+    advection kernel): one step = the rhs of the three velocity components,
+    three Helmholtz solves (FP64, DST block-inverse preconditioner) to the
+    reference's default tolerances (||r|| < 1e-6 or ||r||/||r0|| < 1e-4) and
+    the update u += x, on [n][3] velocities resident in HBM. u^n and u* are
+    the blob velocity of two consecutive steps of the moving-blob sequence;
+    every timed step starts from the same u*."""
+    import torch
+    from cup3d_amd import workloads as W
+    from cup3d_amd.poisson import solver_for
+    dev = torch.device("cuda:0")
+    w = W.c2_uniform_walls() if args.workload == "diff2" else W.c3_amr_sphere()
+    vb = dict(sphere=W.SPHERE)
+    u_old = torch.from_numpy(W.vel_blobs(w.blocks, w.h0, w.bpd, w.bc, 256, (0.02, 0.08),
+                                         **vb)).to(dev)
+    u_star = torch.from_numpy(W.vel_blobs(w.blocks, w.h0, w.bpd, w.bc, 256, (0.02, 0.08),
+                                          step=1, **vb)).to(dev)
+    S = solver_for(w, bmean=0)
+    v = torch.empty_like(u_star)
+    tol_abs, tol_rel = 1e-6, 1e-4  # SimulationData.cpp:92-93
+    ms, its, reps_all = [], [], []
+    for k in range(args.warmup + args.steps):
+        v.copy_(u_star)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = S.diffusion_step(u_old, v, args.nu, args.dt, tol_abs, tol_rel)
+        dt_ = time.perf_counter() - t0
+        if k >= args.warmup:
+            ms.append(dt_ * 1e3)
+            its.append([r["iterations"] for r in reps])
+            reps_all.append(reps)
+    v_gpu = v.cpu().numpy()
+    kern = diffusion_kernel_times(S, args.nu, args.dt)
+    dom = max(kern, key=lambda k: kern[k]["avg_us"] if k != "dst_precond" else 0.0)
+    kd = kern[dom]
+    N = float(w.ncells)
+    msm = float(np.mean(ms))
+    it_sum = float(np.mean([sum(i) for i in its]))
+    out = {
+        "metric": "implicit diffusion steps/s (3 Helmholtz solves per step, FP64)",
+        "value": round(1e3 / msm, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(msm, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{w.name}_implicit_diffusion", "cells": w.ncells,
+                   "blocks": int(len(w.blocks)), "nu": args.nu, "dt": args.dt,
+                   "h2_over_nu_dt_finest": float((w.h0 / 2 ** w.level_max) ** 2 /
+                                                 (args.nu * args.dt)),
+                   "tolerance": "||r|| < 1e-6 or ||r||/||r0|| < 1e-4 (reference defaults)",
+                   "parallelism": "single GPU"},
+        "iterations_per_component": its[-1],
+        "cell_iterations_per_s": round(N * it_sum / (msm * 1e-3), 1),
+        "solve_ms_per_component": [round(r["solve_ms"], 3) for r in reps_all[-1]],
+        "roofline": {"bound": "hbm", "achieved": kd["GB/s"], "peak": HBM_PEAK, "unit": "GB/s",
+                     "frac": kd["frac"], "traffic": None, "kernel": dom,
+                     "avg_us": kd["avg_us"], "alg_bytes_per_launch": kd["alg_bytes_per_launch"],
+                     "timing": "HIP events around back-to-back launches", "kernels": kern},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        O = pyoracle.Oracle(w.blocks, w.bpd, w.h0, w.bc, 0, fast=True)
+        uo, us = u_old.cpu().numpy(), u_star.cpu().numpy()
+        times = []
+        for k in range(3):
+            t0 = time.perf_counter()
+            v_cpu, info = O.diffusion_step(uo, us, args.nu, args.dt, tol_abs, tol_rel)
+            times.append(time.perf_counter() - t0)
+        dcpu = float(np.median(times))
+        out["cpu_baseline"] = {
+            "value": round(1.0 / dcpu, 5), "unit": "steps/s",
+            "cores": pyoracle.num_threads(fast=True), "kind": "port",
+            "physical_cores_available": physical_cores(),
+            "sample": (f"the whole step on the same {w.ncells}-cell inputs by the restated "
+                       f"reference algorithm (pipelined BiCGSTAB + per-block CG, "
+                       f"DiffusionSolver::solve), median of 3 ({dcpu:.2f} s per step), "
+                       f"iterations {info['iters']}"),
+            "gpu_vs_cpu_result_rel_l2": float(np.linalg.norm(v_gpu - v_cpu) /
+                                              np.linalg.norm(v_cpu - us))}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -467,6 +574,8 @@ def main():
         return bench_cpu_full(args)
     if args.workload == "c4":
         return bench_c4(args)
+    if args.workload in ("diff2", "diff3"):
+        return bench_diffusion(args)
     from cup3d_amd.poisson import solver_for
 
     dev = torch.device("cuda:0")

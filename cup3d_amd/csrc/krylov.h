@@ -274,7 +274,13 @@ __device__ __forceinline__ void iface_face(const DevGrid<T>& g, const T* __restr
 // synchronises with itself alone (wave_sync) and never waits for the other
 // three; the workgroup barrier is the partial-sum reduction at the end.
 // ---------------------------------------------------------------------------
-template <typename T, int MODE, int BM, bool IFACE>
+//
+// HELM (FP64, bMean 0): the implicit-diffusion operator of one velocity
+// component, KernelLHSDiffusion (AdvectionDiffusionImplicit.h:40-56): the
+// same stencil and coarse-fine faces, plus -h^3 sigma x and the Dirichlet
+// ghosts of BlockLabBC (Definitions.h:58-122,287-330) on the boundary faces
+// selected by g.dmask, both folded into the diagonal of the cell.
+template <typename T, int MODE, int BM, bool IFACE, bool HELM = false>
 __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __restrict__ x,
                                                      T* __restrict__ y, const T* __restrict__ a,
                                                      T* __restrict__ c, T* __restrict__ c2,
@@ -354,6 +360,21 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
         yv[iz] = h * ((((xm - c0) + (xp - c0)) + ((ym - c0) + (yp - c0))) +
                       ((zm - c0) + (zp - c0)));
       msum += (double)c0;
+    }
+    if (HELM) {
+      // domain-boundary faces of this block with a Dirichlet ghost
+      int dm = 0;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) dm |= (g.fkind[6 * b + f] == FACE_NONE ? 1 : 0) << f;
+      dm &= g.dmask;
+      const T shift = -(h * h * h) * (T)g.helm_sigma;
+      const int nxy = (ix == 0 ? (dm & 1) : 0) + (ix == BS - 1 ? ((dm >> 1) & 1) : 0) +
+                      (iy == 0 ? ((dm >> 2) & 1) : 0) + (iy == BS - 1 ? ((dm >> 3) & 1) : 0);
+#pragma unroll
+      for (int iz = 0; iz < BS; ++iz) {
+        const int nd = nxy + (iz == 0 ? ((dm >> 4) & 1) : 0) + (iz == BS - 1 ? ((dm >> 5) & 1) : 0);
+        yv[iz] += (shift - (T)(2 * nd) * h) * col[iz];
+      }
     }
   }
   if (IFACE) {
@@ -558,15 +579,15 @@ __global__ __launch_bounds__(WG_THREADS) void k_bmean2_apply(DevGrid<T> g, T* __
 
 // Launch for a grid: picks the bMean and interface variants.
 // An event, if given, is recorded right after the main SpMV kernel.
-template <typename T, int MODE, int BM>
+template <typename T, int MODE, int BM, bool HELM = false>
 inline void spmv_launch_iface(hipStream_t s, const DevGrid<T>& g, const T* x, T* y, const T* a,
                               T* c, T* c2, SolverState* st, double* part,
                               hipEvent_t mid = nullptr) {
   int nwg = (g.nb + 3) / 4;
   if (g.has_iface) {
-    k_spmv<T, MODE, BM, true><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
+    k_spmv<T, MODE, BM, true, HELM><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
   } else {
-    k_spmv<T, MODE, BM, false><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
+    k_spmv<T, MODE, BM, false, HELM><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
   }
   if (mid) (void)hipEventRecord(mid, s);
   if (BM == 2) {
@@ -581,6 +602,12 @@ inline void spmv_launch_iface(hipStream_t s, const DevGrid<T>& g, const T* x, T*
 template <typename T, int MODE>
 inline void spmv_launch(hipStream_t s, const DevGrid<T>& g, const T* x, T* y, const T* a, T* c,
                         T* c2, SolverState* st, double* part, hipEvent_t mid = nullptr) {
+  if constexpr (sizeof(T) == 8) {
+    if (g.helm) {  // implicit diffusion: no bMean row
+      spmv_launch_iface<T, MODE, 0, true>(s, g, x, y, a, c, c2, st, part, mid);
+      return;
+    }
+  }
   if (g.bmean == 1) spmv_launch_iface<T, MODE, 1>(s, g, x, y, a, c, c2, st, part, mid);
   else if (g.bmean == 2) spmv_launch_iface<T, MODE, 2>(s, g, x, y, a, c, c2, st, part, mid);
   else if (g.bmean == 3) spmv_launch_iface<T, MODE, 3>(s, g, x, y, a, c, c2, st, part, mid);

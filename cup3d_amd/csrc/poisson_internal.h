@@ -90,6 +90,13 @@ struct DevGrid {
   long long bmean_row = -1;         // cell replaced by sum h^3 x (bMean == 1)
   int bmean = 1;
   int has_iface = 0;                // any coarse-fine interface rows
+  // implicit diffusion of one velocity component (FP64 only): the operator
+  // gains -h^3 helm_sigma x (helm_sigma = 1 / (nu dt)) and, on the domain-
+  // boundary faces whose bit f is set in dmask, the Dirichlet ghost -x
+  // (h (-x - x) per face) instead of the zero-flux ghost x
+  int helm = 0;
+  int dmask = 0;
+  double helm_sigma = 0.0;
 };
 
 // Host-side topology build output.

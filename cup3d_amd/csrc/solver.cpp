@@ -25,6 +25,7 @@
 #include "kernels.h"
 #include "mixed.h"
 #include "poisson_internal.h"
+#include "diffusion.h"
 #include "topology_gpu.h"
 
 using namespace cup3d;
@@ -98,6 +99,7 @@ struct cup3d_poisson {
   // preconditioner
   std::vector<double> P;
   DevBuf d_P64;
+  DevBuf d_dst;  // implicit diffusion's DST block-inverse table (DST_TABLE doubles)
 
   // FP64 Krylov workspace
   DevBuf d_vec64;  // 11 vectors of n doubles
@@ -171,28 +173,37 @@ int upload(cup3d_poisson* H, DevBuf& d, const void* src, size_t bytes) {
 
 // Run the FP64 classical BiCGSTAB on the device. b64/x64 must already hold
 // getVec's b and x0 (in v64.b and v64.x[0]); the state must be initialised.
-int run_bicgstab64(cup3d_poisson* H) {
+// With a Helmholtz grid (g.helm, implicit diffusion) the preconditioner is
+// the DST block inverse of each block's level instead of the P_inv GEMM.
+int run_bicgstab64(cup3d_poisson* H, const DevGrid<double>& g) {
   hipStream_t s = H->stream;
   SolverState* st = H->d_state.as<SolverState>();
   double* part = H->d_part.as<double>();
   const KrylovVectors<double>& v = H->v64;
+  auto precond = [&](int mode) {
+    if (g.helm)
+      launch_precond_dst(s, mode, H->nb, H->d_dst.as<double>(), g.h, g.helm_sigma, v, st);
+    else
+      launch_precond64(s, mode, H->nb, H->d_P64.as<double>(), g.invh, v, st);
+  };
   // r = b - A x0; rhat = r (BiCGSTAB.cu:435-467)
-  launch_spmv64(s, SPMV_INIT, H->g64, v.x[0], v.nu, v.b, v.r, v.rhat, st, part);
+  launch_spmv64(s, SPMV_INIT, g, v.x[0], v.nu, v.b, v.r, v.rhat, st, part);
   const int poll = H->cfg.poll_every > 0 ? H->cfg.poll_every : 8;
   int guard = 0;
   while (true) {
     for (int k = 0; k < poll; ++k) {
-      launch_precond64(s, PREC_P, H->nb, H->d_P64.as<double>(), H->g64.invh, v, st);
-      launch_spmv64(s, SPMV_DOT1, H->g64, v.z1, v.nu, v.rhat, nullptr, nullptr, st, part);
-      launch_precond64(s, PREC_S, H->nb, H->d_P64.as<double>(), H->g64.invh, v, st);
-      launch_spmv64(s, SPMV_DOT2, H->g64, v.z2, v.t, v.r, nullptr, nullptr, st, part);
+      precond(PREC_P);
+      launch_spmv64(s, SPMV_DOT1, g, v.z1, v.nu, v.rhat, nullptr, nullptr, st, part);
+      precond(PREC_S);
+      launch_spmv64(s, SPMV_DOT2, g, v.z2, v.t, v.r, nullptr, nullptr, st, part);
       launch_update64(s, H->n, v, st, part);
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(H->h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     if (H->h_state->status != ST_RUNNING) break;
-    if (++guard > (H->cfg.max_iter / poll) + 4) return fail(H, "device state machine did not stop");
+    if (++guard > (H->h_state->max_iter / poll) + 4)
+      return fail(H, "device state machine did not stop");
   }
   return 0;
 }
@@ -240,7 +251,10 @@ int cup3d_poisson_create(const cup3d_poisson_config* cfg, cup3d_poisson_t* out) 
   build_pinv_dst(H->P);
   std::vector<double> pk;
   pack_pinv_f64(H->P, pk);
+  double dtab[DST_TABLE];
+  dst_tables(dtab);
   if (upload(H.get(), H->d_P64, pk.data(), pk.size() * sizeof(double)) != 0 ||
+      upload(H.get(), H->d_dst, dtab, sizeof(dtab)) != 0 ||
       mixed_init_pinv(H->mixed, H->P, H->err) != 0 || cnn_init(H->cnn, H->err) != 0) {
     g_create_error = H->err;
     return -1;
@@ -401,7 +415,7 @@ static int solve_fp64(cup3d_poisson_t H, const double* d_rhs, double* d_x, const
                                  ? (long long)H->topo.origin_block * BLEN
                                  : -1;
   launch_getvec64(s, H->n, zero_row, d_rhs, d_x0, H->v64.b, H->v64.x[0], st, part, H->cfg.tol_rhs);
-  if (run_bicgstab64(H) != 0) return -1;
+  if (run_bicgstab64(H, H->g64) != 0) return -1;
   launch_copy_opt64(s, H->n, H->v64, st, d_x);
   // true residual ||b - A x_opt|| in FP64 (into nu)
   launch_spmv64(s, SPMV_RESID, H->g64, d_x, nullptr, H->v64.b, H->v64.nu, nullptr, st, part);
@@ -738,6 +752,177 @@ int cup3d_projection_update_velocity(cup3d_poisson_t H, const double* d_p, doubl
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Implicit diffusion (AdvectionDiffusionImplicit.h:21-525,
+// AdvectionDiffusionImplicit.cpp:328-430): the Helmholtz system of one
+// velocity component on the same topology, FP64, classical BiCGSTAB with the
+// DST block inverse of each level as the preconditioner.
+// ---------------------------------------------------------------------------
+
+// Domain faces whose velocity ghost is -u for component comp (BlockLabBC,
+// Definitions.h:413-458): every wall face, and at freespace faces the
+// component normal to the face.
+static int diffusion_dmask(const cup3d_poisson_config& c, int comp) {
+  int m = 0;
+  for (int d = 0; d < 3; ++d)
+    if (c.bc[d] == CUP3D_BC_WALL || (c.bc[d] == CUP3D_BC_FREESPACE && d == comp))
+      m |= 3 << (2 * d);
+  return m;
+}
+
+static int diffusion_grid(cup3d_poisson_t H, int comp, double nu, double dt, DevGrid<double>& g) {
+  if (comp < 0 || comp > 2) return fail(H, "velocity component must be 0, 1 or 2");
+  if (!(nu > 0) || !std::isfinite(nu) || !(dt > 0) || !std::isfinite(dt))
+    return fail(H, "nu and dt must be positive and finite");
+  const double sigma = 1.0 / (nu * dt);
+  if (!std::isfinite(sigma)) return fail(H, "1 / (nu dt) is not finite");
+  g = H->g64;
+  g.bmean = 0;
+  g.bmean_row = -1;
+  g.helm = 1;
+  g.dmask = diffusion_dmask(H->cfg, comp);
+  g.helm_sigma = sigma;
+  return 0;
+}
+
+// DiffusionSolver::solve on the device: b = d_rhs, x0 = d_x0 (null: 0);
+// the best iterate is left in v64.p and its true residual in the report.
+// Stopping rule of the reference (AdvectionDiffusionImplicit.h:235-236,
+// 509): ||r|| < tol_abs or ||r|| / ||r0|| < tol_rel on the recurrence
+// residual, at most cfg.max_iter iterations (1000 in the reference, :247),
+// restarts unlimited (its cap is commented out, :487).
+static int diffusion_solve(cup3d_poisson_t H, const DevGrid<double>& g, const double* d_rhs,
+                           const double* d_x0, double tol_abs, double tol_rel,
+                           cup3d_solve_report* rep) {
+  hipStream_t s = H->stream;
+  SolverState init = make_state(H->cfg);
+  init.tol_abs = tol_abs;
+  init.tol_rel = tol_rel;
+  init.max_restarts = 1 << 30;
+  *H->h_state = init;
+  SolverState* st = H->d_state.as<SolverState>();
+  double* part = H->d_part.as<double>();
+  const KrylovVectors<double>& v = H->v64;
+  HIPCHECK(hipEventRecord(H->ev0, s));
+  HIPCHECK(hipMemcpyAsync(st, H->h_state, sizeof(SolverState), hipMemcpyHostToDevice, s));
+  launch_getvec64(s, H->n, -1, d_rhs, d_x0, v.b, v.x[0], st, part, 0.0);
+  if (run_bicgstab64(H, g) != 0) return -1;
+  launch_copy_opt64(s, H->n, v, st, v.p);
+  launch_spmv64(s, SPMV_RESID, g, v.p, nullptr, v.b, v.nu, nullptr, st, part);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(H->h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipEventRecord(H->ev1, s));
+  HIPCHECK(hipEventSynchronize(H->ev1));
+  float ms = 0;
+  HIPCHECK(hipEventElapsedTime(&ms, H->ev0, H->ev1));
+  const SolverState& S = *H->h_state;
+  if (rep) {
+    std::memset(rep, 0, sizeof(*rep));
+    rep->status = S.status;
+    rep->iterations = S.iters_done;
+    rep->restarts = S.restarts;
+    rep->refinements = 1;
+    rep->err_init = S.err_init;
+    rep->err_final = S.err_opt;
+    rep->rhs_norm = std::sqrt(S.bnorm2);
+    rep->true_rel_resid = S.bnorm2 > 0 ? std::sqrt(S.true_rr / S.bnorm2) : std::sqrt(S.true_rr);
+    rep->solve_ms = ms;
+    rep->guess_used = d_x0 ? CUP3D_GUESS_GIVEN : CUP3D_GUESS_ZERO;
+    rep->guess_resid = S.err_init;
+  }
+  return S.status;
+}
+
+int cup3d_diffusion_solve_device(cup3d_poisson_t H, int comp, double nu, double dt,
+                                 double tol_abs, double tol_rel, const double* d_rhs, double* d_x,
+                                 cup3d_solve_report* rep) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called before solve");
+  if (check_vectors(H, {d_rhs, d_x}) != 0) return -1;
+  if (d_rhs == d_x) return fail(H, "rhs and x must not alias");
+  DevGrid<double> g;
+  if (diffusion_grid(H, comp, nu, dt, g) != 0) return -1;
+  HIPCHECK(hipSetDevice(H->device));
+  const int st = diffusion_solve(H, g, d_rhs, d_x, tol_abs, tol_rel, rep);
+  if (st < 0) return st;
+  HIPCHECK(hipMemcpyAsync(d_x, H->v64.p, (size_t)H->n * sizeof(double), hipMemcpyDeviceToDevice,
+                          H->stream));
+  HIPCHECK(hipStreamSynchronize(H->stream));
+  return st;
+}
+
+int cup3d_diffusion_step_device(cup3d_poisson_t H, double nu, double dt, double tol_abs,
+                                double tol_rel, const double* d_vel_old, double* d_vel,
+                                cup3d_solve_report* reps) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called before solve");
+  if (check_vectors(H, {d_vel_old, d_vel}) != 0) return -1;
+  if (d_vel_old == d_vel) return fail(H, "vel_old and vel must not alias");
+  HIPCHECK(hipSetDevice(H->device));
+  hipStream_t s = H->stream;
+  const KrylovVectors<double>& v = H->v64;
+  int worst = 0;
+  for (int c = 0; c < 3; ++c) {
+    DevGrid<double> g;
+    if (diffusion_grid(H, c, nu, dt, g) != 0) return -1;
+    // rhs = h^3 (-(h lap u*)/h^3 + (u* - u^n)/(dt nu)) (KernelDiffusionRHS and
+    // AdvectionDiffusionImplicit.cpp:376-400) = a - A u* with the Helmholtz
+    // operator A = h lap - h^3 sigma and a = -h^3 sigma u^n: the residual
+    // form of the operator, evaluated before component c is updated (the
+    // Laplacian of component c reads component c only)
+    *H->h_state = make_state(H->cfg);
+    HIPCHECK(hipMemcpyAsync(H->d_state.p, H->h_state, sizeof(SolverState), hipMemcpyHostToDevice,
+                            s));
+    launch_diffusion_gather(s, H->n, c, g.h3, g.helm_sigma, d_vel, d_vel_old, v.z1, v.z2);
+    launch_spmv64(s, SPMV_RESID, g, v.z1, nullptr, v.z2, v.t, nullptr,
+                  H->d_state.as<SolverState>(), H->d_part.as<double>());
+    // x0 = 0 (the reference zeroes sim.pres before each solve, :398)
+    const int st = diffusion_solve(H, g, v.t, nullptr, tol_abs, tol_rel, reps ? reps + c : nullptr);
+    if (st < 0) return st;
+    launch_diffusion_scatter(s, H->n, c, v.p, d_vel);
+    HIPCHECK(hipGetLastError());
+    worst = std::max(worst, st);
+  }
+  HIPCHECK(hipStreamSynchronize(s));
+  return worst;
+}
+
+int cup3d_diffusion_apply_operator(cup3d_poisson_t H, int comp, double nu, double dt,
+                                   const double* d_x, double* d_y) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  if (check_vectors(H, {d_x, d_y}) != 0) return -1;
+  DevGrid<double> g;
+  if (diffusion_grid(H, comp, nu, dt, g) != 0) return -1;
+  HIPCHECK(hipSetDevice(H->device));
+  *H->h_state = make_state(H->cfg);
+  HIPCHECK(hipMemcpyAsync(H->d_state.p, H->h_state, sizeof(SolverState), hipMemcpyHostToDevice,
+                          H->stream));
+  launch_spmv64(H->stream, SPMV_PLAIN, g, d_x, d_y, nullptr, nullptr, nullptr,
+                H->d_state.as<SolverState>(), H->d_part.as<double>());
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(H->stream));
+  return 0;
+}
+
+int cup3d_diffusion_apply_preconditioner(cup3d_poisson_t H, double nu, double dt,
+                                         const double* d_p, double* d_z) {
+  if (!H) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  if (check_vectors(H, {d_p, d_z}) != 0) return -1;
+  DevGrid<double> g;
+  if (diffusion_grid(H, 0, nu, dt, g) != 0) return -1;
+  HIPCHECK(hipSetDevice(H->device));
+  KrylovVectors<double> v = H->v64;
+  v.p = const_cast<double*>(d_p);
+  v.z1 = d_z;
+  launch_precond_dst(H->stream, PREC_PLAIN, H->nb, H->d_dst.as<double>(), g.h, g.helm_sigma, v,
+                     H->d_state.as<SolverState>());
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(H->stream));
+  return 0;
+}
+
 int cup3d_cnn_num_weights(void) { return cnn_num_weights(); }
 
 int cup3d_poisson_cnn_set_weights(cup3d_poisson_t H, const float* w, int nw) {
@@ -894,6 +1079,50 @@ int cup3d_debug_time_kernel(cup3d_poisson_t H, int which, int reps, double* avg_
   }
   if (bytes) *bytes = by;
   if (flops) *flops = fl;
+  return 0;
+}
+
+int cup3d_debug_time_diffusion_kernel(cup3d_poisson_t H, int which, double nu, double dt,
+                                      int reps, double* avg_ms, double* bytes, double* flops) {
+  if (!H || !avg_ms) return -1;
+  if (H->nb <= 0) return fail(H, "set_topology must be called first");
+  if (reps <= 0 || which < 0 || which > 2) return fail(H, "bad arguments");
+  DevGrid<double> g;
+  if (diffusion_grid(H, 0, nu, dt, g) != 0) return -1;
+  HIPCHECK(hipSetDevice(H->device));
+  hipStream_t s = H->stream;
+  SolverState init = make_state(H->cfg);
+  init.beta = 0.0;  // K1 recomputes p = r on every repetition
+  init.iter = 1;
+  *H->h_state = init;
+  SolverState* st = H->d_state.as<SolverState>();
+  double* part = H->d_part.as<double>();
+  HIPCHECK(hipMemcpyAsync(st, H->h_state, sizeof(SolverState), hipMemcpyHostToDevice, s));
+  const KrylovVectors<double>& v = H->v64;
+  HIPCHECK(hipMemsetAsync(v.r, 0, (size_t)H->n * sizeof(double), s));
+  HIPCHECK(hipMemsetAsync(v.nu, 0, (size_t)H->n * sizeof(double), s));
+  HIPCHECK(hipMemsetAsync(v.p, 0, (size_t)H->n * sizeof(double), s));
+  HIPCHECK(hipMemsetAsync(v.z1, 0, (size_t)H->n * sizeof(double), s));
+  auto launch = [&]() {
+    if (which == 0)
+      launch_spmv64(s, SPMV_PLAIN, g, v.z1, v.nu, nullptr, nullptr, nullptr, st, part);
+    else
+      launch_precond_dst(s, which == 1 ? PREC_P : PREC_PLAIN, H->nb, H->d_dst.as<double>(), g.h,
+                         g.helm_sigma, v, st);
+  };
+  launch();  // warm
+  HIPCHECK(hipEventRecord(H->ev0, s));
+  for (int r = 0; r < reps; ++r) launch();
+  HIPCHECK(hipEventRecord(H->ev1, s));
+  HIPCHECK(hipEventSynchronize(H->ev1));
+  float ms = 0;
+  HIPCHECK(hipEventElapsedTime(&ms, H->ev0, H->ev1));
+  *avg_ms = ms / reps;
+  const double N = (double)H->n;
+  const double words[3] = {2, 5, 2};
+  const double fpc[3] = {10, 96 + 5, 96};
+  if (bytes) *bytes = 8.0 * words[which] * N;
+  if (flops) *flops = fpc[which] * N;
   return 0;
 }
 

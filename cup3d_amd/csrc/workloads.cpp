@@ -189,6 +189,44 @@ void remove_mean(int nb, const int* blocks, double h0, double* b) {
   }
 }
 
+// Blobs whose 8-sigma ball meets the box [lo, hi] at the scene's step.
+std::vector<const Blob*> near_blobs(const Scene& S, const double lo[3], const double hi[3]) {
+  std::vector<const Blob*> near;
+  for (const auto& b : S.blobs) {
+    double d2 = 0;
+    for (int a = 0; a < 3; ++a) {
+      const double c = b.c[a] + S.step * S.speed * b.d[a];
+      const double q = std::max(lo[a] - c, std::max(0.0, c - hi[a]));
+      d2 += q * q;
+    }
+    const double rc = 8 * b.sigma;
+    if (d2 <= rc * rc) near.push_back(&b);
+  }
+  return near;
+}
+
+// The penalised blob velocity at the cell centres, [n][3] (Cubism's
+// VectorBlock layout: [block][iz][iy][ix][component]).
+void blob_vel(const Scene& S, int nb, const int* blocks, double h0, double* out) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < nb; ++i) {
+    const double h = h0 / double(1 << blocks[4 * i]);
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = blocks[4 * i + 1 + a] * BS * h;
+      hi[a] = lo[a] + BS * h;
+    }
+    const auto near = near_blobs(S, lo, hi);
+    const auto ells = near_ells(S, lo, hi);
+    double* o = out + (size_t)i * 512 * 3;
+    for (int c = 0; c < 512; ++c) {
+      const double x[3] = {lo[0] + ((c & 7) + 0.5) * h, lo[1] + (((c >> 3) & 7) + 0.5) * h,
+                           lo[2] + ((c >> 6) + 0.5) * h};
+      vel_at(S, x, near, ells, o + 3 * c);
+    }
+  }
+}
+
 void blob_rhs(const Scene& S, int nb, const int* blocks, double h0, double* out) {
 #pragma omp parallel for schedule(dynamic, 4)
   for (int i = 0; i < nb; ++i) {
@@ -200,17 +238,7 @@ void blob_rhs(const Scene& S, int nb, const int* blocks, double h0, double* out)
       hi[a] = lo[a] + (BS + 2) * h;
     }
     // blobs whose 8-sigma ball meets the block's ghost-extended box
-    std::vector<const Blob*> near;
-    for (const auto& b : S.blobs) {
-      double d2 = 0;
-      for (int a = 0; a < 3; ++a) {
-        const double c = b.c[a] + S.step * S.speed * b.d[a];
-        const double q = std::max(lo[a] - c, std::max(0.0, c - hi[a]));
-        d2 += q * q;
-      }
-      const double rc = 8 * b.sigma;
-      if (d2 <= rc * rc) near.push_back(&b);
-    }
+    const auto near = near_blobs(S, lo, hi);
     const auto ells = near_ells(S, lo, hi);
     const double fac = 0.5 * h * h;  // h^2/(2 dt), dt = 1
     double* o = out + (size_t)i * 512;
@@ -493,6 +521,26 @@ void cw_rhs_blobs(int nb, const int* blocks, double h0, const int* bpd, const in
   }
   if (n_ell > 0) S.ells = make_school(n_ell, ext, ell_axes, ell_seed);
   blob_rhs(S, nb, blocks, h0, out);
+}
+
+// The penalised blob velocity itself at the cell centres ([nb*512][3],
+// VectorBlock layout), same parameters as cw_rhs_blobs: the velocity field a
+// time step's implicit diffusion acts on.
+void cw_vel_blobs(int nb, const int* blocks, double h0, const int* bpd, const int* bc,
+                  int nblobs, double smin, double smax, uint64_t seed, int step, double speed,
+                  const double* sphere_c, double sr, int n_ell, const double* ell_axes,
+                  uint64_t ell_seed, double* out) {
+  const double ext[3] = {bpd[0] * BS * h0, bpd[1] * BS * h0, bpd[2] * BS * h0};
+  Scene S = make_blob_scene(ext, bc, nblobs, smin, smax, seed);
+  S.step = step;
+  S.speed = speed;
+  if (sr > 0) {
+    S.sphere = true;
+    for (int a = 0; a < 3; ++a) S.sc[a] = sphere_c[a];
+    S.sr = sr;
+  }
+  if (n_ell > 0) S.ells = make_school(n_ell, ext, ell_axes, ell_seed);
+  blob_vel(S, nb, blocks, h0, out);
 }
 
 // The blob parameters cw_rhs_blobs draws for (ext, bc, nblobs, sigma range,

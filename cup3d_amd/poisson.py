@@ -58,6 +58,8 @@ EXPORTS = [
     "cup3d_poisson_cnn_eval",
     "cup3d_poisson_num_coarse_fine_faces", "cup3d_poisson_remove_mean",
     "cup3d_projection_rhs", "cup3d_projection_increment", "cup3d_projection_update_velocity",
+    "cup3d_diffusion_solve_device", "cup3d_diffusion_step_device",
+    "cup3d_diffusion_apply_operator", "cup3d_diffusion_apply_preconditioner",
     "cup3d_poisson_destroy", "cup3d_poisson_last_error",
 ]
 
@@ -78,6 +80,16 @@ def lib():
     L.cup3d_projection_rhs.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_double, _vp]
     L.cup3d_projection_increment.argtypes = [_vp, _vp, _vp]
     L.cup3d_projection_update_velocity.argtypes = [_vp, _vp, ctypes.c_double, _vp]
+    L.cup3d_diffusion_solve_device.argtypes = [_vp, ctypes.c_int, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, _vp, _vp, ctypes.POINTER(Report)]
+    L.cup3d_diffusion_step_device.argtypes = [_vp, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_double, _vp, _vp,
+                                              ctypes.POINTER(Report)]
+    L.cup3d_diffusion_apply_operator.argtypes = [_vp, ctypes.c_int, ctypes.c_double,
+                                                 ctypes.c_double, _vp, _vp]
+    L.cup3d_diffusion_apply_preconditioner.argtypes = [_vp, ctypes.c_double, ctypes.c_double,
+                                                       _vp, _vp]
     L.cup3d_cnn_num_weights.argtypes = []
     L.cup3d_poisson_cnn_set_weights.argtypes = [_vp, _vp, ctypes.c_int]
     L.cup3d_poisson_cnn_get_weights.argtypes = [_vp, _vp, ctypes.c_int]
@@ -242,6 +254,48 @@ class PoissonSolver:
         if self.L.cup3d_projection_update_velocity(self.h, _ptr(p), float(dt), _ptr(vel)) != 0:
             raise self._err("projection_update_velocity")
         return vel
+
+    # implicit diffusion (AdvectionDiffusionImplicit), FP64 on the same grid
+    def diffusion_apply(self, comp, nu, dt, x, y):
+        """y = A x, the Helmholtz operator of velocity component comp."""
+        import torch
+        torch.cuda.synchronize(self.device)
+        if self.L.cup3d_diffusion_apply_operator(self.h, int(comp), float(nu), float(dt),
+                                                 _ptr(x), _ptr(y)) != 0:
+            raise self._err("diffusion_apply_operator")
+        return y
+
+    def diffusion_precond(self, nu, dt, p, z):
+        import torch
+        torch.cuda.synchronize(self.device)
+        if self.L.cup3d_diffusion_apply_preconditioner(self.h, float(nu), float(dt), _ptr(p),
+                                                       _ptr(z)) != 0:
+            raise self._err("diffusion_apply_preconditioner")
+        return z
+
+    def diffusion_solve(self, comp, nu, dt, rhs, x, tol_abs=1e-6, tol_rel=1e-4):
+        """One DiffusionSolver::solve; x holds the guess and receives the result."""
+        import torch
+        torch.cuda.synchronize(self.device)
+        rep = Report()
+        st = self.L.cup3d_diffusion_solve_device(self.h, int(comp), float(nu), float(dt),
+                                                 float(tol_abs), float(tol_rel), _ptr(rhs),
+                                                 _ptr(x), ctypes.byref(rep))
+        if st < 0:
+            raise self._err("diffusion_solve_device")
+        return rep.as_dict()
+
+    def diffusion_step(self, vel_old, vel, nu, dt, tol_abs=1e-6, tol_rel=1e-4):
+        """The implicit diffusion of one step on [n, 3] velocities: vel (u*,
+        after advection) becomes u^{n+1}. Returns the three solve reports."""
+        import torch
+        torch.cuda.synchronize(self.device)
+        reps = (Report * 3)()
+        st = self.L.cup3d_diffusion_step_device(self.h, float(nu), float(dt), float(tol_abs),
+                                                float(tol_rel), _ptr(vel_old), _ptr(vel), reps)
+        if st < 0:
+            raise self._err("diffusion_step_device")
+        return [r.as_dict() for r in reps]
 
     # learned warm start
     def cnn_num_weights(self):

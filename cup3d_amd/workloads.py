@@ -50,6 +50,8 @@ def _lib():
                                      ctypes.c_uint64, ctypes.c_int, ctypes.c_double, _f64p,
                                      ctypes.c_double, ctypes.c_int, _f64p, ctypes.c_uint64,
                                      _f64p]
+        lib.cw_vel_blobs.restype = None
+        lib.cw_vel_blobs.argtypes = lib.cw_rhs_blobs.argtypes
         lib.cw_blob_params.restype = None
         lib.cw_blob_params.argtypes = [_f64p, _i32p, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_uint64, _f64p]
@@ -120,6 +122,27 @@ def rhs_blobs(blocks, h0, bpd, bc, nblobs, sigma, seed=SEED, step=0, speed=0.005
     ax = np.ascontiguousarray(axes, np.float64)
     out = np.zeros((len(blocks), 8, 8, 8), np.float64)
     _lib().cw_rhs_blobs(len(blocks), _p(blocks, ctypes.c_int), h0, _p(bpd, ctypes.c_int),
+                        _p(bc, ctypes.c_int), nblobs, smin, smax, seed, step, speed,
+                        _p(sc, ctypes.c_double), sr, n_ell, _p(ax, ctypes.c_double), eseed,
+                        _p(out, ctypes.c_double))
+    return out
+
+
+def vel_blobs(blocks, h0, bpd, bc, nblobs, sigma, seed=SEED, step=0, speed=0.005,
+              sphere=None, school=None):
+    """The penalised blob velocity (1 - chi) u_blobs at the cell centres,
+    [nblocks * 512, 3] (VectorBlock layout), the field whose divergence
+    rhs_blobs takes."""
+    blocks = np.ascontiguousarray(blocks, np.int32)
+    bpd = np.ascontiguousarray(bpd, np.int32)
+    bc = np.ascontiguousarray(bc, np.int32)
+    smin, smax = (sigma, sigma) if np.isscalar(sigma) else sigma
+    sc = np.ascontiguousarray(sphere[0] if sphere else (0, 0, 0), np.float64)
+    sr = float(sphere[1]) if sphere else 0.0
+    n_ell, axes, eseed = school if school else (0, (0.0, 0.0, 0.0), 0)
+    ax = np.ascontiguousarray(axes, np.float64)
+    out = np.zeros((len(blocks) * 512, 3), np.float64)
+    _lib().cw_vel_blobs(len(blocks), _p(blocks, ctypes.c_int), h0, _p(bpd, ctypes.c_int),
                         _p(bc, ctypes.c_int), nblobs, smin, smax, seed, step, speed,
                         _p(sc, ctypes.c_double), sr, n_ell, _p(ax, ctypes.c_double), eseed,
                         _p(out, ctypes.c_double))

@@ -188,6 +188,46 @@ int cup3d_poisson_apply_operator(cup3d_poisson_t h, const double* d_x, double* d
 /* z = blockdiag(h_b^-1 P_inv) p on the device, in the handle's precision. */
 int cup3d_poisson_apply_preconditioner(cup3d_poisson_t h, const double* d_p, double* d_z);
 
+/* Implicit diffusion (-implicitDiffusion; AdvectionDiffusionImplicit.h:21-525,
+ * AdvectionDiffusionImplicit.cpp:328-430) on the handle's grid, in FP64
+ * whatever the handle's precision. The system of velocity component comp
+ * (0, 1, 2) is KernelLHSDiffusion's (AdvectionDiffusionImplicit.h:40-56):
+ *   A x = h (sum of the six lab neighbours - 6 x) - h^3 x / (nu dt)
+ * with the solver's coarse-fine interface and, at non-periodic domain faces,
+ * BlockLabBC's velocity ghosts (Definitions.h:58-122, 287-330, 413-458):
+ * -x on walls, and on freespace faces -x for the normal component and +x for
+ * the tangential ones. Classical BiCGSTAB (the state machine of the pressure
+ * solve) preconditioned by the exact block inverse h^-1 A_loc^-1 of each
+ * block's level, applied by discrete sine transforms (the reference
+ * approximates the same inverse by per-block CG,
+ * DiffusionSolverAMRKernels.cpp:149-213). Stopping rule of DiffusionSolver::
+ * solve: ||r|| < tol_abs or ||r|| / ||r0|| < tol_rel (sim.DiffusionErrorTol,
+ * default 1e-6, and sim.DiffusionErrorTolRel, default 1e-4,
+ * SimulationData.cpp:92-93), at most cfg.max_iter iterations, restarts
+ * unlimited. Returns the status like the pressure solve. */
+/* DiffusionSolver::solve for one component: d_x holds the guess on input
+ * (the reference passes 0) and receives the best iterate. */
+int cup3d_diffusion_solve_device(cup3d_poisson_t h, int comp, double nu, double dt,
+                                 double tol_abs, double tol_rel, const double* d_rhs, double* d_x,
+                                 cup3d_solve_report* rep);
+/* AdvectionDiffusionImplicit::euler after its advection kernel
+ * (AdvectionDiffusionImplicit.cpp:354-419): d_vel_old = u^n and d_vel = u*
+ * (the velocity after the explicit advection), both [nblocks][8][8][8][3];
+ * for each component c: rhs = -h lap(u*_c) + h^3 (u*_c - u^n_c) / (nu dt)
+ * (KernelDiffusionRHS, :19-125, and :376-400), x0 = 0, solve, u*_c += x.
+ * d_vel holds u^{n+1} on return; reps (3 reports, or NULL) one per component.
+ * Returns the worst status of the three solves. */
+int cup3d_diffusion_step_device(cup3d_poisson_t h, double nu, double dt, double tol_abs,
+                                double tol_rel, const double* d_vel_old, double* d_vel,
+                                cup3d_solve_report* reps);
+/* y = A x of component comp (DiffusionSolver::_lhs, AdvectionDiffusionImplicit.h:151-190). */
+int cup3d_diffusion_apply_operator(cup3d_poisson_t h, int comp, double nu, double dt,
+                                   const double* d_x, double* d_y);
+/* z = h_b^-1 A_loc(h_b^2 / (nu dt))^-1 p per block (DiffusionSolver::
+ * _preconditioner, AdvectionDiffusionImplicit.h:111-149, solved exactly). */
+int cup3d_diffusion_apply_preconditioner(cup3d_poisson_t h, double nu, double dt,
+                                         const double* d_p, double* d_z);
+
 /* Learned warm start (precision-independent). A 3-layer 3-D CNN per 8^3
  * block with a one-cell halo predicts the pressure from (rhs, previous
  * pressure); inference runs on FP16 matrix cores, training in FP32, both by

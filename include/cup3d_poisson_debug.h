@@ -44,6 +44,14 @@ int cup3d_debug_apply_operator32(cup3d_poisson_t h, const double* d_x, double* d
 int cup3d_debug_time_kernel(cup3d_poisson_t h, int which, int reps, double* avg_ms,
                             double* bytes, double* flops);
 
+/* The same for the implicit-diffusion kernels (cup3d_diffusion_*) of velocity
+ * component 0 with 1 / (nu dt): which 0 = Helmholtz SpMV, 1 = DST block
+ * inverse fused with the p update (the K1 form), 2 = DST block inverse alone;
+ * algorithmic bytes 2, 5 and 2 FP64 vectors per launch. Needs a topology;
+ * overwrites the handle's FP64 Krylov vectors. */
+int cup3d_debug_time_diffusion_kernel(cup3d_poisson_t h, int which, double nu, double dt,
+                                      int reps, double* avg_ms, double* bytes, double* flops);
+
 /* In-solve kernel timing of the mixed-precision form. While enabled, every
  * solve on the handle records HIP events on its stream around the main kernel
  * of K1 (p update + GEMM), K2 (SpMV + (rhat, nu)), K3 (s update + GEMM), K4

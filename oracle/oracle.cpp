@@ -577,15 +577,19 @@ static void spmv(const Grid& g, const Matrix& M, int bMean, const double* x, dou
   }
 }
 
-// getA_local + Cholesky-Crout + Gauss-Jordan — PoissonSolverExp.cpp:17-107
-static double getA_local(int i, int j) {
-  if (i == j) return 6.;
+// getA_local + Cholesky-Crout + Gauss-Jordan — PoissonSolverExp.cpp:17-107.
+// `diag` is the diagonal of -A_loc: 6 for the pressure equation; 6 + h^2 /
+// (nu dt) gives the per-level block inverse of the implicit diffusion
+// operator (DiffusionSolverAMRKernels.cpp:325 coefficient, sign flipped the
+// same way), whose reference preconditioner iterates CG towards it.
+static double getA_local(int i, int j, double diag = 6.) {
+  if (i == j) return diag;
   const int ix = i % NX, iy = (i / NX) % NY, iz = i / (NX * NY);
   const int jx = j % NX, jy = (j / NX) % NY, jz = j / (NX * NY);
   if (std::abs(ix - jx) + std::abs(iy - jy) + std::abs(iz - jz) == 1) return -1.;
   return 0.;
 }
-static void build_pinv(double* P) {
+static void build_pinv(double* P, double diag = 6.) {
   const int n = NXYZ;
   std::vector<std::vector<double>> L(n), Li(n);
   for (int i = 0; i < n; i++) {
@@ -596,11 +600,11 @@ static void build_pinv(double* P) {
   for (int i = 0; i < n; i++) {
     double s1 = 0;
     for (int k = 0; k <= i - 1; k++) s1 += L[i][k] * L[i][k];
-    L[i][i] = std::sqrt(getA_local(i, i) - s1);
+    L[i][i] = std::sqrt(getA_local(i, i, diag) - s1);
     for (int j = i + 1; j < n; j++) {
       double s2 = 0;
       for (int k = 0; k <= i - 1; k++) s2 += L[i][k] * L[j][k];
-      L[j][i] = (getA_local(j, i) - s2) / L[i][i];
+      L[j][i] = (getA_local(j, i, diag) - s2) / L[i][i];
     }
   }
   for (int br = 0; br < n; br++) {
@@ -680,13 +684,14 @@ struct BiRes {
   double err_init, err_opt;
 };
 
-// BiCGSTABSolver::main — BiCGSTAB.cu:416-628
-static BiRes bicgstab(const Oracle& O, const double* rhs, const double* x0, double* xout,
-                      double max_error, double max_rel_error, int max_restarts, int max_iter,
-                      double* hist) {
-  const int n = O.M.n;
-  std::vector<double> b, x, r, rhat, p(n, 0.), nu(n, 0.), t(n, 0.), z(n, 0.), x_opt;
-  get_vec(O, rhs, x0, b, x);
+// BiCGSTABSolver::main — BiCGSTAB.cu:416-628, over an operator `apply`
+// (y = A x) and a preconditioner `prec` (z = M^-1 p); b and x hold getVec's
+// rhs and initial guess.
+template <class Apply, class Prec>
+static BiRes bicgstab_core(int n, const std::vector<double>& b, std::vector<double> x,
+                           double* xout, Apply&& apply, Prec&& prec, double max_error,
+                           double max_rel_error, int max_restarts, int max_iter, double* hist) {
+  std::vector<double> r, rhat, p(n, 0.), nu(n, 0.), t(n, 0.), z(n, 0.), x_opt;
   r = b;
   struct {
     double alpha, beta, omega, eps, rho_prev, rho_curr, buff_1, buff_2;
@@ -695,7 +700,7 @@ static BiRes bicgstab(const Oracle& O, const double* rhs, const double* x0, doub
   bool converged = false;
   int restarts = 0;
   z = x;  // :435
-  spmv(O.g, O.M, O.bMean, z.data(), nu.data());
+  apply(z.data(), nu.data());
   axpy(-1., nu, r);  // :437
   c.buff_1 = nrm2(nu);
   c.buff_2 = nrm2(r);
@@ -737,14 +742,14 @@ static BiRes bicgstab(const Oracle& O, const double* rhs, const double* x0, doub
     axpy(c.buff_1, nu, p);
     for (int i = 0; i < n; ++i) p[i] = c.beta * p[i];
     axpy(1., r, p);
-    precond(O, p.data(), z.data());                   // :526-528
-    spmv(O.g, O.M, O.bMean, z.data(), nu.data());      // :532
+    prec(p.data(), z.data());                   // :526-528
+    apply(z.data(), nu.data());      // :532
     c.buff_1 = dot(rhat, nu);                          // :535
     c.alpha = c.rho_curr / (c.buff_1 + c.eps);         // :540
     axpy(c.alpha, z, x);                               // :544
     axpy(-c.alpha, nu, r);                             // :547-549
-    precond(O, r.data(), z.data());                    // :553-555
-    spmv(O.g, O.M, O.bMean, z.data(), t.data());       // :559
+    prec(r.data(), z.data());                    // :553-555
+    apply(z.data(), t.data());       // :559
     c.buff_1 = dot(t, r);                              // :562
     c.buff_2 = nrm2(t);
     c.buff_2 *= c.buff_2;
@@ -773,10 +778,25 @@ static BiRes bicgstab(const Oracle& O, const double* rhs, const double* x0, doub
   return res;
 }
 
+static BiRes bicgstab(const Oracle& O, const double* rhs, const double* x0, double* xout,
+                      double max_error, double max_rel_error, int max_restarts, int max_iter,
+                      double* hist) {
+  std::vector<double> b, x;
+  get_vec(O, rhs, x0, b, x);
+  return bicgstab_core(
+      O.M.n, b, std::move(x), xout, [&](const double* a, double* y) { spmv(O.g, O.M, O.bMean, a, y); },
+      [&](const double* a, double* y) { precond(O, a, y); }, max_error, max_rel_error,
+      max_restarts, max_iter, hist);
+}
+
 // ---------------------------------------------------------------------------
 // CPU solver: per-block CG preconditioner (PoissonSolverAMRKernels.cpp:135-332)
 // ---------------------------------------------------------------------------
-static void block_cg(const Oracle& O, const double* in, double* out) {
+// `sigma` = 1 / (nu dt) turns it into the diffusion solver's preconditioner
+// (DiffusionSolverAMRKernels.cpp:149-213): the same CG with the diagonal
+// coefficient -6 - h^2 sigma (:206) in place of -6; sigma = 0 is the pressure
+// solver's.
+static void block_cg(const Oracle& O, const double* in, double* out, double sigma = 0.) {
   const int nb = O.g.nblocks();
   constexpr double kDivEpsilon = 1e-55;           // :76
   constexpr double kNormRel = 1e-7, kNormAbs = 1e-16;  // :77-78
@@ -789,6 +809,7 @@ static void block_cg(const Oracle& O, const double* in, double* out) {
     const double* src = in + (size_t)bb * NXYZ;
     std::memset(p, 0, sizeof(p));
     const double invh = 1 / O.g.h(bb);
+    const double coef = -6.0 - O.g.h(bb) * O.g.h(bb) * sigma;
     double rr = 0;
     for (int iz = 0; iz < NZ; ++iz)  // :305-315
       for (int iy = 0; iy < NY; ++iy)
@@ -806,8 +827,8 @@ static void block_cg(const Oracle& O, const double* in, double* out) {
       for (int iz = 0; iz < NZ; ++iz)
         for (int iy = 0; iy < NY; ++iy)
           for (int ix = 0; ix < NX; ++ix) {
-            const double v = p[iz + 1][iy + 1][ix] + p[iz + 1][iy + 1][ix + 2] -
-                             6 * p[iz + 1][iy + 1][ix + 1] + p[iz + 1][iy][ix + 1] +
+            const double v = p[iz + 1][iy + 1][ix] + p[iz + 1][iy + 1][ix + 2] +
+                             coef * p[iz + 1][iy + 1][ix + 1] + p[iz + 1][iy][ix + 1] +
                              p[iz + 1][iy + 2][ix + 1] + p[iz][iy + 1][ix + 1] +
                              p[iz + 2][iy + 1][ix + 1];
             Ax[iz][iy][ix] = v;
@@ -872,41 +893,32 @@ static void lhs_cpu(const Oracle& O, const double* x, double* y) {
   }
 }
 
-// PoissonSolverAMR::solve — PoissonSolverAMR.cpp:13-358 (Cools-Vanroose Alg. 11)
-static BiRes pipelined(const Oracle& O, const double* rhs, const double* x0, double* xout,
-                       double max_error, double max_rel_error, int max_iter) {
-  const size_t N = O.M.n;
+// PoissonSolverAMR::solve — PoissonSolverAMR.cpp:13-358 (Cools-Vanroose
+// Alg. 11), over an operator `lhs` and a preconditioner `prec`; b holds the
+// rhs (after the bMean zeroing) and x the initial guess. The implicit
+// diffusion solver DiffusionSolver::solve (AdvectionDiffusionImplicit.h:
+// 219-525) is the same algorithm with unlimited restarts (its restart cap is
+// commented out, :487).
+template <class Lhs, class Prec>
+static BiRes pipelined_core(size_t N, const std::vector<double>& b, std::vector<double> x,
+                            double* xout, Lhs&& lhs, Prec&& prec, double max_error,
+                            double max_rel_error, int max_iter, int max_restarts) {
   const double eps = 1e-100;
-  const int max_restarts = 100;
   std::vector<double> phat(N), rhat(N), shat(N), what(N), zhat(N), qhat(N), s(N), w(N), z(N),
-      t(N), v(N), q(N), r(N), y(N), x(N), r0(N), b(N), x_opt(N);
+      t(N), v(N), q(N), r(b), y(N), r0(N), x_opt(N);
   bool useXopt = false;
   int restarts = 0;
   double min_norm = 1e50, norm_1 = 0, norm_2 = 0;
-  int origin = -1;
-  for (int bb = 0; bb < O.g.nblocks(); ++bb) {
-    const Blk& B = O.g.blocks[bb];
-    if (B.index[0] == 0 && B.index[1] == 0 && B.index[2] == 0) origin = bb;
-  }
-  for (size_t j = 0; j < N; ++j) {
-    b[j] = rhs[j];
-    r[j] = rhs[j];
-    x[j] = x0 ? x0[j] : 0.;
-  }
-  if ((O.bMean == 1 || O.bMean > 2) && origin >= 0) {  // :68-70
-    b[(size_t)origin * NXYZ] = 0;
-    r[(size_t)origin * NXYZ] = 0;
-  }
-  lhs_cpu(O, x.data(), r0.data());
+  lhs(x.data(), r0.data());
 #pragma omp parallel for
   for (size_t i = 0; i < N; i++) {
     r0[i] = r[i] - r0[i];
     r[i] = r0[i];
   }
-  block_cg(O, r0.data(), rhat.data());
-  lhs_cpu(O, rhat.data(), w.data());
-  block_cg(O, w.data(), what.data());
-  lhs_cpu(O, what.data(), t.data());
+  prec(r0.data(), rhat.data());
+  lhs(rhat.data(), w.data());
+  prec(w.data(), what.data());
+  lhs(what.data(), t.data());
   double alpha = 0, norm = 0, beta = 0, omega = 0, r0r_prev;
   {
     double temp0 = 0, temp1 = 0;
@@ -942,9 +954,9 @@ static BiRes pipelined(const Oracle& O, const double* rhs, const double* x0, dou
     } else {
 #pragma omp parallel for
       for (size_t j = 0; j < N; j++) phat[j] = rhat[j] + beta * (phat[j] - omega * shat[j]);
-      lhs_cpu(O, phat.data(), s.data());
-      block_cg(O, s.data(), shat.data());
-      lhs_cpu(O, shat.data(), z.data());
+      lhs(phat.data(), s.data());
+      prec(s.data(), shat.data());
+      lhs(shat.data(), z.data());
 #pragma omp parallel for reduction(+ : qy, yy)
       for (size_t j = 0; j < N; j++) {
         q[j] = r[j] - alpha * s[j];
@@ -954,8 +966,8 @@ static BiRes pipelined(const Oracle& O, const double* rhs, const double* x0, dou
         yy += y[j] * y[j];
       }
     }
-    block_cg(O, z.data(), zhat.data());
-    lhs_cpu(O, zhat.data(), v.data());
+    prec(z.data(), zhat.data());
+    lhs(zhat.data(), v.data());
     omega = qy / (yy + eps);
     double r0r = 0, r0w = 0, r0s = 0, r0z = 0;
     norm = 0;
@@ -979,11 +991,11 @@ static BiRes pipelined(const Oracle& O, const double* rhs, const double* x0, dou
     } else {
 #pragma omp parallel for
       for (size_t j = 0; j < N; j++) x[j] = x[j] + alpha * phat[j] + omega * qhat[j];
-      lhs_cpu(O, x.data(), r.data());
+      lhs(x.data(), r.data());
 #pragma omp parallel for
       for (size_t j = 0; j < N; j++) r[j] = b[j] - r[j];
-      block_cg(O, r.data(), rhat.data());
-      lhs_cpu(O, rhat.data(), w.data());
+      prec(r.data(), rhat.data());
+      lhs(rhat.data(), w.data());
 #pragma omp parallel for reduction(+ : r0r, r0w, r0s, r0z, norm_1, norm_2, norm)
       for (size_t j = 0; j < N; j++) {
         r0r += r0[j] * r[j];
@@ -995,8 +1007,8 @@ static BiRes pipelined(const Oracle& O, const double* rhs, const double* x0, dou
         norm_2 += r0[j] * r0[j];
       }
     }
-    block_cg(O, w.data(), what.data());
-    lhs_cpu(O, what.data(), t.data());
+    prec(w.data(), what.data());
+    lhs(what.data(), t.data());
     norm = std::sqrt(norm);
     beta = alpha / (omega + eps) * r0r / (r0r_prev + eps);  // :275
     alpha = r0r / (r0w + beta * r0s - beta * omega * r0z);   // :278-281
@@ -1008,15 +1020,15 @@ static BiRes pipelined(const Oracle& O, const double* rhs, const double* x0, dou
     if (serious_breakdown && restarts < max_restarts) {
       restarts++;
       for (size_t i = 0; i < N; i++) r0[i] = r[i];
-      block_cg(O, r0.data(), rhat.data());
-      lhs_cpu(O, rhat.data(), w.data());
+      prec(r0.data(), rhat.data());
+      lhs(rhat.data(), w.data());
       double temp0 = 0, temp1 = 0;
       for (size_t j = 0; j < N; j++) {
         temp0 += r0[j] * r0[j];
         temp1 += r0[j] * w[j];
       }
-      block_cg(O, w.data(), what.data());
-      lhs_cpu(O, what.data(), t.data());
+      prec(w.data(), what.data());
+      lhs(what.data(), t.data());
       alpha = temp0 / (temp1 + eps);
       r0r_prev = temp0;
       beta = 0.0;
@@ -1041,6 +1053,176 @@ static BiRes pipelined(const Oracle& O, const double* rhs, const double* x0, dou
   res.err_init = init_norm;
   res.err_opt = min_norm;
   return res;
+}
+
+
+static BiRes pipelined(const Oracle& O, const double* rhs, const double* x0, double* xout,
+                       double max_error, double max_rel_error, int max_iter) {
+  const size_t N = O.M.n;
+  int origin = -1;
+  for (int bb = 0; bb < O.g.nblocks(); ++bb) {
+    const Blk& B = O.g.blocks[bb];
+    if (B.index[0] == 0 && B.index[1] == 0 && B.index[2] == 0) origin = bb;
+  }
+  std::vector<double> b(rhs, rhs + N), x(N, 0.);
+  if (x0) std::copy(x0, x0 + N, x.begin());
+  if ((O.bMean == 1 || O.bMean > 2) && origin >= 0) b[(size_t)origin * NXYZ] = 0;  // :68-70
+  return pipelined_core(
+      N, b, std::move(x), xout, [&](const double* a, double* y) { lhs_cpu(O, a, y); },
+      [&](const double* a, double* y) { block_cg(O, a, y); }, max_error, max_rel_error,
+      max_iter, 100);
+}
+
+// ---------------------------------------------------------------------------
+// Implicit diffusion: the operator of KernelLHSDiffusion
+// (AdvectionDiffusionImplicit.h:40-110), the per-block CG preconditioner
+// (DiffusionSolverAMRKernels.cpp:149-213, block_cg above with sigma), the
+// pipelined solver DiffusionSolver::solve (AdvectionDiffusionImplicit.h:
+// 219-525) and the step around it (AdvectionDiffusionImplicit.cpp:354-419,
+// from after the advection kernel). sigma = 1 / (nu dt) throughout.
+//
+// Lab values per face, for velocity component `comp` (BlockLabBC<..., comp>,
+// Definitions.h:58-122 open faces, :287-330 walls, :413-458 dispatch):
+//   inside / same level   the neighbouring cell (periodic wrap included);
+//   coarser               the fine ghost of the interface interpolation, as
+//                         for the pressure operator (Cubism's BlockLab
+//                         interpolation is absent from the reference mount);
+//   finer                 the coarse cell's face flux is replaced by the sum
+//                         of the four fine cells' fluxes h_f (f_q - G_q)
+//                         (Cubism's FluxCorrection, the same flux the
+//                         pressure operator's coarse rows carry);
+//   domain boundary       wall: ghost = -f (u = v = w = 0); freespace: ghost
+//                         = -f for the component normal to the face, +f
+//                         for the tangential ones.
+// ---------------------------------------------------------------------------
+static double bc_ghost_sign(const Grid& g, int face, int comp) {
+  const int d = face / 2;
+  if (g.bc[d] == BC_WALL) return -1.;
+  return d == comp ? -1. : 1.;
+}
+
+// Sum over the four fine cells behind a finer face of coarse cell (b, ix,
+// iy, iz) of (f_q - G_q), G_q the fine ghost, in makeFlux's fine-cell order.
+static double fine_flux_sum(const Grid& g, int b, int ix, int iy, int iz, int face,
+                            const double* f, int nc, int comp) {
+  const Indexer I{g, face};
+  int nI[3], cI[3];
+  g.neighbour(b, face, nI);
+  I.Zchild(nI, ix, iy, iz, cI);
+  const int fine = g.find(g.blocks[b].level + 1, cI);
+  if (fine < 0) throw std::runtime_error("finer face without a finer neighbour");
+  const double sgn[4][2] = {{-1., -1.}, {1., -1.}, {-1., 1.}, {1., 1.}};
+  double acc = 0.;
+  for (int q = 0; q < 4; ++q) {
+    const long long fc = I.neiFine(fine, q, ix, iy, iz, 0);
+    const long long ff = I.neiFine(fine, q, ix, iy, iz, 1);
+    Row r;
+    interp_scaled(I, b, ix, iy, iz, fc, ff, 1., sgn[q][0], sgn[q][1], r);
+    acc += fat(f, nc, comp, fc) - row_apply(r, f, nc, comp);
+  }
+  return acc;
+}
+
+// out = h (sum_faces (lab - f)) - h^3 sigma f for component `fcomp` of an
+// `nc`-component field f, with the boundary ghosts of velocity component
+// `bccomp`. sigma = 0 is KernelDiffusionRHS's h lap(u) (facD = h,
+// AdvectionDiffusionImplicit.cpp:33-44 and its flux correction :46-123).
+static void diffusion_lhs(const Grid& g, int bccomp, double sigma, const double* f, int nc,
+                          int fcomp, double* out) {
+  const int nb = g.nblocks();
+#pragma omp parallel for schedule(static)
+  for (int b = 0; b < nb; ++b) {
+    const double h = g.h(b), hf = 0.5 * h, coef = -h * h * h * sigma;
+    const Indexer G{g, 0};
+    for (int iz = 0; iz < NZ; ++iz)
+      for (int iy = 0; iy < NY; ++iy)
+        for (int ix = 0; ix < NX; ++ix) {
+          const long long k = G.This(b, ix, iy, iz);
+          const double fc = fat(f, nc, fcomp, k);
+          double acc = 0.;
+          for (int face = 0; face < 6; ++face) {
+            const int d = face / 2, s = (face % 2) ? 1 : -1;
+            int nbk = -1;
+            const int cls = face_class(g, b, ix, iy, iz, face, nbk);
+            if (cls == FC_INSIDE) {
+              const int c[3] = {ix + (d == 0 ? s : 0), iy + (d == 1 ? s : 0), iz + (d == 2 ? s : 0)};
+              acc += h * (fat(f, nc, fcomp, G.This(b, c[0], c[1], c[2])) - fc);
+            } else if (cls == FC_BOUNDARY) {
+              acc += h * (bc_ghost_sign(g, face, bccomp) * fc - fc);
+            } else if (cls == FC_FINE) {
+              acc += hf * fine_flux_sum(g, b, ix, iy, iz, face, f, nc, fcomp);
+            } else {
+              acc += h * (ghost(g, b, ix, iy, iz, face, cls, nbk, f, nc, fcomp) - fc);
+            }
+          }
+          out[k] = acc + coef * fc;
+        }
+  }
+}
+
+struct DiffusionPinv {
+  std::map<int, std::vector<double>> by_level;  // P_inv(6 + h_l^2 sigma) per level
+};
+
+// z_b = invh_b P_inv(level_b) p_b: the exact per-level block inverse the
+// reference's CG preconditioner iterates towards.
+static void diffusion_precond(const Oracle& O, double sigma, DiffusionPinv& D, const double* p,
+                              double* z) {
+  const int nb = O.g.nblocks();
+  for (int b = 0; b < nb; ++b) {
+    const int l = O.g.blocks[b].level;
+    if (!D.by_level.count(l)) {
+      std::vector<double> P((size_t)NXYZ * NXYZ);
+      const double h = O.g.h(b);
+      build_pinv(P.data(), 6. + h * h * sigma);
+      D.by_level[l] = std::move(P);
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int b = 0; b < nb; ++b) {
+    const double* P = D.by_level.at(O.g.blocks[b].level).data();
+    const double* pb = p + (size_t)b * NXYZ;
+    double* zb = z + (size_t)b * NXYZ;
+    for (int i = 0; i < NXYZ; ++i) {
+      double acc = 0;
+      for (int j = 0; j < NXYZ; ++j) acc += P[(size_t)i * NXYZ + j] * pb[j];
+      zb[i] = O.invh[b] * acc;
+    }
+  }
+}
+
+// One implicit-diffusion step after the advection kernel
+// (AdvectionDiffusionImplicit.cpp:354-419): vel_old = u^n, vel = u* (in) and
+// u^{n+1} (out), both [n][3]. TMPV = h lap(u*) for all three components
+// first; then per component: rhs = h^3 (-TMPV / h^3 + (u* - u^n) / (dt nu)),
+// x0 = 0, the pipelined solve, u*_c += x.
+static void diffusion_step(const Oracle& O, double nu, double dt, double tol_abs,
+                           double tol_rel, int max_iter, const double* vel_old, double* vel,
+                           int* iters, double* errs) {
+  const size_t n = O.M.n;
+  const double sigma = 1.0 / (nu * dt);
+  std::vector<double> tmp(3 * n);
+  for (int c = 0; c < 3; ++c) diffusion_lhs(O.g, c, 0.0, vel, 3, c, tmp.data() + c * n);
+  for (int c = 0; c < 3; ++c) {
+    std::vector<double> b(n), x(n);
+    for (int bb = 0; bb < O.g.nblocks(); ++bb) {
+      const double h3 = O.h3[bb], ih3 = 1.0 / h3;
+      for (int j = 0; j < NXYZ; ++j) {
+        const size_t k = (size_t)bb * NXYZ + j;
+        const double t = -tmp[c * n + k] * ih3 + (vel[3 * k + c] - vel_old[3 * k + c]) / (dt * nu);
+        b[k] = h3 * t;
+      }
+    }
+    BiRes r = pipelined_core(
+        n, b, std::vector<double>(n, 0.), x.data(),
+        [&](const double* a, double* y) { diffusion_lhs(O.g, c, sigma, a, 1, 0, y); },
+        [&](const double* a, double* y) { block_cg(O, a, y, sigma); }, tol_abs, tol_rel,
+        max_iter, 1 << 30);
+    for (size_t k = 0; k < n; ++k) vel[3 * k + c] += x[k];
+    iters[c] = r.iters;
+    errs[2 * c] = r.err_init;
+    errs[2 * c + 1] = r.err_opt;
+  }
 }
 
 thread_local std::string g_err;
@@ -1158,6 +1340,71 @@ void orc_pressure_rhs(void* o, const double* vel, const double* udef, const doub
 // KernelGradP + velocity update, vel [n][3] updated in place.
 void orc_grad_p_update(void* o, const double* p, double dt, double* vel) {
   grad_p_update(((Oracle*)o)->g, p, dt, vel);
+}
+
+// Implicit diffusion (test infrastructure): operator of velocity component
+// `comp` with sigma = 1 / (nu dt) (sigma = 0: h lap), on a scalar field.
+void orc_diffusion_lhs(void* o, int comp, double sigma, const double* x, double* y) {
+  diffusion_lhs(((Oracle*)o)->g, comp, sigma, x, 1, 0, y);
+}
+// The same on component `comp` of a [n][3] field (the rhs Laplacian).
+void orc_diffusion_lap_vec(void* o, int comp, const double* vel, double* y) {
+  diffusion_lhs(((Oracle*)o)->g, comp, 0.0, vel, 3, comp, y);
+}
+// Exact per-level block inverse and the reference's block CG.
+void orc_diffusion_precond(void* o, double sigma, const double* p, double* z) {
+  DiffusionPinv D;
+  diffusion_precond(*(Oracle*)o, sigma, D, p, z);
+}
+void orc_diffusion_block_cg(void* o, double sigma, const double* p, double* z) {
+  block_cg(*(Oracle*)o, p, z, sigma);
+}
+// Classical BiCGSTAB (BiCGSTAB.cu:416-628) on the diffusion system with the
+// exact per-level block inverse.
+int orc_diffusion_bicgstab(void* o, int comp, double sigma, const double* rhs, const double* x0,
+                           double* x, double tol_abs, double tol_rel, int max_restarts,
+                           int max_iter, int* iters_restarts, double* errs, double* hist) {
+  const Oracle& O = *(Oracle*)o;
+  const int n = O.M.n;
+  DiffusionPinv D;
+  std::vector<double> b(rhs, rhs + n), xv(n, 0.);
+  if (x0) std::copy(x0, x0 + n, xv.begin());
+  BiRes r = bicgstab_core(
+      n, b, std::move(xv), x,
+      [&](const double* a, double* y) { diffusion_lhs(O.g, comp, sigma, a, 1, 0, y); },
+      [&](const double* a, double* y) { diffusion_precond(O, sigma, D, a, y); }, tol_abs,
+      tol_rel, max_restarts, max_iter, hist);
+  iters_restarts[0] = r.iters;
+  iters_restarts[1] = r.restarts;
+  errs[0] = r.err_init;
+  errs[1] = r.err_opt;
+  return r.status;
+}
+// DiffusionSolver::solve (pipelined, block CG, unlimited restarts).
+int orc_diffusion_pipelined(void* o, int comp, double sigma, const double* rhs, const double* x0,
+                            double* x, double tol_abs, double tol_rel, int max_iter,
+                            int* iters_restarts, double* errs) {
+  const Oracle& O = *(Oracle*)o;
+  const size_t n = O.M.n;
+  std::vector<double> b(rhs, rhs + n), xv(n, 0.);
+  if (x0) std::copy(x0, x0 + n, xv.begin());
+  BiRes r = pipelined_core(
+      n, b, std::move(xv), x,
+      [&](const double* a, double* y) { diffusion_lhs(O.g, comp, sigma, a, 1, 0, y); },
+      [&](const double* a, double* y) { block_cg(O, a, y, sigma); }, tol_abs, tol_rel, max_iter,
+      1 << 30);
+  iters_restarts[0] = r.iters;
+  iters_restarts[1] = r.restarts;
+  errs[0] = r.err_init;
+  errs[1] = r.err_opt;
+  return r.status;
+}
+// AdvectionDiffusionImplicit::euler after the advection kernel; iters[3],
+// errs[6] = (err_init, err_opt) per component.
+void orc_diffusion_step(void* o, double nu, double dt, double tol_abs, double tol_rel,
+                        int max_iter, const double* vel_old, double* vel, int* iters,
+                        double* errs) {
+  diffusion_step(*(Oracle*)o, nu, dt, tol_abs, tol_rel, max_iter, vel_old, vel, iters, errs);
 }
 
 int orc_num_threads() {

@@ -49,6 +49,23 @@ def _load(fast: bool = False):
     lib.orc_pressure_rhs.argtypes = [ctypes.c_void_p, _f64p, _f64p, _f64p, ctypes.c_double,
                                      _f64p]
     lib.orc_grad_p_update.argtypes = [ctypes.c_void_p, _f64p, ctypes.c_double, _f64p]
+    lib.orc_diffusion_lhs.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _f64p,
+                                      _f64p]
+    lib.orc_diffusion_lap_vec.argtypes = [ctypes.c_void_p, ctypes.c_int, _f64p, _f64p]
+    for f in ("orc_diffusion_precond", "orc_diffusion_block_cg"):
+        getattr(lib, f).argtypes = [ctypes.c_void_p, ctypes.c_double, _f64p, _f64p]
+    lib.orc_diffusion_bicgstab.restype = ctypes.c_int
+    lib.orc_diffusion_bicgstab.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                           _f64p, _f64p, _f64p, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_int, ctypes.c_int, _i32p,
+                                           _f64p, _f64p]
+    lib.orc_diffusion_pipelined.restype = ctypes.c_int
+    lib.orc_diffusion_pipelined.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                            _f64p, _f64p, _f64p, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_int, _i32p, _f64p]
+    lib.orc_diffusion_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, _f64p,
+                                       _f64p, _i32p, _f64p]
     lib.orc_last_error.restype = ctypes.c_char_p
     lib.orc_num_threads.restype = ctypes.c_int
     _libs[name] = lib
@@ -173,6 +190,73 @@ class Oracle:
                                     _p(ir, ctypes.c_int), _p(errs))
         return x, dict(status=st, iters=int(ir[0]), restarts=int(ir[1]),
                        err_init=float(errs[0]), err_final=float(errs[1]))
+
+    # ---- implicit diffusion (sigma = 1 / (nu dt)) ----
+    def diffusion_lhs(self, x, comp, sigma):
+        """KernelLHSDiffusion on a scalar field, boundary ghosts of velocity
+        component `comp`; sigma = 0 gives KernelDiffusionRHS's h lap."""
+        x = self._vec(x)
+        y = np.zeros(self.n, np.float64)
+        self.lib.orc_diffusion_lhs(self.obj, comp, float(sigma), _p(x), _p(y))
+        return y
+
+    def diffusion_lap_vec(self, vel, comp):
+        vel = np.ascontiguousarray(np.asarray(vel, np.float64).reshape(self.n, 3))
+        y = np.zeros(self.n, np.float64)
+        self.lib.orc_diffusion_lap_vec(self.obj, comp, _p(vel), _p(y))
+        return y
+
+    def diffusion_precond(self, p, sigma):
+        y = np.zeros(self.n, np.float64)
+        self.lib.orc_diffusion_precond(self.obj, float(sigma), _p(self._vec(p)), _p(y))
+        return y
+
+    def diffusion_block_cg(self, p, sigma):
+        y = np.zeros(self.n, np.float64)
+        self.lib.orc_diffusion_block_cg(self.obj, float(sigma), _p(self._vec(p)), _p(y))
+        return y
+
+    def diffusion_bicgstab(self, rhs, comp, sigma, x0=None, tol_abs=0.0, tol_rel=1e-7,
+                           max_restarts=100, max_iter=1000, history=False):
+        rhs = self._vec(rhs)
+        x0 = None if x0 is None else self._vec(x0)
+        x = np.zeros(self.n, np.float64)
+        ir = np.zeros(2, np.int32)
+        errs = np.zeros(2, np.float64)
+        hist = np.zeros(max_iter, np.float64) if history else None
+        st = self.lib.orc_diffusion_bicgstab(self.obj, comp, float(sigma), _p(rhs), _p(x0),
+                                             _p(x), tol_abs, tol_rel, max_restarts, max_iter,
+                                             _p(ir, ctypes.c_int), _p(errs), _p(hist))
+        info = dict(status=st, iters=int(ir[0]), restarts=int(ir[1]), err_init=float(errs[0]),
+                    err_final=float(errs[1]))
+        if history:
+            info["history"] = hist[: info["iters"]].copy()
+        return x, info
+
+    def diffusion_pipelined(self, rhs, comp, sigma, x0=None, tol_abs=0.0, tol_rel=1e-7,
+                            max_iter=1000):
+        rhs = self._vec(rhs)
+        x0 = None if x0 is None else self._vec(x0)
+        x = np.zeros(self.n, np.float64)
+        ir = np.zeros(2, np.int32)
+        errs = np.zeros(2, np.float64)
+        st = self.lib.orc_diffusion_pipelined(self.obj, comp, float(sigma), _p(rhs), _p(x0),
+                                              _p(x), tol_abs, tol_rel, max_iter,
+                                              _p(ir, ctypes.c_int), _p(errs))
+        return x, dict(status=st, iters=int(ir[0]), restarts=int(ir[1]),
+                       err_init=float(errs[0]), err_final=float(errs[1]))
+
+    def diffusion_step(self, vel_old, vel, nu, dt, tol_abs=1e-6, tol_rel=1e-4, max_iter=1000):
+        """AdvectionDiffusionImplicit::euler after the advection kernel:
+        returns u^{n+1} [n, 3] from u^n = vel_old and u* = vel."""
+        vo = np.ascontiguousarray(np.asarray(vel_old, np.float64).reshape(self.n, 3))
+        v = np.ascontiguousarray(np.asarray(vel, np.float64).reshape(self.n, 3)).copy()
+        its = np.zeros(3, np.int32)
+        errs = np.zeros(6, np.float64)
+        self.lib.orc_diffusion_step(self.obj, float(nu), float(dt), tol_abs, tol_rel, max_iter,
+                                    _p(vo), _p(v), _p(its, ctypes.c_int), _p(errs))
+        return v, dict(iters=its.tolist(), err_init=errs[0::2].tolist(),
+                       err_final=errs[1::2].tolist())
 
 
 def num_threads(fast=True) -> int:

@@ -39,6 +39,13 @@ def random_field(nb, seed=0):
     return np.random.default_rng(seed).standard_normal((nb, 8, 8, 8))
 
 
+def compatible_rhs(blocks, h0, seed):
+    """A random rhs with zero volume-weighted sum (solvable with walls)."""
+    h3 = np.repeat((h0 / 2.0 ** np.asarray(blocks)[:, 0]) ** 3, 512)
+    b = random_field(len(blocks), seed).reshape(-1)
+    return b - h3 * b.sum() / h3.sum()
+
+
 def cell_centres(blocks, h0):
     """Cell-centre coordinates [nb, 8, 8, 8, 3] and spacing h[nb]."""
     blocks = np.asarray(blocks)

@@ -14,8 +14,8 @@ import torch
 
 from cup3d_amd import workloads as W
 from cup3d_amd.poisson import PoissonSolver, solver_for
-from helpers import (PER, WALL, fft_solution_periodic, grid_cases, index_encoded_case,
-                     random_field, school4_small, tiny_amr, uniform)
+from helpers import (PER, WALL, compatible_rhs, fft_solution_periodic, grid_cases,
+                     index_encoded_case, random_field, school4_small, tiny_amr, uniform)
 
 pytestmark = pytest.mark.gpu
 
@@ -27,12 +27,6 @@ CASES["tiny_amr_periodic"] = tiny_amr(periodic=True)
 
 def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a, np.float64).reshape(-1)).cuda()
-
-
-def compatible_rhs(blocks, h0, seed):
-    h3 = np.repeat((h0 / 2.0 ** np.asarray(blocks)[:, 0]) ** 3, 512)
-    b = random_field(len(blocks), seed).reshape(-1)
-    return b - h3 * b.sum() / h3.sum()
 
 
 def pair(name, bmean, **kw):
@@ -84,7 +78,7 @@ def test_preconditioner_matches_oracle(gpu, name):
 
 
 @pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "uniform_periodic_2x3x2", "amr2_walls",
-                                  "amr3_walls", "tiny_amr_periodic", "school4_small"])
+                                  "amr3_walls", "tiny_amr_periodic"])
 def test_solve_bmean0_same_iterations(gpu, name):
     S, O, blocks, h0 = pair(name, 0, tol_rel=1e-7)
     b = compatible_rhs(blocks, h0, 3)
@@ -96,6 +90,31 @@ def test_solve_bmean0_same_iterations(gpu, name):
     xg = x.cpu().numpy()
     assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= 1e-10
     assert abs(rep["err_init"] - info["err_init"]) <= 1e-13 * info["err_init"]
+
+
+def test_solve_bmean0_school4_small(gpu):
+    """On the 4-level school grid the recurrence amplifies rounding even at
+    bMean = 0: the oracle against itself under 1e-15 rhs perturbations needs
+    107-115 iterations instead of 108 and moves the solution by up to 4e-6
+    (test_oracle_solver.py::test_school4_bmean0_rounding_sensitivity). So the
+    first 10 iterates agree to rounding, and the converged solve agrees
+    within the spread the oracle shows against itself."""
+    name = "school4_small"
+    S, O, blocks, h0 = pair(name, 0, tol_rel=0.0, max_iter=10)
+    b = compatible_rhs(blocks, h0, 3)
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(dev(b), x)
+    xo, info = O.bicgstab(b, tol_rel=0.0, max_iter=10)
+    assert rep["iterations"] == info["iters"] == 10
+    assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-10 * np.linalg.norm(xo)
+    S, O, blocks, h0 = pair(name, 0, tol_rel=1e-7)
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(dev(b), x)
+    xo, info = O.bicgstab(b, tol_rel=1e-7)
+    assert rep["status"] == info["status"] == 0
+    assert abs(rep["iterations"] - info["iters"]) <= 0.15 * info["iters"], (rep, info)
+    assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-5 * np.linalg.norm(xo)
+    assert rep["true_rel_resid"] <= 1.5e-7 * rep["err_init"] / rep["rhs_norm"]
 
 
 @pytest.mark.parametrize("name", list(CASES))

@@ -114,3 +114,22 @@ def test_blob_rhs_penalised_sphere():
     # the obstacle changes the rhs in a shell around its surface
     shell = ~deep & ~away
     assert np.abs((pen - free)[shell] - d.mean()).max() > 1e-3 * np.abs(free).max()
+
+
+def test_vel_blobs_is_the_field_rhs_blobs_differentiates():
+    """The velocity generator and the rhs generator draw the same blobs: the
+    oracle's KernelPressureRHS (dt = 1) of the cell-centre velocity equals
+    the generated rhs, before its mean removal, at every cell whose six
+    neighbours lie inside the domain (uniform grid: the neighbours' centres
+    are the points the rhs generator evaluates)."""
+    import pyoracle
+    from helpers import boundary_cells
+    w = W.c2_uniform_walls((4, 2, 2))
+    vel = W.vel_blobs(w.blocks, w.h0, w.bpd, w.bc, 256, (0.02, 0.08), sphere=W.SPHERE, step=3)
+    rhs = w.rhs(step=3).reshape(-1)
+    O = pyoracle.Oracle(w.blocks, w.bpd, w.h0, w.bc, 0)
+    div = O.pressure_rhs(vel, 1.0)
+    inner = ~boundary_cells(w.blocks, w.bpd, w.bc).reshape(-1)
+    d = (rhs - div)[inner]  # the removed mean: one constant on a uniform grid
+    assert np.abs(d - d.mean()).max() <= 1e-13 * np.abs(div).max()
+    assert np.abs(vel).max() > 0.1
