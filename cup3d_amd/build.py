@@ -34,7 +34,7 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
              "-I", INCLUDE, "-I", CSRC]
 
 POISSON_SOURCES = ["kernels.hip", "mixed.hip", "cnn.hip", "projection.hip", "topology_gpu.hip",
-                   "diffusion.hip", "topology.cpp", "solver.cpp"]
+                   "diffusion.hip", "noalloc.hip", "topology.cpp", "solver.cpp"]
 
 
 def _run(cmd):

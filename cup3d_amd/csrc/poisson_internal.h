@@ -125,6 +125,10 @@ struct TopologyInput {
   bool rows = false;  // also assemble the explicit interface rows (debug matrix only)
 };
 
+// The message cup3d_poisson_last_error(NULL) returns (entry points without
+// a handle).
+void set_global_error(const std::string& m);
+
 // topology.cpp
 bool build_topology(const TopologyInput& in, HostTopology& out, std::string& err);
 void build_pinv_dst(std::vector<double>& P);  // 512x512 row-major, = A_loc^-1 with sign of the reference

@@ -34,6 +34,12 @@ namespace {
 
 thread_local std::string g_create_error;
 
+}  // namespace
+
+void cup3d::set_global_error(const std::string& m) { g_create_error = m; }
+
+namespace {
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;

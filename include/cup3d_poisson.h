@@ -156,6 +156,23 @@ int cup3d_poisson_solve_device(cup3d_poisson_t h, const double* d_rhs, double* d
  * returned in *mean when mean is not NULL. */
 int cup3d_poisson_remove_mean(cup3d_poisson_t h, double* d_x, double* mean);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). cup3d_poisson_remove_mean without a handle and without
+ * any allocation: d_h3 holds h_b^3 per block (nblocks doubles), d_x the
+ * field ([nblocks][512], 16-byte aligned), both on the current device. The
+ * partial sums of the reduction live in a static device array of the
+ * library sized for CUP3D_NOALLOC_MAX_BLOCKS blocks (4 MiB per GPU, held
+ * until the process ends); larger inputs are rejected on the host. Launches
+ * on `stream` (a hipStream_t; NULL = the default stream). If mean is not
+ * NULL the removed mean is copied to *mean and the call waits for the
+ * stream; with mean NULL it only enqueues (capturable). The array is shared
+ * by every call on a GPU: calls that overlap on one GPU race. Results are
+ * bitwise those of cup3d_poisson_remove_mean on the same grid. */
+#define CUP3D_NOALLOC_MAX_BLOCKS (1 << 20)
+int cup3d_remove_mean_noalloc(int64_t nblocks, const double* d_h3, double* d_x, void* stream,
+                              double* mean);
+
 /* The caller around the solve, on the handle's grid and stream
  * (PressureProjection::operator(), PressureProjection.cpp:333-466). Vector
  * fields are Cubism VectorBlocks: [nblocks][8][8][8][3] doubles ([iz][iy][ix],
