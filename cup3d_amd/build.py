@@ -5,6 +5,8 @@
   libcup3d_workloads.so synthetic grids and right-hand sides (g++, OpenMP)
   example_projection    a caller of the header-only C++ PoissonSolverBase
                         adapter (cup3d_amd/host) on the standalone stand-in types
+  libcup3d_h5.so        block fields in the reference's multi-block HDF5 layout
+                        (include/cup3d_h5.h; host only, the image's HDF5)
 
 Run: python -m cup3d_amd.build
 """
@@ -72,6 +74,24 @@ def build_workloads():
     return out
 
 
+HDF5 = os.environ.get("CUP3D_HDF5", "/opt/conda")  # the image's serial HDF5 1.10
+
+
+def build_h5():
+    """libcup3d_h5.so: block fields in the reference's multi-block HDF5 layout
+    (host only); skipped, with a message, where no HDF5 is installed."""
+    if not os.path.exists(os.path.join(HDF5, "include", "hdf5.h")):
+        sys.stderr.write(f"libcup3d_h5.so not built: no HDF5 under {HDF5}\n")
+        return None
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libcup3d_h5.so")
+    _run([CXX, "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-I", INCLUDE,
+          "-I", os.path.join(HDF5, "include"), os.path.join(CSRC, "h5io.cpp"), "-o", out + ".tmp",
+          "-L", os.path.join(HDF5, "lib"), "-lhdf5", f"-Wl,-rpath,{os.path.join(HDF5, 'lib')}"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_example():
     out = os.path.join(LIB, "example_projection")
     _run([CXX, "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, "-I", HOST,
@@ -82,7 +102,7 @@ def build_example():
 
 
 def build_all():
-    return [build_workloads(), build_poisson(), build_example()]
+    return [p for p in (build_workloads(), build_poisson(), build_example(), build_h5()) if p]
 
 
 if __name__ == "__main__":

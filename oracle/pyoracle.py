@@ -259,5 +259,57 @@ class Oracle:
                        err_final=errs[1::2].tolist())
 
 
+def _load_h5():
+    if "h5" in _libs:
+        return _libs["h5"]
+    path = os.path.join(HERE, "liboracle_h5.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} missing: run `make -C oracle` (needs HDF5)")
+    lib = ctypes.CDLL(path)
+    lib.orc_h5_groups.restype = ctypes.c_int64
+    lib.orc_h5_groups.argtypes = [ctypes.c_char_p, _f64p, _i32p, ctypes.c_int64]
+    lib.orc_h5_dataset.restype = ctypes.c_int64
+    lib.orc_h5_dataset.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_float),
+                                   ctypes.c_int64]
+    lib.orc_h5_to_uniform.restype = ctypes.c_int64
+    lib.orc_h5_to_uniform.argtypes = [ctypes.c_char_p, _i32p, ctypes.POINTER(ctypes.c_float),
+                                      ctypes.c_int64]
+    _libs["h5"] = lib
+    return lib
+
+
+def h5_groups(base, cap=1 << 22):
+    """get_amr_groups (tools/uniform_conversion/main.cpp:59-99): origins
+    [g, 4] and indices [g, 7] of <base>-groups.h5."""
+    o = np.zeros((cap, 4), np.float64)
+    ix = np.zeros((cap, 7), np.int32)
+    n = _load_h5().orc_h5_groups(str(base).encode(), _p(o), _p(ix, ctypes.c_int), cap)
+    if n < 0:
+        raise RuntimeError("orc_h5_groups failed")
+    return o[:n].copy(), ix[:n].copy()
+
+
+def h5_dataset(base, cap=1 << 28):
+    """get_amr_dataset (:102-150): dset of <base>.h5 as FP32."""
+    out = np.zeros(cap, np.float32)
+    n = _load_h5().orc_h5_dataset(str(base).encode(), _p(out, ctypes.c_float), cap)
+    if n < 0:
+        raise RuntimeError("orc_h5_dataset failed")
+    return out[:n].copy()
+
+
+def h5_to_uniform(base, cap=1 << 26):
+    """convert_to_uniform (:152-238) on one rank: the uniform grid [z][y][x]
+    at 1/8 of the finest resolution."""
+    dims = np.zeros(3, np.int32)
+    out = np.zeros(cap, np.float32)
+    n = _load_h5().orc_h5_to_uniform(str(base).encode(), _p(dims, ctypes.c_int),
+                                     _p(out, ctypes.c_float), cap)
+    if n < 0:
+        raise RuntimeError("orc_h5_to_uniform failed")
+    nx, ny, nz = (int(d) for d in dims)
+    return out[:n].reshape(nz, ny, nx).copy()
+
+
 def num_threads(fast=True) -> int:
     return _load(fast).orc_num_threads()
