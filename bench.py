@@ -72,6 +72,7 @@ HBM_PEAK = 8000.0       # GB/s, MI355X
 FP64_MFMA_PEAK = 78.6   # TFLOP/s, dense FP64 matrix
 F16_MFMA_PEAK = 2500.0  # TFLOP/s, dense FP16/BF16 matrix
 MAX_ITER = 3000         # iteration cap of every timed form
+CPU_MAX_ITER = 1000     # the reference CPU solver's cap (PoissonSolverAMR.cpp:127)
 KERNELS = {0: "spmv", 1: "precond_gemm", 2: "update_K5", 3: "precond_update_K1",
            4: "spmv_dot_K2"}
 # Algorithmic FP32 words per cell of the mixed form's five kernels: vectors read
@@ -129,6 +130,10 @@ def parse():
     p.add_argument("--c4-steps", type=int, default=400)
     p.add_argument("--c4-train", type=int, default=300)
     p.add_argument("--c4-fresh", type=int, default=40)
+    p.add_argument("--no-dst", action="store_true",
+                   help="skip the leg with the sine-transform block inverse")
+    p.add_argument("--fp64-dst-steps", type=int, default=4,
+                   help="timed steps of the FP64 form with the sine-transform block inverse")
     p.add_argument("--nu", type=float, default=1e-3,
                    help="diffusion workloads: kinematic viscosity")
     p.add_argument("--dt", type=float, default=0.01, help="diffusion workloads: time step")
@@ -167,7 +172,7 @@ def in_solve_profile(S, enable=None):
 
 
 def time_form(w, form, bmean, seq, warmup, err_fn, weights=None, profile=False, tol=1e-7,
-              extrapolate=False, log=None):
+              extrapolate=False, log=None, precond="gemm"):
     """Solve every step of the sequence with `form`, each handed the previous
     step's reference pressure (given net weights, the learned guess with that
     pressure as the fallback; with `extrapolate`, also the extrapolation in
@@ -179,7 +184,8 @@ def time_form(w, form, bmean, seq, warmup, err_fn, weights=None, profile=False, 
     import torch
     from cup3d_amd.poisson import solver_for
     S = solver_for(w, tol_rel=0.0, tol_rhs=tol, precision=form, bmean=bmean, max_iter=MAX_ITER,
-                   guess="given" if weights is None else "cnn", extrapolate=extrapolate)
+                   guess="given" if weights is None else "cnn", extrapolate=extrapolate,
+                   precond=precond)
     if weights is not None:
         S.cnn_set_weights(weights)
     if profile:
@@ -276,25 +282,37 @@ def host_solve_ms(S, rhs, p_prev):
     return round((time.perf_counter() - t0) * 1e3, 3), rep["status"]
 
 
+_HOST_CPUS = None
+
+
 def physical_cores():
-    """Physical cores (distinct (package, core) pairs) among the CPUs this
-    process may run on."""
+    """The value of host_cpus() taken when bench.py started, before an OpenMP
+    runtime bound this thread to one place."""
+    return _HOST_CPUS
+
+
+def host_cpus():
+    """Physical cores (distinct (socket, core) pairs of `lscpu -p`, or of
+    /proc/cpuinfo) among the CPUs this process may run on, with the count of
+    those CPUs: {"physical": P, "logical_allowed": L}."""
+    import subprocess
     try:
         allowed = os.sched_getaffinity(0)
-        cores, cpu, phys = set(), None, None
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                k, _, v = line.partition(":")
-                k = k.strip()
-                if k == "processor":
-                    cpu = int(v)
-                elif k == "physical id":
-                    phys = int(v)
-                elif k == "core id" and cpu in allowed:
-                    cores.add((phys, int(v)))
-        return len(cores) or len(allowed)
-    except (OSError, ValueError):
+    except OSError:
         return None
+    cores = set()
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True,
+                             timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("#"):
+                continue
+            cpu, core, sock = (int(v) if v else -1 for v in line.split(","))
+            if cpu in allowed:
+                cores.add((sock, core))
+    except (OSError, ValueError, subprocess.SubprocessError):
+        cores = set()
+    return {"physical": len(cores) or None, "logical_allowed": len(allowed)}
 
 
 def cpu_threads():
@@ -322,7 +340,7 @@ def cpu_baseline(w, bmean, rhs, x0, scale_iters, scale_what, iters):
         "value": round(1.0 / (dt * scale_iters), 5),
         "unit": "solves/s",
         "cores": pyoracle.num_threads(fast=True),
-        "physical_cores_available": physical_cores(),
+        "host_cpus": physical_cores(),
         "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
         "kind": "port",
         "sample": (f"pipelined BiCGSTAB + block-CG CPU solver on the full {w.ncells}-cell "
@@ -387,7 +405,7 @@ def bench_cpu_full(args):
     dev = torch.device("cuda:0")
     out = {"metric": "CPU baseline: complete solves at ||b-Ax||/||b|| <= %g" % args.tol,
            "cores": pyoracle.num_threads(fast=True),
-           "physical_cores_available": physical_cores(),
+           "host_cpus": physical_cores(),
            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND",
                                                   "OMP_PLACES")},
            "kind": "port", "workloads": {}}
@@ -551,7 +569,7 @@ def bench_diffusion(args):
         out["cpu_baseline"] = {
             "value": round(1.0 / dcpu, 5), "unit": "steps/s",
             "cores": pyoracle.num_threads(fast=True), "kind": "port",
-            "physical_cores_available": physical_cores(),
+            "host_cpus": physical_cores(),
             "sample": (f"the whole step on the same {w.ncells}-cell inputs by the restated "
                        f"reference algorithm (pipelined BiCGSTAB + per-block CG, "
                        f"DiffusionSolver::solve), median of 3 ({dcpu:.2f} s per step), "
@@ -562,6 +580,8 @@ def bench_diffusion(args):
 
 
 def main():
+    global _HOST_CPUS
+    _HOST_CPUS = host_cpus()
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU (one MI355X per process); the solver has no "
@@ -626,8 +646,13 @@ def main():
               flush=True)
         return S, res
 
-    # the FP64 drop-in, the same iteration cap as every other form
+    # the FP64 drop-in, the same iteration cap as every other form; and the
+    # same with its exact block inverse applied by sine transforms
     S64, res64 = leg("fp64", "fp64")
+    res64_dst = None
+    if args.fp64_dst_steps > 0:
+        _, res64_dst = time_form(w, "fp64", args.bmean, seq[:args.warmup + args.fp64_dst_steps],
+                                 args.warmup, rel, tol=tol, log="fp64_dst", precond="dst")
     legs = {}
     if args.form == "fp64":
         S, res = S64, res64
@@ -638,6 +663,12 @@ def main():
         _, legs["learned_only"] = leg("learned_only", args.form, weights=weights)
         _, legs["extrapolation_only"] = leg("extrapolation_only", args.form, extrapolate=True)
         S, res = leg("headline", args.form, weights=weights, extrapolate=True, profile=True)
+    # the headline form with the exact block inverse applied by sine
+    # transforms in FP32 instead of the FP16/BF16 GEMM (cfg.precond = DST)
+    res_dst = None
+    if args.form != "fp64" and not args.no_dst:
+        _, res_dst = leg("headline_dst", args.form, weights=weights, extrapolate=learned,
+                         precond="dst")
     tight = None
     if args.tight_tol > 0 and args.form != "fp64":
         _, rt = leg("tight", args.form, weights=weights, extrapolate=learned, tol=args.tight_tol)
@@ -767,11 +798,26 @@ def main():
                       "true_rel_resid_max": max(ref_res)},
         "tight": tight,
         "fp64": {**sm64, "max_iter": MAX_ITER,
+                 "ms_per_iteration": round(float(np.mean([r[0] / r[1]["iterations"]
+                                                          for r in res64])), 4),
                  "solves_per_s_converged_steps": (round(1e3 / float(np.mean(fp64_conv_ms)), 4)
                                                   if fp64_conv_ms else None),
                  "gemm_TFLOP/s": round(g64["TFLOP/s"], 2),
                  "gemm_mfma_frac": round(g64["TFLOP/s"] / FP64_MFMA_PEAK, 4),
                  "kernels_us": {KERNELS[k]: round(v["avg_us"], 2) for k, v in k64.items()}},
+        "fp64_dst": ({**summary(res64_dst, tol),
+                      "ms_per_iteration": round(float(np.mean([r[0] / r[1]["iterations"]
+                                                               for r in res64_dst])), 4),
+                      "note": "FP64 drop-in with precond = DST: the same exact block inverse "
+                              "by sine transforms instead of the P_inv GEMM"}
+                     if res64_dst else None),
+        "precond_dst": ({**summary(res_dst, tol),
+                         "speedup_vs_gemm_form": round(ms / float(np.mean([r[0] for r in res_dst])),
+                                                       3),
+                         "note": "the headline form with cfg.precond = DST: the exact block "
+                                 "inverse by sine transforms in FP32 instead of the "
+                                 f"{args.form} GEMM"}
+                        if res_dst else None),
         "speedup_vs_fp64": speedup,
         "speedup_vs_fp64_steps": f"{len(both)}/{len(res)} (both forms converged)",
         "per_iteration_speedup_vs_fp64": round((float(np.mean([r[0] for r in res64])) /
@@ -792,10 +838,14 @@ def main():
         out["learned"] = learn
         out.update(legs_out)
     if not args.no_cpu:
+        # the reference CPU solver stops after 1000 iterations
+        # (PoissonSolverAMR.cpp:127), returning its best iterate
         conv_its = [r[1]["iterations"] for r in res64 if converged(r[1], tol)]
-        scale = (int(round(np.mean(conv_its))) if conv_its else MAX_ITER)
-        what = (f"the FP64 GPU drop-in's mean over its {len(conv_its)} converged steps"
-                if conv_its else "the iteration cap: the FP64 GPU drop-in converged on no step")
+        scale = min(int(round(np.mean(conv_its))), CPU_MAX_ITER) if conv_its else CPU_MAX_ITER
+        what = (f"the FP64 GPU drop-in's mean over its {len(conv_its)} converged steps (the "
+                f"same algorithm), capped at the CPU solver's own 1000" if conv_its else
+                "the CPU solver's own cap of 1000 iterations (PoissonSolverAMR.cpp:127): "
+                "the FP64 GPU drop-in reached the tolerance on no step within 3000")
         out["cpu_baseline"] = cpu_baseline(w, args.bmean, rhs.cpu().numpy(),
                                            p_last.cpu().numpy(), scale, what, args.cpu_iters)
     print(json.dumps(out))

@@ -16,12 +16,16 @@ namespace cup3d {
 // eigenvalues l_k = 2 cos(pi (k+1) / 9) - 2 of the zero-padded 1-D second
 // difference.
 constexpr int DST_TABLE = BS * BS + BS;
-void dst_tables(double* tab);
+void dst_tables(double* tab, float* tab32);  // FP64 table and its FP32 rounding
 
 // mode PREC_P / PREC_S / PREC_PLAIN as launch_precond64; h per block,
 // sigma = 1 / (nu dt).
 void launch_precond_dst(hipStream_t s, int mode, int nb, const double* tab, const double* h,
                         double sigma, const KrylovVectors<double>& v, SolverState* st);
+// The same in FP32 for the mixed forms' inner solve (sigma = 0: the pressure
+// equation's block inverse h^-1 P_inv).
+void launch_precond_dst32(hipStream_t s, int mode, int nb, const float* tab, const float* h,
+                          const KrylovVectors<float>& v, SolverState* st);
 // u = vel[:, comp], a = -h^3 sigma vel_old[:, comp] (vel, vel_old: [n][3]).
 void launch_diffusion_gather(hipStream_t s, long long n, int comp, const double* h3, double sigma,
                              const double* vel, const double* vel_old, double* u, double* a);

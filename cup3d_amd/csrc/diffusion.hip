@@ -33,17 +33,17 @@
 namespace cup3d {
 namespace {
 
-constexpr int DL_ROW = 9;               // padded row (ix) stride, doubles
+constexpr int DL_ROW = 9;               // padded row (ix) stride, values
 constexpr int DL_PLANE = BS * DL_ROW;   // one z plane (or z mode)
 constexpr int DL_WAVE = BS * DL_PLANE;  // one wavefront's slice
 
 // out[k] = sum_i S[k][i] in[i] (S symmetric: also the inverse transform);
 // the table index is the same in every lane (scalar loads)
-__device__ __forceinline__ void dst8(const double* __restrict__ S, const double (&in)[BS],
-                                     double (&out)[BS]) {
+template <typename T>
+__device__ __forceinline__ void dst8(const T* __restrict__ S, const T (&in)[BS], T (&out)[BS]) {
 #pragma unroll
   for (int k = 0; k < BS; ++k) {
-    double acc = 0.0;
+    T acc = 0;
 #pragma unroll
     for (int i = 0; i < BS; ++i) acc = fma(S[k * BS + i], in[i], acc);
     out[k] = acc;
@@ -54,41 +54,42 @@ __device__ __forceinline__ void dst8(const double* __restrict__ S, const double 
 // slice L: this lane (a, c) = (lane & 7, lane >> 3) produces, for every z
 // plane k, the value at output index c (y) or a (x); row holds S[c][*] or
 // S[a][*] in registers.
-template <int AXIS>
-__device__ __forceinline__ void dst_lds(const double (&row)[BS], const double* L, int lane,
-                                        double (&out)[BS]) {
+template <int AXIS, typename T>
+__device__ __forceinline__ void dst_lds(const T (&row)[BS], const T* L, int lane, T (&out)[BS]) {
   const int a = lane & 7, c = lane >> 3;
 #pragma unroll
   for (int k = 0; k < BS; ++k) {
-    double acc = 0.0;
+    T acc = 0;
 #pragma unroll
     for (int j = 0; j < BS; ++j) {
-      const double v = AXIS == 1 ? L[k * DL_PLANE + j * DL_ROW + a] : L[k * DL_PLANE + c * DL_ROW + j];
+      const T v = AXIS == 1 ? L[k * DL_PLANE + j * DL_ROW + a] : L[k * DL_PLANE + c * DL_ROW + j];
       acc = fma(row[j], v, acc);
     }
     out[k] = acc;
   }
 }
 
-__device__ __forceinline__ void to_lds(double* L, int lane, const double (&v)[BS]) {
+template <typename T>
+__device__ __forceinline__ void to_lds(T* L, int lane, const T (&v)[BS]) {
   const int a = lane & 7, c = lane >> 3;
 #pragma unroll
   for (int k = 0; k < BS; ++k) L[k * DL_PLANE + c * DL_ROW + a] = v[k];
 }
 
 // z = h^-1 A_loc(sigma_b)^-1 v for the column v of this lane (in place).
-// tab: S1 (64 doubles, row-major) then l_0..l_7.
-__device__ __forceinline__ void dst_apply(const double* __restrict__ tab, double* L, int lane,
-                                          double h, double sigma, double (&v)[BS]) {
+// tab: S1 (64 values, row-major) then l_0..l_7.
+template <typename T>
+__device__ __forceinline__ void dst_apply(const T* __restrict__ tab, T* L, int lane, T h,
+                                          T sigma, T (&v)[BS]) {
   const int kx = lane & 7, ky = lane >> 3;
-  double rx[BS], ry[BS];
+  T rx[BS], ry[BS];
 #pragma unroll
   for (int j = 0; j < BS; ++j) {
     rx[j] = tab[kx * BS + j];
     ry[j] = tab[ky * BS + j];
   }
-  const double base = tab[BS * BS + kx] + tab[BS * BS + ky] - h * h * sigma;
-  double w[BS];
+  const T base = tab[BS * BS + kx] + tab[BS * BS + ky] - h * h * sigma;
+  T w[BS];
   dst8(tab, v, w);  // z transform: w[kz] at (ix, iy)
   to_lds(L, lane, w);
   wave_sync();
@@ -110,29 +111,29 @@ __device__ __forceinline__ void dst_apply(const double* __restrict__ tab, double
   dst8(tab, w, v);             // inverse z: v[iz]
 }
 
-// K1 / K3 of the classical iteration (kernels.hip) with this preconditioner:
+// K1 / K3 of the classical iteration (kernels.hip) with this preconditioner,
+// in FP64 (the FP64 forms) or FP32 (the inner solve of the mixed forms):
 //   PREC_P  p = r + beta (p - omega nu) [p = r after a restart, rhat = r];
 //           z1 = M^-1 p
 //   PREC_S  s = r - alpha nu (into r); z2 = M^-1 s
 //   PREC_PLAIN  z1 = M^-1 p
-template <int MODE>
-__global__ __launch_bounds__(WG_THREADS) void k_precond_dst(int nb,
-                                                            const double* __restrict__ tab,
-                                                            const double* __restrict__ hb,
-                                                            double sigma, KrylovVectors<double> v,
+template <typename T, int MODE>
+__global__ __launch_bounds__(WG_THREADS) void k_precond_dst(int nb, const T* __restrict__ tab,
+                                                            const T* __restrict__ hb, T sigma,
+                                                            KrylovVectors<T> v,
                                                             SolverState* st) {
-  __shared__ double Ls[WG_THREADS / 64][DL_WAVE];
+  __shared__ T Ls[WG_THREADS / 64][DL_WAVE];
   if (MODE != PREC_PLAIN && st->status != ST_RUNNING) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int b = blockIdx.x * (WG_THREADS / 64) + w;
   if (b >= nb) return;  // no workgroup barrier below: only this wavefront's slice
-  const double beta = MODE == PREC_P ? st->beta : 0.0;
-  const double omega = MODE == PREC_P ? st->omega : 0.0;
-  const double alpha = MODE == PREC_S ? st->alpha : 0.0;
+  const T beta = MODE == PREC_P ? (T)st->beta : (T)0;
+  const T omega = MODE == PREC_P ? (T)st->omega : (T)0;
+  const T alpha = MODE == PREC_S ? (T)st->alpha : (T)0;
   const int pzero = MODE == PREC_P ? st->p_zero : 0;
   const int restart = MODE == PREC_P ? st->restart_pending : 0;
   const size_t off = (size_t)b * BLEN + lane;
-  double c[BS], ra[BS], rb[BS];
+  T c[BS], ra[BS], rb[BS];
 #pragma unroll
   for (int iz = 0; iz < BS; ++iz) {
     const size_t i = off + iz * 64;
@@ -160,10 +161,21 @@ __global__ __launch_bounds__(WG_THREADS) void k_precond_dst(int nb,
       v.r[i] = c[iz];
     }
   }
-  dst_apply(tab, Ls[w], lane, hb[b], sigma, c);
-  double* __restrict__ out = MODE == PREC_S ? v.z2 : v.z1;
+  dst_apply<T>(tab, Ls[w], lane, hb[b], sigma, c);
+  T* __restrict__ out = MODE == PREC_S ? v.z2 : v.z1;
 #pragma unroll
   for (int iz = 0; iz < BS; ++iz) out[off + iz * 64] = c[iz];
+}
+
+template <typename T>
+void launch_precond_dst_t(hipStream_t s, int mode, int nb, const T* tab, const T* h, T sigma,
+                          const KrylovVectors<T>& v, SolverState* st) {
+  const int nwg = (nb + WG_THREADS / 64 - 1) / (WG_THREADS / 64);
+  switch (mode) {
+    case PREC_P: k_precond_dst<T, PREC_P><<<nwg, WG_THREADS, 0, s>>>(nb, tab, h, sigma, v, st); break;
+    case PREC_S: k_precond_dst<T, PREC_S><<<nwg, WG_THREADS, 0, s>>>(nb, tab, h, sigma, v, st); break;
+    default: k_precond_dst<T, PREC_PLAIN><<<nwg, WG_THREADS, 0, s>>>(nb, tab, h, sigma, v, st); break;
+  }
 }
 
 // Component c of the velocity into the solver's vectors: u = u*_c and
@@ -193,23 +205,24 @@ __global__ __launch_bounds__(WG_THREADS) void k_diff_scatter(long long n, int co
 
 }  // namespace
 
-void dst_tables(double* tab) {
+void dst_tables(double* tab, float* tab32) {
   const double pi = 3.14159265358979323846;
   for (int k = 0; k < BS; ++k) {
     tab[BS * BS + k] = 2.0 * std::cos(pi * (k + 1) / 9.0) - 2.0;
     for (int i = 0; i < BS; ++i)
       tab[k * BS + i] = std::sqrt(2.0 / 9.0) * std::sin(pi * (k + 1) * (i + 1) / 9.0);
   }
+  for (int i = 0; i < DST_TABLE; ++i) tab32[i] = (float)tab[i];
 }
 
 void launch_precond_dst(hipStream_t s, int mode, int nb, const double* tab, const double* h,
                         double sigma, const KrylovVectors<double>& v, SolverState* st) {
-  const int nwg = (nb + WG_THREADS / 64 - 1) / (WG_THREADS / 64);
-  switch (mode) {
-    case PREC_P: k_precond_dst<PREC_P><<<nwg, WG_THREADS, 0, s>>>(nb, tab, h, sigma, v, st); break;
-    case PREC_S: k_precond_dst<PREC_S><<<nwg, WG_THREADS, 0, s>>>(nb, tab, h, sigma, v, st); break;
-    default: k_precond_dst<PREC_PLAIN><<<nwg, WG_THREADS, 0, s>>>(nb, tab, h, sigma, v, st); break;
-  }
+  launch_precond_dst_t<double>(s, mode, nb, tab, h, sigma, v, st);
+}
+
+void launch_precond_dst32(hipStream_t s, int mode, int nb, const float* tab, const float* h,
+                          const KrylovVectors<float>& v, SolverState* st) {
+  launch_precond_dst_t<float>(s, mode, nb, tab, h, 0.0f, v, st);
 }
 
 void launch_diffusion_gather(hipStream_t s, long long n, int comp, const double* h3, double sigma,

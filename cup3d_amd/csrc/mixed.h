@@ -24,6 +24,11 @@ struct MixedWorkspace {
   int nb = 0;
   int timing_precision = 1;
   KrylovVectors<float> v32{};
+  // cfg.precond = DST: the FP32 inner solve applies the exact block inverse
+  // by sine transforms (table dst32, DST_TABLE floats) instead of the
+  // FP16/BF16 GEMM
+  bool use_dst = false;
+  const float* dst32 = nullptr;
   // in-solve kernel timing: HIP events around the main kernels of K1 .. K5
   // (the finalize kernels excluded) of the middle iteration of every polled
   // batch, summed while prof is set

@@ -38,6 +38,7 @@
 #include "device_common.h"
 #include "kernels.h"
 #include "krylov.h"
+#include "diffusion.h"
 #include "mixed.h"
 
 namespace cup3d {
@@ -466,6 +467,15 @@ void launch_precond_h(hipStream_t s, int mode, int nb, const void* Ppk, float un
 
 void launch_precond_mixed(const MixedWorkspace& M, hipStream_t s, int precision, int mode,
                           const DevGrid<float>& g, const float* pin, float* pout, SolverState* st) {
+  if (M.use_dst) {
+    KrylovVectors<float> v = M.v32;
+    if (mode == PREC_PLAIN) {
+      v.p = const_cast<float*>(pin);
+      v.z1 = pout;
+    }
+    launch_precond_dst32(s, mode, g.nb, M.dst32, g.h, v, st);
+    return;
+  }
   if (precision == CUP3D_PREC_MIXED_FP16)
     launch_precond_h<_Float16>(s, mode, g.nb, M.d_P16, 1.0f / P16_SCALE, M.split, g.invh, M.v32,
                                pin, pout, st);
@@ -796,7 +806,8 @@ int mixed_time_kernel(MixedWorkspace& M, hipStream_t s, int which, int reps,
   // 2 x 512 flops per cell, two passes with the split input
   const double words[5] = {2, 2, 8, 5, 3};
   const double passes = M.split ? 2.0 : 1.0;
-  const double fpc[5] = {8, 1024 * passes, 8, 1024 * passes + 4, 10};
+  const double pf = M.use_dst ? 96.0 : 1024 * passes;  // sine transforms: 96 flops per cell
+  const double fpc[5] = {8, pf, 8, pf + 4, 10};
   *bytes = 4.0 * words[which] * N;
   *flops = fpc[which] * N;
   return 0;

@@ -105,7 +105,8 @@ struct cup3d_poisson {
   // preconditioner
   std::vector<double> P;
   DevBuf d_P64;
-  DevBuf d_dst;  // implicit diffusion's DST block-inverse table (DST_TABLE doubles)
+  DevBuf d_dst;    // DST block-inverse table (DST_TABLE doubles), for cfg.precond = DST
+  DevBuf d_dst32;  // its FP32 rounding (the mixed forms' inner solve) and implicit diffusion
 
   // FP64 Krylov workspace
   DevBuf d_vec64;  // 11 vectors of n doubles
@@ -177,6 +178,18 @@ int upload(cup3d_poisson* H, DevBuf& d, const void* src, size_t bytes) {
   return 0;
 }
 
+// M^-1 of the FP64 form: the P_inv GEMM on the FP64 matrix cores, or (Helmholtz
+// grids, or cfg.precond = DST) the exact block inverse by sine transforms.
+void launch_precond_fp64(cup3d_poisson* H, const DevGrid<double>& g, int mode,
+                         const KrylovVectors<double>& v) {
+  SolverState* st = H->d_state.as<SolverState>();
+  if (g.helm || H->cfg.precond == CUP3D_PRECOND_DST)
+    launch_precond_dst(H->stream, mode, H->nb, H->d_dst.as<double>(), g.h,
+                       g.helm ? g.helm_sigma : 0.0, v, st);
+  else
+    launch_precond64(H->stream, mode, H->nb, H->d_P64.as<double>(), g.invh, v, st);
+}
+
 // Run the FP64 classical BiCGSTAB on the device. b64/x64 must already hold
 // getVec's b and x0 (in v64.b and v64.x[0]); the state must be initialised.
 // With a Helmholtz grid (g.helm, implicit diffusion) the preconditioner is
@@ -186,12 +199,7 @@ int run_bicgstab64(cup3d_poisson* H, const DevGrid<double>& g) {
   SolverState* st = H->d_state.as<SolverState>();
   double* part = H->d_part.as<double>();
   const KrylovVectors<double>& v = H->v64;
-  auto precond = [&](int mode) {
-    if (g.helm)
-      launch_precond_dst(s, mode, H->nb, H->d_dst.as<double>(), g.h, g.helm_sigma, v, st);
-    else
-      launch_precond64(s, mode, H->nb, H->d_P64.as<double>(), g.invh, v, st);
-  };
+  auto precond = [&](int mode) { launch_precond_fp64(H, g, mode, v); };
   // r = b - A x0; rhat = r (BiCGSTAB.cu:435-467)
   launch_spmv64(s, SPMV_INIT, g, v.x[0], v.nu, v.b, v.r, v.rhat, st, part);
   const int poll = H->cfg.poll_every > 0 ? H->cfg.poll_every : 8;
@@ -235,6 +243,7 @@ int cup3d_poisson_create(const cup3d_poisson_config* cfg, cup3d_poisson_t* out) 
   if (c.precision < 0 || c.precision > 2) { g_create_error = "unknown precision"; return -1; }
   if (c.guess < 0 || c.guess > 2) { g_create_error = "unknown guess mode"; return -1; }
   if (c.extrapolate < 0 || c.extrapolate > 1) { g_create_error = "extrapolate must be 0 or 1"; return -1; }
+  if (c.precond < 0 || c.precond > 1) { g_create_error = "precond must be 0 (GEMM) or 1 (DST)"; return -1; }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     g_create_error = "no HIP device available";
@@ -246,6 +255,7 @@ int cup3d_poisson_create(const cup3d_poisson_config* cfg, cup3d_poisson_t* out) 
   if (H->cfg.inner_tol <= 0) H->cfg.inner_tol = 1e-7;
   if (H->cfg.max_refine <= 0) H->cfg.max_refine = 10;
   H->mixed.split = c.gemm_passes != 1;
+  H->mixed.use_dst = c.precond == CUP3D_PRECOND_DST;
   H->device = c.device;
   if (hipSetDevice(c.device) != hipSuccess || hipStreamCreate(&H->stream) != hipSuccess ||
       hipEventCreate(&H->ev0) != hipSuccess || hipEventCreate(&H->ev1) != hipSuccess ||
@@ -258,13 +268,16 @@ int cup3d_poisson_create(const cup3d_poisson_config* cfg, cup3d_poisson_t* out) 
   std::vector<double> pk;
   pack_pinv_f64(H->P, pk);
   double dtab[DST_TABLE];
-  dst_tables(dtab);
+  float dtab32[DST_TABLE];
+  dst_tables(dtab, dtab32);
   if (upload(H.get(), H->d_P64, pk.data(), pk.size() * sizeof(double)) != 0 ||
       upload(H.get(), H->d_dst, dtab, sizeof(dtab)) != 0 ||
+      upload(H.get(), H->d_dst32, dtab32, sizeof(dtab32)) != 0 ||
       mixed_init_pinv(H->mixed, H->P, H->err) != 0 || cnn_init(H->cnn, H->err) != 0) {
     g_create_error = H->err;
     return -1;
   }
+  H->mixed.dst32 = H->d_dst32.as<float>();
   if (H->d_state.alloc(sizeof(SolverState)) != hipSuccess) {
     g_create_error = "hipMalloc(state) failed";
     return -1;
@@ -693,8 +706,7 @@ int cup3d_poisson_apply_preconditioner(cup3d_poisson_t H, const double* d_p, dou
     KrylovVectors<double> v = H->v64;
     v.p = const_cast<double*>(d_p);
     v.z1 = d_z;
-    launch_precond64(H->stream, PREC_PLAIN, H->nb, H->d_P64.as<double>(), H->g64.invh, v,
-                     H->d_state.as<SolverState>());
+    launch_precond_fp64(H, H->g64, PREC_PLAIN, v);
   } else {
     if (mixed_apply_precond(H->mixed, H->stream, H->cfg.precision, H->g32, d_p, d_z, H->err) != 0)
       return -1;
@@ -1064,9 +1076,9 @@ int cup3d_debug_time_kernel(cup3d_poisson_t H, int which, int reps, double* avg_
     auto launch = [&]() {
       switch (which) {
         case 0: launch_spmv64(s, SPMV_PLAIN, H->g64, v.z1, v.nu, nullptr, nullptr, nullptr, st, part); break;
-        case 1: launch_precond64(s, PREC_PLAIN, H->nb, H->d_P64.as<double>(), H->g64.invh, v, st); break;
+        case 1: launch_precond_fp64(H, H->g64, PREC_PLAIN, v); break;
         case 2: launch_update64(s, H->n, v, st, part); break;
-        case 3: launch_precond64(s, PREC_P, H->nb, H->d_P64.as<double>(), H->g64.invh, v, st); break;
+        case 3: launch_precond_fp64(H, H->g64, PREC_P, v); break;
         default: launch_spmv64(s, SPMV_DOT1, H->g64, v.z1, v.nu, v.rhat, nullptr, nullptr, st, part); break;
       }
     };
@@ -1079,7 +1091,8 @@ int cup3d_debug_time_kernel(cup3d_poisson_t H, int which, int reps, double* avg_
     HIPCHECK(hipEventElapsedTime(&ms, H->ev0, H->ev1));
     *avg_ms = ms / reps;
     const double words[5] = {2, 2, 8, 5, 3};
-    const double fpc[5] = {8, 1024, 8, 1024 + 4, 10};
+    const bool dst = H->cfg.precond == CUP3D_PRECOND_DST;
+    const double fpc[5] = {8, dst ? 96.0 : 1024.0, 8, dst ? 96.0 + 4 : 1024.0 + 4, 10};
     by = 8.0 * words[which] * N;
     fl = fpc[which] * N;
   }

@@ -27,7 +27,7 @@ class Config(ctypes.Structure):
                 ("precision", ctypes.c_int), ("guess", ctypes.c_int), ("device", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("max_refine", ctypes.c_int),
                 ("poll_every", ctypes.c_int), ("gemm_passes", ctypes.c_int),
-                ("extrapolate", ctypes.c_int)]
+                ("extrapolate", ctypes.c_int), ("precond", ctypes.c_int)]
 
 
 class BlockDesc(ctypes.Structure):
@@ -124,7 +124,7 @@ class PoissonSolver:
     def __init__(self, bpd, h0, bc, level_max=0, tol_abs=0.0, tol_rel=1e-7, tol_rhs=0.0,
                  bmean=1, max_iter=1000, max_restarts=100, precision="fp64", guess="given",
                  device=0, inner_tol=1e-7, max_refine=10, poll_every=8, gemm_passes=3,
-                 extrapolate=False):
+                 extrapolate=False, precond="gemm"):
         self.L = lib()
         c = Config()
         for d in range(3):
@@ -139,6 +139,7 @@ class PoissonSolver:
         c.poll_every = int(poll_every)
         c.gemm_passes = int(gemm_passes)
         c.extrapolate = int(bool(extrapolate))
+        c.precond = {"gemm": 0, "dst": 1}[precond] if isinstance(precond, str) else int(precond)
         self.cfg = c
         h = _vp()
         if self.L.cup3d_poisson_create(ctypes.byref(c), ctypes.byref(h)) != 0:

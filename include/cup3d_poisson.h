@@ -105,7 +105,16 @@ typedef struct {
                         passes, linear to FP32 precision; 1 = one FP16/BF16 pass */
   int extrapolate;   /* 0 = off (the reference contract), 1 = also extrapolate the guess
                         in time from the previous solve's (see above) */
+  int precond;       /* how M^-1 = blockdiag(h_b^-1 P_inv) is applied.
+                        CUP3D_PRECOND_GEMM (0): P_inv x block matrix on the matrix cores
+                        (FP64, as the reference, BiCGSTAB.cu:526-528; the mixed forms:
+                        FP16/BF16 with FP32 accumulate); CUP3D_PRECOND_DST (1): the same
+                        exact inverse by 8-point sine transforms (P_inv is diagonalised
+                        by the 3-D DST) in the form's own precision (FP64, or FP32 in the
+                        mixed forms' inner solve), bound by HBM, not the matrix cores. */
 } cup3d_poisson_config;
+
+enum { CUP3D_PRECOND_GEMM = 0, CUP3D_PRECOND_DST = 1 };
 
 typedef struct {
   int level;     /* BlockInfo::level */

@@ -92,6 +92,38 @@ def test_solve_bmean0_same_iterations(gpu, name):
     assert abs(rep["err_init"] - info["err_init"]) <= 1e-13 * info["err_init"]
 
 
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "amr3_walls", "amr2_periodic_offcentre",
+                                  "school4_small"])
+def test_dst_preconditioner_matches_oracle(gpu, name):
+    """precond = DST: the exact block inverse by sine transforms equals the
+    oracle's Cholesky P_inv product (PoissonSolverExp.cpp:17-111, BiCGSTAB.cu:
+    526-528) to rounding."""
+    blocks, bpd, h0, bc = CASES[name]
+    S = PoissonSolver(bpd, h0, bc, bmean=1, precond="dst").set_topology(blocks)
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 1)
+    p = random_field(len(blocks), 2).reshape(-1)
+    z = S.apply_preconditioner(dev(p), torch.zeros(O.n, dtype=torch.float64, device=gpu))
+    zo = O.precond(p)
+    assert np.abs(z.cpu().numpy() - zo).max() <= 1e-13 * np.abs(zo).max()
+
+
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "uniform_periodic_2x3x2", "amr2_walls",
+                                  "amr3_walls", "tiny_amr_periodic"])
+def test_dst_solve_bmean0_same_iterations(gpu, name):
+    """The FP64 drop-in with precond = DST: the reference's iteration, same
+    count and iterate as the oracle's (which multiplies by P_inv)."""
+    blocks, bpd, h0, bc = CASES[name]
+    S = PoissonSolver(bpd, h0, bc, bmean=0, tol_rel=1e-7, precond="dst").set_topology(blocks)
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
+    b = compatible_rhs(blocks, h0, 3)
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(dev(b), x)
+    xo, info = O.bicgstab(b, tol_rel=1e-7)
+    assert rep["status"] == info["status"] == 0
+    assert rep["iterations"] == info["iters"] and rep["restarts"] == info["restarts"]
+    assert np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) <= 1e-10
+
+
 def test_solve_bmean0_school4_small(gpu):
     """On the 4-level school grid the recurrence amplifies rounding even at
     bMean = 0: the oracle against itself under 1e-15 rhs perturbations needs
@@ -370,11 +402,14 @@ def test_c5_full_size_first_iterations_match_oracle(gpu):
     30.2 M cells, bMean = 0, step-0 rhs, zero guess) against the oracle's
     classical BiCGSTAB on the same system (tests/golden/c5_fp64_fixture.json,
     made by tests/golden/make_c5_fp64_fixture.py: about two minutes of CPU
-    for 30 iterations): the same initial residual, the same best residual
-    after 10 iterations to rounding and the same best iterate at 64 cells;
-    after 30 iterations the recurrences have drifted apart by the rounding
-    amplification this grid shows (the school4 grid's 1e-9 to 1e-8 after 20
-    to 40 iterations under 1e-15 perturbations), within 1e-6."""
+    for 30 iterations). The oracle sums its norms and dots over the 30.2 M
+    cells one after the other, whose rounding is bounded by n eps = 3.4e-9
+    (measured on the initial residual: 8e-11), the GPU in a fixed tree: the
+    initial residual and, after 10 iterations, the best residual agree to
+    5e-9 and the best iterate at 64 cells to 1e-8; after 30 iterations the
+    recurrence has amplified these differences as on the school4 grid
+    (1e-15 perturbations grow to 1e-9 - 1e-8 by iterations 20 - 40), within
+    1e-6."""
     import json
     import os
     fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
@@ -382,17 +417,17 @@ def test_c5_full_size_first_iterations_match_oracle(gpu):
     w = W.c5_amr_school()
     assert w.ncells == fx["cells"]
     b = dev(w.rhs(step=0).reshape(-1))
-    for k, bar in ((10, 1e-10), (30, 1e-6)):
+    for k, bar in ((10, 5e-9), (30, 1e-6)):
         S = solver_for(w, tol_rel=0.0, bmean=0, precision="fp64", max_iter=k)
         x = torch.zeros(w.ncells, dtype=torch.float64, device=gpu)
         rep = S.solve_device(b, x)
         best = min(fx["history"][:k])
         print(f"k={k}: GPU best residual {rep['err_final']:.12e}, oracle {best:.12e}")
         assert rep["iterations"] == k
-        assert abs(rep["err_init"] - fx["err_init"]) <= 1e-12 * fx["err_init"]
+        assert abs(rep["err_init"] - fx["err_init"]) <= 5e-9 * fx["err_init"]
         assert abs(rep["err_final"] - best) <= bar * best
         if k == 10:
             got = x[torch.tensor(fx["x10_index"], device=gpu)].cpu().numpy()
             want = np.asarray(fx["x10_values"])
-            assert np.abs(got - want).max() <= 1e-9 * np.abs(want).max()
+            assert np.abs(got - want).max() <= 1e-8 * np.abs(want).max()
         del S, x

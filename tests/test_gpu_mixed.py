@@ -274,3 +274,42 @@ def test_split_preconditioner_is_linear(gpu, prec):
     # hi + lo keeps 22 significant bits in FP16, 16 in BF16
     assert res[3] < {"mixed_fp16": 3e-6, "mixed_bf16": 2e-5}[prec], res
     assert res[1] > 10 * res[3], res
+
+
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "amr3_walls", "amr2_periodic_offcentre",
+                                  "school4_small"])
+def test_dst_fp32_preconditioner_close_to_fp64(gpu, name):
+    """precond = DST in the mixed forms: the exact block inverse by sine
+    transforms in FP32, per block within FP32 rounding of the FP64 P_inv
+    product (relative L2 2e-6), over blocks whose magnitudes span 15
+    decades (the transform is linear: no per-block scaling needed)."""
+    blocks, bpd, h0, bc = CASES[name]
+    S = PoissonSolver(bpd, h0, bc, precision="mixed_fp16", precond="dst").set_topology(blocks)
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 1)
+    p = random_field(len(blocks), 2).reshape(len(blocks), 512)
+    p *= np.logspace(-12, 3, len(blocks))[:, None]
+    z = S.apply_preconditioner(dev(p), torch.zeros(O.n, dtype=torch.float64, device=gpu))
+    zo = O.precond(p.reshape(-1)).reshape(len(blocks), 512)
+    zg = z.cpu().numpy().reshape(len(blocks), 512)
+    rel = np.linalg.norm(zg - zo, axis=1) / np.linalg.norm(zo, axis=1)
+    assert rel.max() < 2e-6, rel.max()
+
+
+@pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "amr3_walls", "tiny_amr3_walls"])
+def test_mixed_dst_solve_reaches_fp64_tolerance(gpu, name):
+    """The mixed form with the FP32 sine-transform preconditioner: FP64
+    refinement to a true relative residual of 1e-10, within 1e-7 of the FP64
+    oracle's solution at that residual."""
+    blocks, bpd, h0, bc = CASES[name]
+    b = compatible_rhs(blocks, h0, 3)
+    S = PoissonSolver(bpd, h0, bc, precision="mixed_fp16", precond="dst", tol_rel=0.0,
+                      tol_rhs=1e-10, inner_tol=1e-4).set_topology(blocks)
+    x = torch.zeros(len(blocks) * 512, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(dev(b), x)
+    assert rep["status"] == 0 and rep["true_rel_resid"] <= 1e-10
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 1)
+    bb = b.copy()
+    bb[O.bmean_row] = 0.0
+    xo, info = O.bicgstab(b, tol_abs=1e-10 * np.linalg.norm(bb), tol_rel=0.0)
+    err = np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo)
+    assert err < 1e-7, err
