@@ -401,15 +401,17 @@ def test_c5_full_size_first_iterations_match_oracle(gpu):
     """The FP64 drop-in on the full 4-level AMR school grid (59 012 blocks,
     30.2 M cells, bMean = 0, step-0 rhs, zero guess) against the oracle's
     classical BiCGSTAB on the same system (tests/golden/c5_fp64_fixture.json,
-    made by tests/golden/make_c5_fp64_fixture.py: about two minutes of CPU
-    for 30 iterations). The oracle sums its norms and dots over the 30.2 M
-    cells one after the other, whose rounding is bounded by n eps = 3.4e-9
-    (measured on the initial residual: 8e-11), the GPU in a fixed tree: the
-    initial residual and, after 10 iterations, the best residual agree to
-    5e-9 and the best iterate at 64 cells to 1e-8; after 30 iterations the
-    recurrence has amplified these differences as on the school4 grid
-    (1e-15 perturbations grow to 1e-9 - 1e-8 by iterations 20 - 40), within
-    1e-6."""
+    made by tests/golden/make_c5_fp64_fixture.py, about ten minutes of CPU).
+    The oracle sums its norms and dots over the 30.2 M cells one after the
+    other, whose rounding is bounded by n eps = 3.4e-9 (measured on the
+    initial residual: 8e-11), the GPU in a fixed tree: the initial residual
+    and, after 10 iterations, the best residual agree to 5e-9 and the best
+    iterate at 64 cells to 1e-8. Later the recurrence amplifies rounding: the
+    fixture holds the oracle's own residual history on the rhs perturbed by
+    1e-15 and 1e-11 relative, which moves the best residual by 2.6e-8 / 4.6e-8
+    after 20 iterations and 4.0e-4 / 1.1e-3 after 30. After 20 and 30
+    iterations the GPU must stay within ten times the larger of these two
+    spreads, or 1e-6 if that is larger."""
     import json
     import os
     fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
@@ -417,12 +419,20 @@ def test_c5_full_size_first_iterations_match_oracle(gpu):
     w = W.c5_amr_school()
     assert w.ncells == fx["cells"]
     b = dev(w.rhs(step=0).reshape(-1))
-    for k, bar in ((10, 5e-9), (30, 1e-6)):
+    hist = fx["history"]
+
+    def spread(k):  # the oracle against itself under the perturbations
+        best = min(hist[:k])
+        return max(abs(min(h[:k]) - best) / best for h in fx["history_perturbed"].values())
+
+    for k in (10, 20, 30):
         S = solver_for(w, tol_rel=0.0, bmean=0, precision="fp64", max_iter=k)
         x = torch.zeros(w.ncells, dtype=torch.float64, device=gpu)
         rep = S.solve_device(b, x)
-        best = min(fx["history"][:k])
-        print(f"k={k}: GPU best residual {rep['err_final']:.12e}, oracle {best:.12e}")
+        best = min(hist[:k])
+        bar = 5e-9 if k == 10 else max(1e-6, 10 * spread(k))
+        print(f"k={k}: GPU best residual {rep['err_final']:.12e}, oracle {best:.12e}, "
+              f"oracle spread {spread(k):.2e}, bar {bar:.2e}")
         assert rep["iterations"] == k
         assert abs(rep["err_init"] - fx["err_init"]) <= 5e-9 * fx["err_init"]
         assert abs(rep["err_final"] - best) <= bar * best
