@@ -32,8 +32,9 @@ iteration cap (3000):
   * tight: the headline form to ||b - A x|| <= 1e-8 ||b||, the residual at
     which the pressure meets the 1e-6 relative-L2 bar.
 Every solve's pressure is compared with the reference solution of its step
-(FP64 true residual <= 1e-9, by the FP64-refined form: the exact discrete
-solution to within that residual).
+(FP64 true residual <= 1e-10, by the FP64-refined form: on the 625-block
+school grid a 1e-10 solve is 8e-9 from the discrete solution, a 1e-9 solve
+2.4e-7).
 
 The line carries the roofline of the dominant kernel (HIP events on the
 library's stream inside the timed solves) and of the other four, and a CPU
@@ -72,6 +73,7 @@ HBM_PEAK = 8000.0       # GB/s, MI355X
 FP64_MFMA_PEAK = 78.6   # TFLOP/s, dense FP64 matrix
 F16_MFMA_PEAK = 2500.0  # TFLOP/s, dense FP16/BF16 matrix
 MAX_ITER = 3000         # iteration cap of every timed form
+REF_TOL = 1e-10         # true relative residual of the untimed reference solves
 CPU_MAX_ITER = 1000     # the reference CPU solver's cap (PoissonSolverAMR.cpp:127)
 KERNELS = {0: "spmv", 1: "precond_gemm", 2: "update_K5", 3: "precond_update_K1",
            4: "spmv_dot_K2"}
@@ -353,7 +355,7 @@ def cpu_baseline(w, bmean, rhs, x0, scale_iters, scale_what, iters):
 
 def reference_sequence(w, bmean, nsteps, dev, seed=None):
     """Untimed setup: the reference solution of steps 0..nsteps, each from the
-    previous step's pressure, ||b - Ax|| <= 1e-9 ||b|| on the FP64 true
+    previous step's pressure, ||b - Ax|| <= REF_TOL ||b|| on the FP64 true
     residual by the FP64-refined solve; the caller removes the
     volume-weighted mean after every solve (PressureProjection.cpp:404-431),
     so every guess is mean-free. Returns the steps 1..nsteps as dicts (rhs,
@@ -361,8 +363,8 @@ def reference_sequence(w, bmean, nsteps, dev, seed=None):
     import torch
     from cup3d_amd import workloads as W
     from cup3d_amd.poisson import solver_for
-    R = solver_for(w, tol_rel=0.0, tol_rhs=1e-9, bmean=bmean, precision="mixed_fp16",
-                   max_iter=2000, max_refine=8)
+    R = solver_for(w, tol_rel=0.0, tol_rhs=REF_TOL, bmean=bmean, precision="mixed_fp16",
+                   max_iter=2000, max_refine=12)
     p_prev = torch.zeros(w.ncells, dtype=torch.float64, device=dev)
     seq, ref_res, ref_ok = [], [], 0
     for t in range(nsteps + 1):
@@ -792,8 +794,8 @@ def main():
         "solves": sm,
         "pressure_err_vs_reference_max": sm["err_vs_reference_max"],
         "pressure_within_1e-6": bool(sm["err_vs_reference_max"] <= 1e-6),
-        "reference": {"form": "mixed_fp16 refined to ||b-Ax|| <= 1e-9 ||b|| (FP64 true "
-                              "residual): the exact discrete solution to within that residual",
+        "reference": {"form": f"mixed_fp16 refined to ||b-Ax|| <= {REF_TOL:g} ||b|| (FP64 "
+                              "true residual): the discrete solution to within that residual",
                       "converged": f"{ref_ok}/{len(ref_res)}",
                       "true_rel_resid_max": max(ref_res)},
         "tight": tight,

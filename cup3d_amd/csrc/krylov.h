@@ -194,10 +194,21 @@ __device__ __forceinline__ T taylor_sum(F&& get, int k1, int k2, T s1, T s2, T r
 // xs holds block b, pl is this wavefront's 64-value staging plane; nb0 is the
 // face's first partner (the coarse block), nbq the finer block of this
 // lane's quadrant (both prefetched by the caller).
+__device__ __forceinline__ size_t coarse_plane_index(int f, int cb, int lane) {
+  // cell (lane & 7, lane >> 3) of the face layer of coarse block cb that
+  // touches face f of the finer block
+  const int d = f >> 1;
+  const bool minus = (f & 1) == 0;
+  const int st1 = d == 0 ? 8 : (d == 1 ? 64 : 1), st2 = d == 0 ? 64 : (d == 1 ? 1 : 8);
+  const int sd = d == 0 ? 1 : (d == 1 ? 8 : 64);
+  return (size_t)cb * BLEN + (minus ? 7 : 0) * sd + (lane & 7) * st1 + (lane >> 3) * st2;
+}
+
 template <typename T>
 __device__ __forceinline__ void iface_face(const DevGrid<T>& g, const T* __restrict__ x,
                                            const T* xs, T* pl, T* ys, int b, int f, int fk,
-                                           int par, int lane, int nb0, int nbq) {
+                                           int par, int lane, int nb0, int nbq, bool have_plane,
+                                           T plane) {
   const int d = f >> 1;
   const bool minus = (f & 1) == 0;
   const int t1 = d == 0 ? 1 : (d == 1 ? 2 : 0), t2 = d == 0 ? 2 : (d == 1 ? 0 : 1);
@@ -210,9 +221,9 @@ __device__ __forceinline__ void iface_face(const DevGrid<T>& g, const T* __restr
   const T ref = diff ? xc : (T)0;
   T acc = 0;
   if (fk == FACE_COARSE) {
-    // stage the coarse block's face layer, [t2][t1]
-    const int cb = nb0;
-    pl[lane] = x[(size_t)cb * BLEN + (minus ? 7 : 0) * sd + u * st1 + v * st2];
+    // stage the coarse block's face layer, [t2][t1] (prefetched by the
+    // caller for the first two such faces)
+    pl[lane] = have_plane ? plane : x[coarse_plane_index(f, nb0, lane)];
     wave_sync();
     const T h = g.h[b];
     const int c1 = (u >> 1) + (((par >> t1) & 1) ? BS / 2 : 0);
@@ -291,7 +302,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
   __shared__ T pl[IFACE ? 4 : 1][IFACE ? 64 : 1];
   // per wavefront: the face partners fnb[24 b .. 24 b + 23], the six face
   // kinds and the index parities of its block
-  __shared__ int fmeta[IFACE ? 4 : 1][IFACE ? 31 : 1];
+  __shared__ int fmeta[IFACE ? 4 : 1][IFACE ? 32 : 1];
   // with coarse-fine faces the epilogue operand waits in LDS, not registers
   constexpr bool A_LDS = IFACE && NEED_A;
   __shared__ T as[A_LDS ? 4 : 1][A_LDS ? BLEN : 1];
@@ -302,9 +313,13 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
   const bool valid = b < g.nb;
   const size_t off = (size_t)b * BLEN;
   T col[BS], av[BS], hx[BS], hy[BS], zlo = 0, zhi = 0;
-  T h = 0;
-  int meta = 0;
+  T h = 0, pre0 = 0, pre1 = 0;
+  int meta = 0, cf0 = -1, cf1 = -1;
   if (valid) {
+    // face partners, kinds and parities: one word per lane of the block's
+    // record, loaded first and by every lane (lanes 32-63 repeat 0-31), so
+    // that the load is not predicated and its value arrives first
+    if (IFACE) meta = g.bmeta[(size_t)b * 32 + (lane & 31)];
     const int* nb6 = g.nbr + b * 6;
     const int nxm = nb6[0], nxp = nb6[1], nym = nb6[2], nyp = nb6[3], nzm = nb6[4], nzp = nb6[5];
     h = g.h[b];
@@ -314,22 +329,36 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
 #pragma unroll
       for (int iz = 0; iz < BS; ++iz) av[iz] = a[off + iz * 64 + lane];
     }
-    // face halos: a lane is on at most one x face and one y face
+    // face halos: a lane is on at most one x face and one y face. A lane on
+    // a face with a same-level neighbour reads the neighbour's face cell;
+    // every other lane reads its own cell (the zero-flux ghost of a domain
+    // face, unused inside the block). The addresses are selected, not the
+    // loads, so every halo load issues with the column without waiting for it.
     const int nx = ix == 0 ? nxm : (ix == 7 ? nxp : -1);
     const int sx = ix == 0 ? 7 : 0;
     const int ny = iy == 0 ? nym : (iy == 7 ? nyp : -1);
     const int sy = iy == 0 ? 7 : 0;
+    const T* px = x + (nx >= 0 ? (size_t)nx * BLEN + iy * 8 + sx : off + lane);
+    const T* py = x + (ny >= 0 ? (size_t)ny * BLEN + sy * 8 + ix : off + lane);
 #pragma unroll
     for (int iz = 0; iz < BS; ++iz) {
-      hx[iz] = nx >= 0 ? x[(size_t)nx * BLEN + iz * 64 + iy * 8 + sx] : col[iz];
-      hy[iz] = ny >= 0 ? x[(size_t)ny * BLEN + iz * 64 + sy * 8 + ix] : col[iz];
+      hx[iz] = px[iz * 64];
+      hy[iz] = py[iz * 64];
     }
-    zlo = nzm >= 0 ? x[(size_t)nzm * BLEN + 7 * 64 + lane] : col[0];
-    zhi = nzp >= 0 ? x[(size_t)nzp * BLEN + lane] : col[BS - 1];
-    if (IFACE) {  // face kinds and partners, loaded with the column (one per lane)
-      if (lane < 24) meta = g.fnb[(size_t)b * 24 + lane];
-      else if (lane < 30) meta = g.fkind[b * 6 + lane - 24];
-      else if (lane == 30) meta = g.bpar[b];
+    zlo = x[nzm >= 0 ? (size_t)nzm * BLEN + 7 * 64 + lane : off + lane];
+    zhi = x[nzp >= 0 ? (size_t)nzp * BLEN + lane : off + 7 * 64 + lane];
+    if (IFACE) {
+      // the face layers of the coarser neighbours of the first two faces
+      // that have one, loaded now so that their latency overlaps the stencil
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        if (__builtin_amdgcn_readlane(meta, 24 + f) == FACE_COARSE) {
+          if (cf0 < 0) cf0 = f;
+          else if (cf1 < 0) cf1 = f;
+        }
+      }
+      if (cf0 >= 0) pre0 = x[coarse_plane_index(cf0, __builtin_amdgcn_readlane(meta, 4 * cf0), lane)];
+      if (cf1 >= 0) pre1 = x[coarse_plane_index(cf1, __builtin_amdgcn_readlane(meta, 4 * cf1), lane)];
     }
 #pragma unroll
     for (int iz = 0; iz < BS; ++iz) xs[w][iz * 64 + lane] = col[iz];
@@ -337,7 +366,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
 #pragma unroll
       for (int iz = 0; iz < BS; ++iz) as[w][iz * 64 + lane] = av[iz];
     }
-    if (IFACE && lane < 31) fmeta[w][lane] = meta;
+    if (IFACE) fmeta[w][lane & 31] = meta;
   }
   wave_sync();
   T yv[BS];
@@ -347,10 +376,14 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
     for (int iz = 0; iz < BS; ++iz) {
       const T c0 = col[iz];
       const int li = iz * 64 + lane;
-      const T xm = ix > 0 ? xs[w][li - 1] : hx[iz];
-      const T xp = ix < 7 ? xs[w][li + 1] : hx[iz];
-      const T ym = iy > 0 ? xs[w][li - 8] : hy[iz];
-      const T yp = iy < 7 ? xs[w][li + 8] : hy[iz];
+      // every lane reads LDS (a face lane its own cell) and then selects, so
+      // the reads are not predicated one by one
+      const T lxm = xs[w][li - (ix > 0 ? 1 : 0)], lxp = xs[w][li + (ix < 7 ? 1 : 0)];
+      const T lym = xs[w][li - (iy > 0 ? 8 : 0)], lyp = xs[w][li + (iy < 7 ? 8 : 0)];
+      const T xm = ix > 0 ? lxm : hx[iz];
+      const T xp = ix < 7 ? lxp : hx[iz];
+      const T ym = iy > 0 ? lym : hy[iz];
+      const T yp = iy < 7 ? lyp : hy[iz];
       const T zm = iz > 0 ? col[iz - 1] : zlo;
       const T zp = iz < BS - 1 ? col[iz + 1] : zhi;
       if (sizeof(T) == 8)
@@ -393,7 +426,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
         const int fk = __builtin_amdgcn_readfirstlane(fmeta[w][24 + f]);
         if (fk >= FACE_COARSE)
           iface_face<T>(g, x, xs[w], pl[w], ys[w], b, f, fk, par, lane, fmeta[w][4 * f],
-                        fmeta[w][4 * f + fq]);
+                        fmeta[w][4 * f + fq], f == cf0 || f == cf1, f == cf0 ? pre0 : pre1);
       }
     }
     wave_sync();

@@ -87,6 +87,8 @@ struct DevGrid {
   const int8_t* fkind = nullptr;    // [nb*6]
   const int* fnb = nullptr;         // [nb*24]
   const int8_t* bpar = nullptr;     // [nb]
+  // the same per block in one 128-byte record: fnb (24), fkind (6), bpar, 0
+  const int* bmeta = nullptr;       // [nb][32]
   long long bmean_row = -1;         // cell replaced by sum h^3 x (bMean == 1)
   int bmean = 1;
   int has_iface = 0;                // any coarse-fine interface rows

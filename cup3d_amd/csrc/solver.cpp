@@ -79,7 +79,7 @@ struct DevBuf {
 
 // Device topology tables of one generation (topology_gpu.hip).
 struct DevTopo {
-  DevBuf blocks, nbr, fkind, fnb, bpar, h, invh, h3, hf, invhf, keys, vals, info;
+  DevBuf blocks, nbr, fkind, fnb, bpar, h, invh, h3, hf, invhf, meta, keys, vals, info;
 };
 
 struct cup3d_poisson {
@@ -203,21 +203,28 @@ int run_bicgstab64(cup3d_poisson* H, const DevGrid<double>& g) {
   // r = b - A x0; rhat = r (BiCGSTAB.cu:435-467)
   launch_spmv64(s, SPMV_INIT, g, v.x[0], v.nu, v.b, v.r, v.rhat, st, part);
   const int poll = H->cfg.poll_every > 0 ? H->cfg.poll_every : 8;
-  int guard = 0;
+  // Batches of 1, 2, 4, ... iterations up to `poll` between status polls: a
+  // short solve (the implicit-diffusion solves take a few iterations) is not
+  // held up by the host enqueueing the rest of a full batch, whose kernels
+  // would only find the state converged; a long solve polls a few more times.
+  int batch = 1;
+  long long launched = 0;
   while (true) {
-    for (int k = 0; k < poll; ++k) {
+    for (int k = 0; k < batch; ++k) {
       precond(PREC_P);
       launch_spmv64(s, SPMV_DOT1, g, v.z1, v.nu, v.rhat, nullptr, nullptr, st, part);
       precond(PREC_S);
       launch_spmv64(s, SPMV_DOT2, g, v.z2, v.t, v.r, nullptr, nullptr, st, part);
       launch_update64(s, H->n, v, st, part);
     }
+    launched += batch;
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(H->h_state, st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     if (H->h_state->status != ST_RUNNING) break;
-    if (++guard > (H->h_state->max_iter / poll) + 4)
+    if (launched > (long long)H->h_state->max_iter + 4LL * poll)
       return fail(H, "device state machine did not stop");
+    batch = std::min(2 * batch, poll);
   }
   return 0;
 }
@@ -306,7 +313,8 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
       D.fkind.reserve(nb * 6) || D.fnb.reserve(nb * 24 * sizeof(int)) || D.bpar.reserve(nb) ||
       D.h.reserve(nb * sizeof(double)) || D.invh.reserve(nb * sizeof(double)) ||
       D.h3.reserve(nb * sizeof(double)) || D.hf.reserve(nb * sizeof(float)) ||
-      D.invhf.reserve(nb * sizeof(float)) || D.keys.reserve(cap * sizeof(unsigned long long)) ||
+      D.invhf.reserve(nb * sizeof(float)) || D.meta.reserve(nb * 32 * sizeof(int)) ||
+      D.keys.reserve(cap * sizeof(unsigned long long)) ||
       D.vals.reserve(cap * sizeof(int)) || D.info.reserve(TOPO_INFO_WORDS * sizeof(int)))
     return fail(H, "hipMalloc(topology tables) failed");
   hipStream_t s = H->stream;
@@ -322,7 +330,8 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
   const TopoTable ttab{D.keys.as<unsigned long long>(), D.vals.as<int>(), (unsigned long long)cap - 1};
   const TopoOut tout{D.nbr.as<int>(),  D.fkind.as<int8_t>(), D.fnb.as<int>(),
                      D.bpar.as<int8_t>(), D.h.as<double>(),  D.invh.as<double>(),
-                     D.h3.as<double>(), D.hf.as<float>(),  D.invhf.as<float>()};
+                     D.h3.as<double>(), D.hf.as<float>(),  D.invhf.as<float>(),
+                     D.meta.as<int>()};
   launch_topology_build(s, tpar, ttab, tout, D.info.as<int>());
   HIPCHECK(hipGetLastError());
   int info[TOPO_INFO_WORDS];
@@ -386,6 +395,7 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
     g.fkind = C.fkind.as<int8_t>();
     g.fnb = C.fnb.as<int>();
     g.bpar = C.bpar.as<int8_t>();
+    g.bmeta = C.meta.as<int>();
     g.bmean = H->cfg.bmean;
     g.has_iface = T.n_coarse_faces + T.n_fine_faces > 0 ? 1 : 0;
     g.bmean_row = (H->cfg.bmean == 1 || H->cfg.bmean == 3) ? origin_row : -1;

@@ -33,6 +33,7 @@ struct TopoOut {
   double* h3;
   float* hf;
   float* invhf;
+  int* meta;      // [nb][32]: fnb (24), kind (6), bpar, 0 (the SpMV's one load per lane)
 };
 
 // Info words written by the build. The error words hold the smallest

@@ -134,6 +134,8 @@ __global__ __launch_bounds__(WG_THREADS) void k_topo_blocks(TopoParams p, TopoTa
   o.hf[b] = (float)h;
   o.invhf[b] = (float)ih;
   o.bpar[b] = (int8_t)((B.y & 1) | ((B.z & 1) << 1) | ((B.w & 1) << 2));
+  o.meta[(size_t)b * 32 + 30] = o.bpar[b];
+  o.meta[(size_t)b * 32 + 31] = 0;
   if (B.y == 0 && B.z == 0 && B.w == 0) atomicMax(&info[TI_ORIGIN], b);
 }
 
@@ -186,6 +188,8 @@ __global__ __launch_bounds__(WG_THREADS) void k_topo_faces(TopoParams p, TopoTab
   o.kind[gid] = (int8_t)kind;
   o.nbr[gid] = nbr;
   reinterpret_cast<int4*>(o.fnb)[gid] = make_int4(part[0], part[1], part[2], part[3]);
+  reinterpret_cast<int4*>(o.meta + (size_t)b * 32)[f] = make_int4(part[0], part[1], part[2], part[3]);
+  o.meta[(size_t)b * 32 + 24 + f] = kind;
 }
 
 }  // namespace

@@ -2,6 +2,7 @@
 accumulation, FP32 Krylov vectors, FP64 refinement. The error against the FP64
 oracle is stated by each assertion."""
 import ctypes
+import functools
 import numpy as np
 import pyoracle
 import pytest
@@ -70,21 +71,52 @@ def test_mixed_solve_reaches_fp64_tolerance(gpu, name, prec):
     assert err < 1e-7, err
 
 
+@functools.lru_cache(maxsize=None)
+def school4_oracle_solution():
+    """The FP64 oracle's solve of the school4 system to ||b - Ax|| <= 1e-12
+    ||b|| (204 iterations), volume-weighted mean removed: the discrete
+    solution to about 1e-9 (the oracle's own 1e-11 solve is 1.3e-9 from it,
+    its 1e-10 solve 7.8e-9 and its 1e-9 solve 2.4e-7)."""
+    blocks, bpd, h0, bc = CASES["school4_small"]
+    b = compatible_rhs(blocks, h0, 4).reshape(-1)
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
+    xo, info = O.bicgstab(b, tol_abs=1e-12 * np.linalg.norm(b), tol_rel=0.0, max_iter=5000)
+    assert info["status"] == 0
+    return school4_centred(xo)
+
+
+def school4_centred(v):
+    blocks, _, h0, _ = CASES["school4_small"]
+    hw = np.repeat((h0 / 2.0 ** blocks[:, 0]) ** 3, 512)
+    return v - (hw * v).sum() / hw.sum()
+
+
 @pytest.mark.parametrize("prec", ["mixed_fp16", "mixed_bf16"])
-def test_mixed_school4_residual_checked_by_oracle(gpu, prec):
+@pytest.mark.parametrize("tol", [1e-9, 1e-10])
+def test_mixed_school4_residual_checked_by_oracle(gpu, prec, tol):
     """Four-level grid, singular walls system (bMean = 0, compatible rhs):
-    the refined solve reports true relative residual <= 1e-9, and the oracle's
-    own operator, applied on the CPU to the returned pressure, confirms it."""
+    the refined solve reports true relative residual <= tol, the oracle's
+    own operator, applied on the CPU to the returned pressure, confirms it,
+    and the pressure (volume-weighted mean removed: the constant the
+    singular system leaves free) is close to the oracle's discrete solution,
+    as the oracle's own solve to the same residual is: within 1e-6 at 1e-9
+    (the oracle's 1e-9 solve: 2.4e-7) and 1e-7 at 1e-10 (7.8e-9)."""
     blocks, bpd, h0, bc = CASES["school4_small"]
     b = compatible_rhs(blocks, h0, 4)
-    S = PoissonSolver(bpd, h0, bc, precision=prec, bmean=0, tol_rel=0.0, tol_rhs=1e-9,
+    S = PoissonSolver(bpd, h0, bc, precision=prec, bmean=0, tol_rel=0.0, tol_rhs=tol,
                       max_iter=4000).set_topology(blocks)
     x = torch.zeros(len(blocks) * 512, dtype=torch.float64, device=gpu)
     rep = S.solve_device(dev(b), x)
-    assert rep["status"] == 0 and rep["true_rel_resid"] <= 1e-9, rep
+    assert rep["status"] == 0 and rep["true_rel_resid"] <= tol, rep
     O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
-    r = b.reshape(-1) - O.spmv(x.cpu().numpy())
-    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.0001e-9
+    xg = x.cpu().numpy()
+    r = b.reshape(-1) - O.spmv(xg)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.0001 * tol
+    xo = school4_oracle_solution()
+    err = np.linalg.norm(school4_centred(xg) - xo) / np.linalg.norm(xo)
+    print(f"{prec} at {tol:g}: {rep['iterations']} iterations, error vs the oracle's "
+          f"discrete solution {err:.2e}")
+    assert err <= (1e-6 if tol >= 1e-9 else 1e-7), err
 
 
 @pytest.mark.parametrize("name", list(CASES))
@@ -219,21 +251,22 @@ def test_c5_sequence_refined_solves_converge(gpu):
     """The 4-level AMR school sequence at full size (59 012 blocks, 30.2 M
     cells), solved step after step from the previous pressure with its
     volume-weighted mean removed (PressureProjection.cpp:404-431), to
-    ||b - Ax|| <= 1e-9 ||b||: every step converges within 2000 iterations per
-    refinement step and 8 restarts. (On step 23 of this sequence a restart
-    for stalling once repeated identically until the restart budget was
-    spent, ending at 1e-4: the stall window now doubles after a restart that
-    did not halve the residual.)"""
+    ||b - Ax|| <= 1e-10 ||b||, the tolerance of the bench's reference
+    solves: every step converges within 2000 iterations per refinement step
+    and 12 restarts. (On step 23 of this sequence a restart for stalling once
+    repeated identically until the restart budget was spent, ending at 1e-4:
+    the stall window now doubles after a restart that did not halve the
+    residual.)"""
     w = W.c5_amr_school()
-    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-9, bmean=0, precision="mixed_fp16",
-                   max_iter=2000, max_refine=8)
+    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-10, bmean=0, precision="mixed_fp16",
+                   max_iter=2000, max_refine=12)
     p = torch.zeros(w.ncells, dtype=torch.float64, device=gpu)
     for t in range(25):
         b = dev(w.rhs(step=t))
         rep = S.solve_device(b, p)
         print(t, rep["status"], rep["iterations"], rep["true_rel_resid"], flush=True)
         assert rep["status"] == 0, (t, rep)
-        assert rep["true_rel_resid"] <= 1e-9, (t, rep)
+        assert rep["true_rel_resid"] <= 1e-10, (t, rep)
         S.remove_mean(p)
 
 
