@@ -159,10 +159,9 @@ class PoissonSolver:
 
     def set_topology(self, blocks):
         blocks = np.ascontiguousarray(blocks, np.int32).reshape(-1, 4)
-        arr = (BlockDesc * len(blocks))()
-        for i, (l, a, b, c) in enumerate(blocks):
-            arr[i].level = int(l)
-            arr[i].index[0], arr[i].index[1], arr[i].index[2] = int(a), int(b), int(c)
+        # cup3d_block_desc is {level, index[3]}: four int32, the row layout
+        assert ctypes.sizeof(BlockDesc) == 16
+        arr = (BlockDesc * len(blocks)).from_buffer_copy(blocks.tobytes())
         if self.L.cup3d_poisson_set_topology(self.h, len(blocks), arr) != 0:
             raise self._err("set_topology")
         self.nb = len(blocks)
