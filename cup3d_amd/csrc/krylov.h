@@ -271,6 +271,27 @@ __device__ __forceinline__ void iface_face(const DevGrid<T>& g, const T* __restr
   wave_sync();
 }
 
+// Loads and stores of vectors a kernel touches once: nontemporal when the
+// vectors are too large to stay in the Infinity Cache between kernels.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+// Vectors of at least this many bytes are streamed with nontemporal access
+// by the hot SpMV (K2, K4) and preconditioner (K1, K3) kernels. Measured on
+// MI355X (256 MB Infinity Cache): with it, K2 on the 30.2 M-cell grid (FP32
+// vectors of 121 MB) 97 -> 90 us and K1 146 -> 134 us; on a 4.19 M-cell
+// grid (17 MB FP32, 34 MB FP64 vectors) it made K2 12-15 % slower, the
+// vectors no longer found in the cache by the next kernel.
+constexpr size_t STREAM_MIN_BYTES = size_t(64) << 20;
+
 // ---------------------------------------------------------------------------
 // Matrix-free operator y = A x (PoissonSolverExp.cpp:305-367 + interface rows)
 // with the bMean handling of hd_cusparseSpMV (BiCGSTAB.cu:392-413) and a
@@ -291,7 +312,7 @@ __device__ __forceinline__ void iface_face(const DevGrid<T>& g, const T* __restr
 // same stencil and coarse-fine faces, plus -h^3 sigma x and the Dirichlet
 // ghosts of BlockLabBC (Definitions.h:58-122,287-330) on the boundary faces
 // selected by g.dmask, both folded into the diagonal of the cell.
-template <typename T, int MODE, int BM, bool IFACE, bool HELM = false>
+template <typename T, int MODE, int BM, bool IFACE, bool HELM = false, bool NT = false>
 __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __restrict__ x,
                                                      T* __restrict__ y, const T* __restrict__ a,
                                                      T* __restrict__ c, T* __restrict__ c2,
@@ -327,7 +348,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
     for (int iz = 0; iz < BS; ++iz) col[iz] = x[off + iz * 64 + lane];
     if (NEED_A) {
 #pragma unroll
-      for (int iz = 0; iz < BS; ++iz) av[iz] = a[off + iz * 64 + lane];
+      for (int iz = 0; iz < BS; ++iz) av[iz] = ld_stream<NT>(&a[off + iz * 64 + lane]);
     }
     // face halos: a lane is on at most one x face and one y face. A lane on
     // a face with a same-level neighbour reads the neighbour's face cell;
@@ -451,19 +472,19 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
       const T yy = yv[iz];
       if (A_LDS) av[iz] = as[w][iz * 64 + lane];
       if (MODE == SPMV_PLAIN) {
-        y[i] = yy;
+        st_stream<NT>(&y[i], yy);
       } else if (MODE == SPMV_INIT) {
-        y[i] = yy;
+        st_stream<NT>(&y[i], yy);
         const T rr = av[iz] - yy;
         c[i] = rr;
         c2[i] = rr;
         v[1] += (double)yy * (double)yy;
         v[2] += (double)rr * (double)rr;
       } else if (MODE == SPMV_DOT1) {
-        y[i] = yy;
+        st_stream<NT>(&y[i], yy);
         v[1] += (double)av[iz] * (double)yy;
       } else if (MODE == SPMV_DOT2) {
-        y[i] = yy;
+        st_stream<NT>(&y[i], yy);
         v[1] += (double)av[iz] * (double)yy;
         v[2] += (double)yy * (double)yy;
       } else if (MODE == SPMV_RESID) {
@@ -612,16 +633,27 @@ __global__ __launch_bounds__(WG_THREADS) void k_bmean2_apply(DevGrid<T> g, T* __
 
 // Launch for a grid: picks the bMean and interface variants.
 // An event, if given, is recorded right after the main SpMV kernel.
+template <typename T, int MODE, int BM, bool HELM, bool NT>
+inline void spmv_kernel(hipStream_t s, int nwg, const DevGrid<T>& g, const T* x, T* y,
+                        const T* a, T* c, T* c2, SolverState* st, double* part) {
+  if (g.has_iface)
+    k_spmv<T, MODE, BM, true, HELM, NT><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
+  else
+    k_spmv<T, MODE, BM, false, HELM, NT><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
+}
 template <typename T, int MODE, int BM, bool HELM = false>
 inline void spmv_launch_iface(hipStream_t s, const DevGrid<T>& g, const T* x, T* y, const T* a,
                               T* c, T* c2, SolverState* st, double* part,
                               hipEvent_t mid = nullptr) {
   int nwg = (g.nb + 3) / 4;
-  if (g.has_iface) {
-    k_spmv<T, MODE, BM, true, HELM><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
-  } else {
-    k_spmv<T, MODE, BM, false, HELM><<<nwg, WG_THREADS, 0, s>>>(g, x, y, a, c, c2, st, part);
+  bool launched = false;
+  if constexpr (MODE == SPMV_DOT1 || MODE == SPMV_DOT2) {  // the per-iteration SpMVs
+    if ((size_t)g.n * sizeof(T) >= STREAM_MIN_BYTES) {
+      spmv_kernel<T, MODE, BM, HELM, true>(s, nwg, g, x, y, a, c, c2, st, part);
+      launched = true;
+    }
   }
+  if (!launched) spmv_kernel<T, MODE, BM, HELM, false>(s, nwg, g, x, y, a, c, c2, st, part);
   if (mid) (void)hipEventRecord(mid, s);
   if (BM == 2) {
     k_bmean2_sum<MODE><<<1, WG_THREADS, 0, s>>>(st, part, nwg);

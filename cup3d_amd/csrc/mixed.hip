@@ -43,6 +43,30 @@
 
 namespace cup3d {
 
+// float4 access to the vectors the preconditioner kernels stream through
+// once: nontemporal (NT) for vectors too large to stay in the Infinity Cache
+// between kernels (STREAM_MIN_BYTES).
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+template <bool NT>
+__device__ __forceinline__ float4 ld4(const float* base, size_t i4) {
+  if constexpr (NT) {
+    const nt_f4 q = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(base) + i4);
+    return make_float4(q.x, q.y, q.z, q.w);
+  } else {
+    return reinterpret_cast<const float4*>(base)[i4];
+  }
+}
+template <bool NT>
+__device__ __forceinline__ void st4(float* base, size_t i4, const float4& v) {
+  if constexpr (NT) {
+    const nt_f4 q = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(q, reinterpret_cast<nt_f4*>(base) + i4);
+  } else {
+    reinterpret_cast<float4*>(base)[i4] = v;
+  }
+}
+
+
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf168_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -116,7 +140,7 @@ struct FillRegs {
   float4 ra[F4], rb[F4], rc[F4];
 };
 
-template <int MODE, int NBW, int NWP>
+template <int MODE, int NBW, int NWP, bool NT>
 __device__ __forceinline__ void precond_load(int nb, int j0, int w, int l,
                                              const KrylovVectors<float>& v,
                                              const float* __restrict__ pin,
@@ -129,14 +153,14 @@ __device__ __forceinline__ void precond_load(int nb, int j0, int w, int l,
     const size_t i4 = ((size_t)b * BLEN + 256 * (u & 1) + 4 * l) / 4;
     if (b < nb) {
       if (MODE == PREC_P) {
-        R.ra[u] = reinterpret_cast<const float4*>(v.r)[i4];
-        R.rb[u] = reinterpret_cast<const float4*>(v.nu)[i4];
-        R.rc[u] = reinterpret_cast<const float4*>(v.p)[i4];
+        R.ra[u] = ld4<NT>(v.r, i4);
+        R.rb[u] = ld4<NT>(v.nu, i4);
+        R.rc[u] = ld4<NT>(v.p, i4);
       } else if (MODE == PREC_S) {
-        R.ra[u] = reinterpret_cast<const float4*>(v.r)[i4];
-        R.rb[u] = reinterpret_cast<const float4*>(v.nu)[i4];
+        R.ra[u] = ld4<NT>(v.r, i4);
+        R.rb[u] = ld4<NT>(v.nu, i4);
       } else {
-        R.ra[u] = reinterpret_cast<const float4*>(pin)[i4];
+        R.ra[u] = ld4<NT>(pin, i4);
       }
     } else {
       R.ra[u] = R.rb[u] = R.rc[u] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -144,7 +168,7 @@ __device__ __forceinline__ void precond_load(int nb, int j0, int w, int l,
   }
 }
 
-template <int MODE, typename H, int NBW, bool SPLIT, int NWP>
+template <int MODE, typename H, int NBW, bool SPLIT, int NWP, bool NT>
 __device__ __forceinline__ void precond_convert(int nb, int j0, int w, int l, float p_unscale,
                                                 const float* __restrict__ invh,
                                                 const KrylovVectors<float>& v,
@@ -169,15 +193,15 @@ __device__ __forceinline__ void precond_convert(int nb, int j0, int w, int l, fl
         val.w = fmaf(c.beta, fmaf(-c.omega, R.rb[u].w, R.rc[u].w), R.ra[u].w);
       }
       if (b < nb) {
-        reinterpret_cast<float4*>(v.p)[i4] = val;
-        if (c.restart) reinterpret_cast<float4*>(v.rhat)[i4] = R.ra[u];
+        st4<NT>(v.p, i4, val);
+        if (c.restart) st4<NT>(v.rhat, i4, R.ra[u]);
       }
     } else if (MODE == PREC_S) {
       val.x = fmaf(-c.alpha, R.rb[u].x, R.ra[u].x);
       val.y = fmaf(-c.alpha, R.rb[u].y, R.ra[u].y);
       val.z = fmaf(-c.alpha, R.rb[u].z, R.ra[u].z);
       val.w = fmaf(-c.alpha, R.rb[u].w, R.ra[u].w);
-      if (b < nb) reinterpret_cast<float4*>(v.r)[i4] = val;
+      if (b < nb) st4<NT>(v.r, i4, val);
     }
     R.ra[u] = val;
     amax[u / 2] = fmaxf(amax[u / 2], fmaxf(fmaxf(fabsf(val.x), fabsf(val.y)),
@@ -215,7 +239,7 @@ __device__ __forceinline__ void precond_convert(int nb, int j0, int w, int l, fl
 // 4 (cell tiles) x NBW/16 (block tiles) x (1 or 2 passes) MFMAs, the P_inv
 // fragments of the next k-step prefetched from L2 while the current ones are
 // used; FP32 epilogue scaled by h_b^-1 and the two scales.
-template <typename H, int NBW, bool SPLIT>
+template <typename H, int NBW, bool SPLIT, bool NT>
 __device__ __forceinline__ void precond_gemm(int nb, int j0, int w, int l,
                                              const H* __restrict__ Ppk, const H* A0,
                                              const H* A1, const float* sc,
@@ -266,14 +290,14 @@ __device__ __forceinline__ void precond_gemm(int nb, int j0, int w, int l,
         const float s = sc[jr];
 #pragma unroll
         for (int t = 0; t < 4; ++t)
-          out[(size_t)b * BLEN + (4 * w + t) * 16 + lr] = acc[t][jt][q] * s;
+          st_stream<NT>(&out[(size_t)b * BLEN + (4 * w + t) * 16 + lr], acc[t][jt][q] * s);
       }
     }
 }
 
 // One workgroup applies M^-1 to NBW consecutive blocks: every wave fills
 // (phase 1), then every wave multiplies (phases 2 and 3).
-template <int MODE, typename H, int NBW, bool SPLIT>
+template <int MODE, typename H, int NBW, bool SPLIT, bool NT = false>
 __global__ __launch_bounds__(GT) void k_precond_h(int nb, const H* __restrict__ Ppk,
                                                   float p_unscale,
                                                   const float* __restrict__ invh,
@@ -289,11 +313,11 @@ __global__ __launch_bounds__(GT) void k_precond_h(int nb, const H* __restrict__ 
   float* __restrict__ out = MODE == PREC_PLAIN ? pout : (MODE == PREC_S ? v.z2 : v.z1);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   FillRegs<NBW, GW> R;
-  precond_load<MODE, NBW, GW>(nb, j0, w, l, v, pin, R);
-  precond_convert<MODE, H, NBW, SPLIT, GW>(nb, j0, w, l, p_unscale, invh, v, c, R, As[0],
+  precond_load<MODE, NBW, GW, NT>(nb, j0, w, l, v, pin, R);
+  precond_convert<MODE, H, NBW, SPLIT, GW, NT>(nb, j0, w, l, p_unscale, invh, v, c, R, As[0],
                                            As[NS - 1], sc);
   __syncthreads();
-  precond_gemm<H, NBW, SPLIT>(nb, j0, w, l, Ppk, As[0], As[NS - 1], sc, out);
+  precond_gemm<H, NBW, SPLIT, NT>(nb, j0, w, l, Ppk, As[0], As[NS - 1], sc, out);
 }
 
 // Inner-solve start: r = rhat = (float)(r64 * scale), x = 0, ||r||^2, and the
@@ -438,8 +462,14 @@ void launch_precond_hw(hipStream_t s, int mode, int nb, const H* P, float unscal
                        const float* invh, const KrylovVectors<float>& v, const float* pin,
                        float* pout, SolverState* st) {
   const int nwg = (nb + NBW - 1) / NBW;
-  if (mode == PREC_P)
+  // K1 and K3 stream their vectors nontemporally when they are large
+  const bool nt = (size_t)nb * BLEN * sizeof(float) >= STREAM_MIN_BYTES;
+  if (mode == PREC_P && nt)
+    k_precond_h<PREC_P, H, NBW, SPLIT, true><<<nwg, GT, 0, s>>>(nb, P, unscale, invh, v, pin, pout, st);
+  else if (mode == PREC_P)
     k_precond_h<PREC_P, H, NBW, SPLIT><<<nwg, GT, 0, s>>>(nb, P, unscale, invh, v, pin, pout, st);
+  else if (mode == PREC_S && nt)
+    k_precond_h<PREC_S, H, NBW, SPLIT, true><<<nwg, GT, 0, s>>>(nb, P, unscale, invh, v, pin, pout, st);
   else if (mode == PREC_S)
     k_precond_h<PREC_S, H, NBW, SPLIT><<<nwg, GT, 0, s>>>(nb, P, unscale, invh, v, pin, pout, st);
   else
