@@ -87,9 +87,9 @@ COPY_BYTES = 256 << 20  # calibration copy for the FETCH_SIZE / WRITE_SIZE count
 # --kernels-only); the summaries are kept in profiles/ (file named per entry).
 TRAFFIC = {
     ("amr4_school50_blobs64_sequence", "mixed_fp16", "K1"):
-        (618.95e6, "profiles/mixed_fp16_amr4_school_traffic.md"),
+        (611.8e6, "profiles/mixed_fp16_amr4_school_traffic.md"),
     ("amr4_school50_blobs64_sequence", "mixed_fp16", "K2"):
-        (402.15e6, "profiles/mixed_fp16_amr4_school_traffic.md"),
+        (392.9e6, "profiles/mixed_fp16_amr4_school_traffic.md"),
     ("amr4_school50_blobs64_sequence", "mixed_fp16", "K5"):
         (971.88e6, "profiles/mixed_fp16_amr4_school_traffic.md"),
     ("uniform_256x128x128_walls_blobs256_sphere", "mixed_fp16", "K1"):
