@@ -318,6 +318,8 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
                                                      T* __restrict__ c, T* __restrict__ c2,
                                                      SolverState* st, double* part) {
   constexpr bool NEED_A = MODE != SPMV_PLAIN;
+  // sum_b h_b^3 sum x feeds the bMean = 1 row and the bMean = 2 term only
+  constexpr bool NEED_SUM = BM == 1 || BM == 2;
   __shared__ T xs[4][BLEN];
   __shared__ T ys[IFACE ? 4 : 1][IFACE ? BLEN : 1];
   __shared__ T pl[IFACE ? 4 : 1][IFACE ? 64 : 1];
@@ -413,7 +415,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
             // the result keeps full relative precision on smooth fields
         yv[iz] = h * ((((xm - c0) + (xp - c0)) + ((ym - c0) + (yp - c0))) +
                       ((zm - c0) + (zp - c0)));
-      msum += (double)c0;
+      if (NEED_SUM) msum += (double)c0;
     }
     if (HELM) {
       // domain-boundary faces of this block with a Dirichlet ghost
@@ -462,7 +464,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_spmv(DevGrid<T> g, const T* __re
   // epilogue: stores and partial sums
   double v[3] = {0.0, 0.0, 0.0};
   if (valid) {
-    v[0] = (double)h * (double)h * (double)h * msum;
+    if (NEED_SUM) v[0] = (double)h * (double)h * (double)h * msum;
 #pragma unroll
     for (int iz = 0; iz < BS; ++iz) {
       const size_t i = off + iz * 64 + lane;
