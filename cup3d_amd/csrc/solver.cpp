@@ -19,6 +19,7 @@
 #include <initializer_list>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "cnn.h"
@@ -227,6 +228,28 @@ int run_bicgstab64(cup3d_poisson* H, const DevGrid<double>& g) {
     batch = std::min(2 * batch, poll);
   }
   return 0;
+}
+
+// Runs fn(b0, b1) over [0, nb) split into contiguous ranges on up to 16 host
+// threads (one for small grids): the gather and scatter of the host solve
+// are memory-bound copies that one core cannot feed at the host's bandwidth.
+template <typename F>
+void parallel_blocks(int nb, F&& fn) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::min<unsigned>(std::min(hw, 16u), (unsigned)std::max(1, nb / 512));
+  if (nt <= 1) {
+    fn(0, nb);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve(nt - 1);
+  const int chunk = (nb + nt - 1) / nt;
+  for (int t = 1; t < nt; ++t) {
+    const int b0 = t * chunk, b1 = std::min(nb, b0 + chunk);
+    if (b0 < b1) pool.emplace_back([&fn, b0, b1] { fn(b0, b1); });
+  }
+  fn(0, std::min(nb, chunk));
+  for (std::thread& th : pool) th.join();
 }
 
 }  // namespace
@@ -667,21 +690,25 @@ int cup3d_poisson_solve(cup3d_poisson_t H, const double* const* rhs_blocks,
   double *hb = H->h_stage, *hx = H->h_stage + n;
   double *db = H->d_stage.as<double>(), *dx = db + n;
   // getVec gather (PoissonSolverExp.cpp:389-417)
-  for (int b = 0; b < H->nb; ++b) {
-    std::memcpy(hb + (size_t)b * BLEN, rhs_blocks ? rhs_blocks[b] : rhs_contig + (size_t)b * BLEN,
-                BLEN * sizeof(double));
-    std::memcpy(hx + (size_t)b * BLEN, x_blocks ? x_blocks[b] : x_contig + (size_t)b * BLEN,
-                BLEN * sizeof(double));
-  }
+  parallel_blocks(H->nb, [&](int b0, int b1) {
+    for (int b = b0; b < b1; ++b) {
+      std::memcpy(hb + (size_t)b * BLEN,
+                  rhs_blocks ? rhs_blocks[b] : rhs_contig + (size_t)b * BLEN, BLEN * sizeof(double));
+      std::memcpy(hx + (size_t)b * BLEN, x_blocks ? x_blocks[b] : x_contig + (size_t)b * BLEN,
+                  BLEN * sizeof(double));
+    }
+  });
   int st = -1;
   if (hipMemcpy(db, hb, bytes, hipMemcpyHostToDevice) == hipSuccess &&
       hipMemcpy(dx, hx, bytes, hipMemcpyHostToDevice) == hipSuccess) {
     st = cup3d_poisson_solve_device(H, db, dx, rep);
     if (st >= 0 && hipMemcpy(hx, dx, bytes, hipMemcpyDeviceToHost) == hipSuccess) {
       // scatter into the pressure blocks (.cpp:452-472)
-      for (int b = 0; b < H->nb; ++b)
-        std::memcpy(x_blocks ? x_blocks[b] : x_contig + (size_t)b * BLEN, hx + (size_t)b * BLEN,
-                    BLEN * sizeof(double));
+      parallel_blocks(H->nb, [&](int b0, int b1) {
+        for (int b = b0; b < b1; ++b)
+          std::memcpy(x_blocks ? x_blocks[b] : x_contig + (size_t)b * BLEN, hx + (size_t)b * BLEN,
+                      BLEN * sizeof(double));
+      });
     } else if (st >= 0) {
       st = fail(H, "hipMemcpy D2H failed");
     }

@@ -242,6 +242,28 @@ def test_host_paths_match_device_path(gpu):
     assert rep["iterations"] == reph["iterations"] == repb["iterations"]
 
 
+def test_host_paths_match_device_path_threaded(gpu):
+    """8192 blocks (the full uniform walls grid): the host entry point
+    gathers and scatters the blocks on several host threads, each over its
+    own contiguous range of blocks; host arrays, block pointers and device
+    vectors give the same pressure bitwise."""
+    w = W.c2_uniform_walls()
+    S = solver_for(w, tol_rel=0.0, tol_rhs=1e-6, bmean=0, max_iter=3000)
+    b = w.rhs(step=1).reshape(-1)
+    x = torch.zeros(w.ncells, dtype=torch.float64, device=gpu)
+    rep = S.solve_device(dev(b), x)
+    xh = np.zeros(w.ncells)
+    reph = S.solve_host(b, xh)
+    nb = len(w.blocks)
+    rb = [b[i * 512:(i + 1) * 512].copy() for i in range(nb)]
+    xb = [np.zeros(512) for _ in range(nb)]
+    repb = S.solve_blocks(rb, xb)
+    assert rep["status"] == reph["status"] == repb["status"] == 0
+    assert np.array_equal(x.cpu().numpy(), xh)
+    assert np.array_equal(xh, np.concatenate(xb))
+    assert rep["iterations"] == reph["iterations"] == repb["iterations"]
+
+
 def test_host_staging_follows_topology_change(gpu):
     """The host entry point keeps its staging buffers between solves and
     replaces them when a new topology changes the cell count: host and
