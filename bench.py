@@ -24,11 +24,15 @@ iteration cap (3000):
     trained, untimed, by the library's HIP kernels on the (rhs, p_prev, p)
     triples of the --train steps before the timed ones), previous pressure
     as the fallback;
-  * extrapolation_only: the same from the linear extrapolation in time of
+  * extrapolation_linear: the same from the linear extrapolation in time of
     the previous two pressures (no net);
-  * the headline `value`: the learned guess and the extrapolation together,
-    weighted by FP64 least squares on their residuals (fallback: the previous
-    pressure);
+  * extrapolation_only: the same from the least-squares combination of the
+    differences of the last --history (3) pressures (no net): the linear,
+    quadratic and cubic extrapolations are in its span;
+  * the headline `value`: the learned guess and that extrapolation
+    together, weighted by FP64 least squares on their residuals (fallback:
+    the previous pressure); the handle's history is primed with the steps
+    before the warm-up, so every timed solve sees its full history;
   * tight: the headline form to ||b - A x|| <= 1e-8 ||b||, the residual at
     which the pressure meets the 1e-6 relative-L2 bar.
 Every solve's pressure is compared with the reference solution of its step
@@ -132,6 +136,8 @@ def parse():
     p.add_argument("--c4-steps", type=int, default=400)
     p.add_argument("--c4-train", type=int, default=300)
     p.add_argument("--c4-fresh", type=int, default=40)
+    p.add_argument("--history", type=int, default=3,
+                   help="previous pressures the warm start extrapolates from (1 = linear)")
     p.add_argument("--no-dst", action="store_true",
                    help="skip the leg with the sine-transform block inverse")
     p.add_argument("--fp64-dst-steps", type=int, default=4,
@@ -637,12 +643,18 @@ def main():
         del T
         print(f"setup: net trained on {ntrain} triples, loss {l0:.3e} -> {l1:.3e}",
               file=sys.stderr, flush=True)
-    seq = seq[ntrain:]
-    timed = seq[args.warmup:]
+    # the legs with the extrapolation keep the pressures of their last solves:
+    # the history is primed with the steps before the warm-up (training
+    # steps, untimed), so every timed solve has its full history
+    prime = min(max(0, args.history - args.warmup), ntrain)
+    seq_all = seq
+    seq = seq_all[ntrain - prime:]
+    warm = args.warmup + prime
+    timed = seq[warm:]
     tol = args.tol
 
     def leg(name, form, **kw):
-        S, res = time_form(w, form, args.bmean, seq, args.warmup, rel, tol=kw.pop("tol", tol),
+        S, res = time_form(w, form, args.bmean, seq, warm, rel, tol=kw.pop("tol", tol),
                            log=name, **kw)
         print(f"{name}: {np.mean([r[0] for r in res]):.2f} ms per step", file=sys.stderr,
               flush=True)
@@ -653,8 +665,8 @@ def main():
     S64, res64 = leg("fp64", "fp64")
     res64_dst = None
     if args.fp64_dst_steps > 0:
-        _, res64_dst = time_form(w, "fp64", args.bmean, seq[:args.warmup + args.fp64_dst_steps],
-                                 args.warmup, rel, tol=tol, log="fp64_dst", precond="dst")
+        _, res64_dst = time_form(w, "fp64", args.bmean, seq[:warm + args.fp64_dst_steps],
+                                 warm, rel, tol=tol, log="fp64_dst", precond="dst")
     legs = {}
     if args.form == "fp64":
         S, res = S64, res64
@@ -663,17 +675,21 @@ def main():
     else:
         _, legs["prev_guess"] = leg("prev_guess", args.form)
         _, legs["learned_only"] = leg("learned_only", args.form, weights=weights)
-        _, legs["extrapolation_only"] = leg("extrapolation_only", args.form, extrapolate=True)
-        S, res = leg("headline", args.form, weights=weights, extrapolate=True, profile=True)
+        _, legs["extrapolation_linear"] = leg("extrapolation_linear", args.form, extrapolate=1)
+        _, legs["extrapolation_only"] = leg("extrapolation_only", args.form,
+                                            extrapolate=args.history)
+        S, res = leg("headline", args.form, weights=weights, extrapolate=args.history,
+                     profile=True)
     # the headline form with the exact block inverse applied by sine
     # transforms in FP32 instead of the FP16/BF16 GEMM (cfg.precond = DST)
     res_dst = None
     if args.form != "fp64" and not args.no_dst:
-        _, res_dst = leg("headline_dst", args.form, weights=weights, extrapolate=learned,
-                         precond="dst")
+        _, res_dst = leg("headline_dst", args.form, weights=weights,
+                         extrapolate=args.history if learned else 0, precond="dst")
     tight = None
     if args.tight_tol > 0 and args.form != "fp64":
-        _, rt = leg("tight", args.form, weights=weights, extrapolate=learned, tol=args.tight_tol)
+        _, rt = leg("tight", args.form, weights=weights, extrapolate=args.history if learned else 0,
+                    tol=args.tight_tol)
         tight = {"tol": args.tight_tol, **summary(rt, args.tight_tol),
                  "pressure_within_1e-6": bool(max(r[2] for r in rt) <= 1e-6)}
     sm, sm64 = summary(res, tol), summary(res64, tol)
@@ -714,7 +730,10 @@ def main():
             "learned_only": {**summary(lo, tol), **gains(lo),
                              "net_guess_used": f"{sum(r[1]['guess_used'] == 2 for r in lo)}/"
                                                f"{len(lo)}"},
-            "extrapolation_only": {**summary(eo, tol), **gains(eo)},
+            "extrapolation_only": {**summary(eo, tol), **gains(eo),
+                                   "history": args.history},
+            "extrapolation_linear": {**summary(legs["extrapolation_linear"], tol),
+                                     **gains(legs["extrapolation_linear"])},
         }
     rep, rhs = reps[-1], timed[-1]["rhs"]
     p_last = timed[-1]["p_prev"]
@@ -783,8 +802,9 @@ def main():
         "config": {"workload": w.name, "form": args.form, "cells": w.ncells,
                    "blocks": int(len(w.blocks)), "tolerance": f"||b-Ax||/||b|| <= {tol:g}",
                    "steps": "consecutive time steps t = warmup+1 .. warmup+steps",
-                   "warm_start": ("learned CNN guess + linear extrapolation in time, FP64 "
-                                  "least-squares weights; previous-step pressure as fallback"
+                   "warm_start": (f"learned CNN guess + extrapolation in time from the last "
+                                  f"{args.history} pressures, FP64 least-squares weights; "
+                                  "previous-step pressure as fallback"
                                   if learned else "previous-step pressure"),
                    "bMeanConstraint": args.bmean, "max_iter": MAX_ITER,
                    "parallelism": "single GPU"},

@@ -39,18 +39,26 @@ void launch_remove_mean(hipStream_t s, long long n, double* x, const double* h3,
 // b = a over n4 float4 (the copy-rate probe of the diagnostics).
 void launch_stream_copy(hipStream_t s, long long n4, const void* a, void* b);
 
-// Warm start from several guesses (device vectors of n doubles):
-// out = 2 x - hist (linear extrapolation in time from the last two steps);
-void launch_extrapolate(hipStream_t s, long long n, const double* x, const double* hist,
-                        double* out);
+// Warm start from several guesses (device vectors of n doubles).
+constexpr int GUESS_MAXV = 5;  // r0 + up to three history candidates + the net's
+struct GuessVecs {
+  const double* v[GUESS_MAXV];
+};
+struct GuessWeights {
+  double w[GUESS_MAXV];
+};
+// c = x0 + (a - b): the candidate one difference of previous pressures away;
+void launch_history_candidate(hipStream_t s, long long n, const double* x0, const double* a,
+                              const double* b, double* c);
 // x[row] = 0 (nothing for row < 0): the bMean = 1 / 3 cell that getVec zeroes;
 void launch_zero_cell(hipStream_t s, double* x, long long row);
-// dots[0..4] = (r0, r1), (r1, r1), (r0, r2), (r2, r2), (r1, r2) in a fixed
-// order (r1 or r2 may be null: their dots are 0);
-void launch_guess_dots(hipStream_t s, long long n, const double* r0, const double* r1,
-                       const double* r2, double* part, double* dots);
-// x = x0 + g1 (c1 - x0) + g2 (c2 - x0) (a null candidate must have weight 0).
-void launch_guess_blend(hipStream_t s, long long n, const double* x0, const double* c1,
-                        const double* c2, double g1, double g2, double* x);
+// dots = the upper triangle of the Gram matrix of r.v[0 .. nv), row by row,
+// GUESS_MAXV (GUESS_MAXV + 1) / 2 values (those of absent vectors 0), in a
+// fixed order; part holds 15 partials per 2048 cells;
+void launch_guess_gram(hipStream_t s, long long n, const GuessVecs& r, int nv, double* part,
+                       double* dots);
+// x = x0 + sum_k g_k (c_k - x0), k < nc (a candidate with weight 0 is not read).
+void launch_guess_blend(hipStream_t s, long long n, const double* x0, const GuessVecs& c,
+                        const GuessWeights& g, int nc, double* x);
 
 }  // namespace cup3d

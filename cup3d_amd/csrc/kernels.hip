@@ -228,57 +228,68 @@ void launch_copy_opt64(hipStream_t s, long long n, const KrylovVectors<double>& 
 
 // ---------------------------------------------------------------------------
 // Warm start from the previous steps: candidates and their least-squares
-// weights (the weights themselves are solved on the host from five dots).
+// weights (the weights themselves are solved on the host from the Gram
+// matrix of the candidates' residuals).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(WG_THREADS) void k_extrapolate(long long n,
-                                                            const double* __restrict__ x,
-                                                            const double* __restrict__ hist,
-                                                            double* __restrict__ out) {
+// c = x0 + (a - b): the candidate one history difference away from x0
+__global__ __launch_bounds__(WG_THREADS) void k_history_candidate(long long n,
+                                                                  const double* __restrict__ x0,
+                                                                  const double* __restrict__ a,
+                                                                  const double* __restrict__ b,
+                                                                  double* __restrict__ c) {
   for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n;
        i += (long long)gridDim.x * WG_THREADS)
-    out[i] = 2.0 * x[i] - hist[i];
+    c[i] = x0[i] + (a[i] - b[i]);
 }
 
-__global__ __launch_bounds__(WG_THREADS) void k_guess_dots(long long n,
-                                                           const double* __restrict__ r0,
-                                                           const double* __restrict__ r1,
-                                                           const double* __restrict__ r2,
+// Gram matrix of up to GUESS_MAXV residual vectors (upper triangle, row by
+// row: (0,0), (0,1), ..., (1,1), ...), fixed-order partial sums.
+__global__ __launch_bounds__(WG_THREADS) void k_guess_gram(long long n, GuessVecs r, int nv,
                                                            double* part) {
-  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  constexpr int ND = GUESS_MAXV * (GUESS_MAXV + 1) / 2;
+  double v[ND];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) v[k] = 0.0;
   const long long base = (long long)blockIdx.x * (4 * BLEN);
   for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
     const long long i = base + e;
     if (i < n) {
-      const double a = r0[i], p = r1 ? r1[i] : 0.0, q = r2 ? r2[i] : 0.0;
-      v[0] += a * p;
-      v[1] += p * p;
-      v[2] += a * q;
-      v[3] += q * q;
-      v[4] += p * q;
+      double q[GUESS_MAXV];
+#pragma unroll
+      for (int a = 0; a < GUESS_MAXV; ++a) q[a] = a < nv ? r.v[a][i] : 0.0;
+      int k = 0;
+#pragma unroll
+      for (int a = 0; a < GUESS_MAXV; ++a)
+#pragma unroll
+        for (int b = a; b < GUESS_MAXV; ++b) v[k++] += q[a] * q[b];
     }
   }
-  grid_store_partial<5>(v, part);
+  grid_store_partial<ND>(v, part);
 }
 
-__global__ __launch_bounds__(WG_THREADS) void k_guess_dots_fin(const double* part, int nparts,
+__global__ __launch_bounds__(WG_THREADS) void k_guess_gram_fin(const double* part, int nparts,
                                                                double* dots) {
-  double tot[5];
-  grid_sum_partials<5>(part, nparts, tot);
-  if (threadIdx.x < 5) dots[threadIdx.x] = tot[threadIdx.x];
+  constexpr int ND = GUESS_MAXV * (GUESS_MAXV + 1) / 2;
+  double tot[ND];
+  grid_sum_partials<ND>(part, nparts, tot);
+  if (threadIdx.x < ND) {
+#pragma unroll
+    for (int k = 0; k < ND; ++k)
+      if ((int)threadIdx.x == k) dots[k] = tot[k];
+  }
 }
 
+// x = x0 + sum_k g_k (c_k - x0) over the candidates with a nonzero weight
 __global__ __launch_bounds__(WG_THREADS) void k_guess_blend(long long n,
                                                             const double* __restrict__ x0,
-                                                            const double* __restrict__ c1,
-                                                            const double* __restrict__ c2,
-                                                            double g1, double g2,
+                                                            GuessVecs c, GuessWeights g, int nc,
                                                             double* __restrict__ x) {
   for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n;
        i += (long long)gridDim.x * WG_THREADS) {
     const double b = x0[i];
     double v = b;
-    if (c1) v = fma(g1, c1[i] - b, v);
-    if (c2) v = fma(g2, c2[i] - b, v);
+    for (int k = 0; k < nc; ++k)
+      if (g.w[k] != 0.0) v = fma(g.w[k], c.v[k][i] - b, v);
     x[i] = v;
   }
 }
@@ -343,11 +354,6 @@ void launch_remove_mean(hipStream_t s, long long n, double* x, const double* h3,
   k_sub_mean<<<nsub, WG_THREADS, 0, s>>>(n, x, wsum, inv_vol);
 }
 
-void launch_extrapolate(hipStream_t s, long long n, const double* x, const double* hist,
-                        double* out) {
-  const int nwg = (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 8192);
-  k_extrapolate<<<nwg, WG_THREADS, 0, s>>>(n, x, hist, out);
-}
 
 __global__ void k_zero_cell(double* __restrict__ x, long long row) { x[row] = 0.0; }
 
@@ -355,17 +361,23 @@ void launch_zero_cell(hipStream_t s, double* x, long long row) {
   if (row >= 0) k_zero_cell<<<1, 1, 0, s>>>(x, row);
 }
 
-void launch_guess_dots(hipStream_t s, long long n, const double* r0, const double* r1,
-                       const double* r2, double* part, double* dots) {
-  const int nwg = (int)((n + 4 * BLEN - 1) / (4 * BLEN));
-  k_guess_dots<<<nwg, WG_THREADS, 0, s>>>(n, r0, r1, r2, part);
-  k_guess_dots_fin<<<1, WG_THREADS, 0, s>>>(part, nwg, dots);
+void launch_history_candidate(hipStream_t s, long long n, const double* x0, const double* a,
+                              const double* b, double* c) {
+  const int nwg = (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 4096);
+  k_history_candidate<<<nwg, WG_THREADS, 0, s>>>(n, x0, a, b, c);
 }
 
-void launch_guess_blend(hipStream_t s, long long n, const double* x0, const double* c1,
-                        const double* c2, double g1, double g2, double* x) {
+void launch_guess_gram(hipStream_t s, long long n, const GuessVecs& r, int nv, double* part,
+                       double* dots) {
+  const int nwg = (int)((n + 4 * BLEN - 1) / (4 * BLEN));
+  k_guess_gram<<<nwg, WG_THREADS, 0, s>>>(n, r, nv, part);
+  k_guess_gram_fin<<<1, WG_THREADS, 0, s>>>(part, nwg, dots);
+}
+
+void launch_guess_blend(hipStream_t s, long long n, const double* x0, const GuessVecs& c,
+                        const GuessWeights& g, int nc, double* x) {
   const int nwg = (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 8192);
-  k_guess_blend<<<nwg, WG_THREADS, 0, s>>>(n, x0, c1, c2, g1, g2, x);
+  k_guess_blend<<<nwg, WG_THREADS, 0, s>>>(n, x0, c, g, nc, x);
 }
 
 // Fragment-packed P for the B operand: Ppk[(it*128 + kt)*64 + l] = P[4kt + l/16][16it + l%16].

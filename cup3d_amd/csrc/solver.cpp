@@ -127,10 +127,12 @@ struct cup3d_poisson {
   DevBuf d_stage;             // 2 n doubles
   long long stage_n = 0;
 
-  // warm start: the guess handed in by the previous solve on this topology
-  // (the caller's pressure one step earlier), and the five guess dots
+  // warm start: the guesses handed in by the last cfg.extrapolate solves on
+  // this topology (the caller's pressures of the previous steps), most
+  // recent first in hist[0 .. hist_count), and the candidates' Gram matrix
   DevBuf d_hist, d_gdots;
-  bool hist_valid = false;
+  double* hist[3] = {nullptr, nullptr, nullptr};
+  int hist_count = 0;
 };
 
 namespace {
@@ -272,7 +274,7 @@ int cup3d_poisson_create(const cup3d_poisson_config* cfg, cup3d_poisson_t* out) 
   if (c.max_iter <= 0) { g_create_error = "max_iter must be positive"; return -1; }
   if (c.precision < 0 || c.precision > 2) { g_create_error = "unknown precision"; return -1; }
   if (c.guess < 0 || c.guess > 2) { g_create_error = "unknown guess mode"; return -1; }
-  if (c.extrapolate < 0 || c.extrapolate > 1) { g_create_error = "extrapolate must be 0 or 1"; return -1; }
+  if (c.extrapolate < 0 || c.extrapolate > 3) { g_create_error = "extrapolate must be 0..3"; return -1; }
   if (c.precond < 0 || c.precond > 1) { g_create_error = "precond must be 0 (GEMM) or 1 (DST)"; return -1; }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
@@ -439,12 +441,16 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
   v.z1 = base + 8 * n;
   v.z2 = base + 9 * n;
   v.b = base + 10 * n;
-  H->part_len = (size_t)((H->nb + 3) / 4 + 8) * 6;  // up to 5 partials per workgroup
+  // up to 15 partials per workgroup of 2048 cells (the warm start's Gram matrix)
+  H->part_len = (size_t)((H->nb + 3) / 4 + 8) * 16;
   if (H->d_part.reserve(H->part_len * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(partials) failed");
-  H->hist_valid = false;
-  if (H->cfg.extrapolate && H->d_hist.reserve(n * sizeof(double)) != hipSuccess)
-    return fail(H, "hipMalloc(history) failed");
-  if (!H->d_gdots.p && H->d_gdots.alloc(8 * sizeof(double)) != hipSuccess)
+  H->hist_count = 0;
+  if (H->cfg.extrapolate > 0) {
+    if (H->d_hist.reserve((size_t)H->cfg.extrapolate * n * sizeof(double)) != hipSuccess)
+      return fail(H, "hipMalloc(history) failed");
+    for (int k = 0; k < H->cfg.extrapolate; ++k) H->hist[k] = H->d_hist.as<double>() + k * n;
+  }
+  if (!H->d_gdots.p && H->d_gdots.alloc(16 * sizeof(double)) != hipSuccess)
     return fail(H, "hipMalloc(guess dots) failed");
   HIPCHECK(hipMemset(H->d_state.p, 0, sizeof(SolverState)));
   if (mixed_set_topology(H->mixed, H->nb, H->n, H->err) != 0) return -1;
@@ -517,14 +523,49 @@ static int resid_into(cup3d_poisson_t H, const double* d_x, double* r) {
   return hipGetLastError() == hipSuccess ? 0 : fail(H, "residual launch failed");
 }
 
-// Warm start from up to two candidates next to the given guess x0 (the
-// caller's pressure of the previous step): the learned net's prediction
-// (CUP3D_GUESS_CNN, with trained weights) and the linear extrapolation in
-// time 2 x0 - x_hist from the guess of the previous solve (cfg.extrapolate).
-// The guess becomes x0 + g1 (c1 - x0) + g2 (c2 - x0) with the weights that
-// minimise the FP64 residual ||b - A x|| (a 2x2 least-squares problem on
-// the candidates' residuals); g = 0 is one of the choices, so the result is
-// never worse than x0, which is kept (the fallback) when no candidate helps.
+// Solves the nk x nk system G g = f (symmetric positive semi-definite, nk
+// <= GUESS_MAXV - 1) by Gaussian elimination with partial pivoting; false if
+// it is singular to rounding.
+static bool small_solve(int nk, double G[][GUESS_MAXV], const double* f, double* g) {
+  double M[GUESS_MAXV][GUESS_MAXV + 1];
+  double scale = 0;
+  for (int i = 0; i < nk; ++i) {
+    for (int j = 0; j < nk; ++j) M[i][j] = G[i][j];
+    M[i][nk] = f[i];
+    scale = std::max(scale, std::fabs(G[i][i]));
+  }
+  for (int c = 0; c < nk; ++c) {
+    int p = c;
+    for (int r = c + 1; r < nk; ++r)
+      if (std::fabs(M[r][c]) > std::fabs(M[p][c])) p = r;
+    if (!(std::fabs(M[p][c]) > 1e-12 * scale)) return false;
+    for (int j = 0; j <= nk; ++j) std::swap(M[c][j], M[p][j]);
+    for (int r = c + 1; r < nk; ++r) {
+      const double m = M[r][c] / M[c][c];
+      for (int j = c; j <= nk; ++j) M[r][j] -= m * M[c][j];
+    }
+  }
+  for (int c = nk - 1; c >= 0; --c) {
+    double v = M[c][nk];
+    for (int j = c + 1; j < nk; ++j) v -= M[c][j] * g[j];
+    g[c] = v / M[c][c];
+  }
+  for (int c = 0; c < nk; ++c)
+    if (!std::isfinite(g[c])) return false;
+  return true;
+}
+
+// Warm start next to the given guess x0 (the caller's pressure of the
+// previous step): up to cfg.extrapolate candidates x0 + d_k from the
+// differences of the previous pressures, d_1 = x0 - h_1, d_2 = h_1 - h_2,
+// d_3 = h_2 - h_3, and the learned net's prediction (CUP3D_GUESS_CNN, with
+// trained weights). The guess becomes x0 + sum_k g_k (c_k - x0) with the
+// weights that minimise the FP64 residual ||b - A x|| (least squares on the
+// candidates' residuals, options: the first m time directions, m = 0..M,
+// with or without the net; a larger option replaces the best smaller one
+// only if it lowers the squared residual by at least 1 %); g = 0 is one of
+// the choices, so the result is never worse than x0, which is kept (the
+// fallback) when no candidate helps.
 static int warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* used,
                       double* resid, double* w_net, double* w_time) {
   hipStream_t s = H->stream;
@@ -535,81 +576,101 @@ static int warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* 
   *resid = r_prev;
   const KrylovVectors<double>& v = H->v64;
   const double* x0 = v.x[0];
+  const long long n = H->n;
   // With bMean 1 or 3 getVec zeroes the origin cell of the guess; every
   // candidate gets the same zero before its residual is taken, so the
   // weights minimise the residual the solve actually starts from.
   const long long zero_row = (H->cfg.bmean == 1 || H->cfg.bmean == 3) && H->topo.origin_block >= 0
                                  ? (long long)H->topo.origin_block * BLEN
                                  : -1;
-  double *c1 = nullptr, *c2 = nullptr, *r1 = nullptr, *r2 = nullptr;
-  if (H->cfg.extrapolate && H->hist_valid) {
-    c2 = v.z2;
-    r2 = v.p;
-    launch_extrapolate(s, H->n, x0, H->d_hist.as<double>(), c2);
-    launch_zero_cell(s, c2, zero_row);
-    if (resid_into(H, c2, r2) != 0) return -1;
+  // candidates (time directions first, then the net) and their residuals,
+  // in Krylov vectors the solve initialises afterwards
+  double* const tc[3] = {v.z2, v.z1, v.x[1]};
+  double* const tr[3] = {v.p, v.r, v.rhat};
+  GuessVecs cand{}, res{};
+  int nc = 0;
+  const int K = H->cfg.extrapolate;
+  const int mt = std::min(K, H->hist_count);  // time directions available
+  for (int k = 0; k < mt; ++k) {
+    launch_history_candidate(s, n, x0, k == 0 ? x0 : H->hist[k - 1], H->hist[k], tc[k]);
+    launch_zero_cell(s, tc[k], zero_row);
+    if (resid_into(H, tc[k], tr[k]) != 0) return -1;
+    cand.v[nc] = tc[k];
+    res.v[nc] = tr[k];
+    ++nc;
   }
-  if (H->cfg.extrapolate) {  // this step's pressure is the next solve's history
-    HIPCHECK(hipMemcpyAsync(H->d_hist.p, d_x, (size_t)H->n * sizeof(double),
-                            hipMemcpyDeviceToDevice, s));
-    H->hist_valid = true;
+  if (K > 0) {  // this step's pressure becomes h_1 of the next solve
+    double* oldest = H->hist[K - 1];
+    for (int k = K - 1; k > 0; --k) H->hist[k] = H->hist[k - 1];
+    H->hist[0] = oldest;
+    HIPCHECK(hipMemcpyAsync(oldest, d_x, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    H->hist_count = std::min(H->hist_count + 1, K);
   }
+  int inet = -1;
   if (H->cfg.guess == CUP3D_GUESS_CNN && H->cnn.have_weights) {
-    c1 = H->cnn.d_pred;
-    r1 = v.t;
-    if (cnn_predict(H->cnn, s, v.nu, x0, c1, H->err) != 0) return -1;
-    launch_zero_cell(s, c1, zero_row);
-    if (resid_into(H, c1, r1) != 0) return -1;
+    if (cnn_predict(H->cnn, s, v.nu, x0, H->cnn.d_pred, H->err) != 0) return -1;
+    launch_zero_cell(s, H->cnn.d_pred, zero_row);
+    if (resid_into(H, H->cnn.d_pred, v.t) != 0) return -1;
+    inet = nc;
+    cand.v[nc] = H->cnn.d_pred;
+    res.v[nc] = v.t;
+    ++nc;
   }
-  if (!c1 && !c2) return 0;
-  double d[5];
-  launch_guess_dots(s, H->n, v.nu, r1, r2, H->d_part.as<double>(), H->d_gdots.as<double>());
-  HIPCHECK(hipMemcpyAsync(d, H->d_gdots.p, sizeof(d), hipMemcpyDeviceToHost, s));
+  if (nc == 0) return 0;
+  // Gram matrix of [r0, r_1 .. r_nc]; u_k = A (c_k - x0) = r0 - r_k
+  GuessVecs gv{};
+  gv.v[0] = v.nu;
+  for (int k = 0; k < nc; ++k) gv.v[k + 1] = res.v[k];
+  constexpr int ND = GUESS_MAXV * (GUESS_MAXV + 1) / 2;
+  double D[ND];
+  launch_guess_gram(s, n, gv, nc + 1, H->d_part.as<double>(), H->d_gdots.as<double>());
+  HIPCHECK(hipMemcpyAsync(D, H->d_gdots.p, sizeof(D), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
-  // u_k = A (c_k - x0) = r0 - r_k; minimise |r0 - g1 u1 - g2 u2|^2
-  const double a = r_prev * r_prev;
-  const double G11 = a - 2 * d[0] + d[1], G22 = a - 2 * d[2] + d[3];
-  const double G12 = a - d[0] - d[2] + d[4];
-  const double f1 = a - d[0], f2 = a - d[2];
-  // Options: the given guess, either candidate alone (1-D least squares), or
-  // both (2x2). A direction is only taken if it lowers the squared residual
-  // by at least 1 % against the option without it: a direction that barely
-  // helps (the net next to the extrapolation has weight ~1e-3) would only
-  // perturb the guess.
+  auto dot = [&](int a, int b) {  // (r_a, r_b), index 0 = r0
+    if (a > b) std::swap(a, b);
+    return D[a * GUESS_MAXV - a * (a - 1) / 2 + (b - a)];
+  };
+  const double a2 = r_prev * r_prev;
+  double Gf[GUESS_MAXV][GUESS_MAXV], ff[GUESS_MAXV];
+  for (int i = 0; i < nc; ++i) {
+    ff[i] = a2 - dot(0, i + 1);
+    for (int j = 0; j < nc; ++j) Gf[i][j] = a2 - dot(0, i + 1) - dot(0, j + 1) + dot(i + 1, j + 1);
+  }
+  // options in order of size: the first m time directions, with or without
+  // the net; each replaces the best so far only if it gains at least 1 %
   constexpr double KEEP = 0.99;
-  auto ok = [](double v) { return std::isfinite(v); };
-  auto one = [&](double G, double f) { return G > 0 && ok(G) && ok(f) ? f / G : 0.0; };
-  const double gn = c1 ? one(G11, f1) : 0.0, gt = c2 ? one(G22, f2) : 0.0;
-  const double r_net = a - gn * f1, r_time = a - gt * f2;  // residual^2 of each alone
-  double g1 = 0, g2 = 0, r2n = a;
-  if (c2 && ok(r_time) && r_time < KEEP * r2n) {
-    g2 = gt;
-    r2n = r_time;
-  }
-  if (c1 && ok(r_net) && r_net < KEEP * r2n) {
-    g1 = gn;
-    g2 = 0;
-    r2n = r_net;
-  }
-  const double det = G11 * G22 - G12 * G12;
-  if (c1 && c2 && ok(det) && det > 1e-12 * G11 * G22) {
-    const double b1 = (f1 * G22 - f2 * G12) / det, b2 = (f2 * G11 - f1 * G12) / det;
-    const double r_both = a - (b1 * f1 + b2 * f2);
-    if (ok(b1) && ok(b2) && ok(r_both) && r_both < KEEP * std::min(r_net, r_time)) {
-      g1 = b1;
-      g2 = b2;
-      r2n = r_both;
+  double best_r2 = a2, best_g[GUESS_MAXV] = {0, 0, 0, 0, 0};
+  for (int size = 1; size <= nc; ++size) {
+    for (int with_net = 0; with_net <= (inet >= 0 ? 1 : 0); ++with_net) {
+      const int m = size - with_net;
+      if (m < 0 || m > mt) continue;
+      int idx[GUESS_MAXV], nk = 0;
+      for (int k = 0; k < m; ++k) idx[nk++] = k;
+      if (with_net) idx[nk++] = inet;
+      double G[GUESS_MAXV][GUESS_MAXV], f[GUESS_MAXV], g[GUESS_MAXV];
+      for (int i = 0; i < nk; ++i) {
+        f[i] = ff[idx[i]];
+        for (int j = 0; j < nk; ++j) G[i][j] = Gf[idx[i]][idx[j]];
+      }
+      if (!small_solve(nk, G, f, g)) continue;
+      double r2 = a2;
+      for (int i = 0; i < nk; ++i) r2 -= g[i] * f[i];
+      if (!std::isfinite(r2) || !(r2 < KEEP * best_r2)) continue;
+      best_r2 = r2;
+      for (int k = 0; k < GUESS_MAXV; ++k) best_g[k] = 0;
+      for (int i = 0; i < nk; ++i) best_g[idx[i]] = g[i];
     }
   }
-  if (!ok(g1) || !ok(g2) || !ok(r2n) || (g1 == 0 && g2 == 0) || !(r2n < a))
-    return 0;  // fallback: the given guess
+  if (!(best_r2 < a2)) return 0;  // fallback: the given guess
+  GuessWeights gw{};
+  for (int k = 0; k < nc; ++k) gw.w[k] = best_g[k];
   // a candidate with weight 0 is not read (it may hold non-finite values)
-  launch_guess_blend(s, H->n, x0, g1 != 0 ? c1 : nullptr, g2 != 0 ? c2 : nullptr, g1, g2, d_x);
+  launch_guess_blend(s, n, x0, cand, gw, nc, d_x);
   HIPCHECK(hipGetLastError());
-  *used = g1 != 0 ? CUP3D_GUESS_CNN : CUP3D_GUESS_GIVEN;
-  *w_net = g1;
-  *w_time = g2;
-  *resid = std::sqrt(r2n > 0 ? r2n : 0.0);
+  *w_time = mt > 0 ? best_g[0] : 0.0;
+  *w_net = inet >= 0 ? best_g[inet] : 0.0;
+  *used = *w_net != 0 ? CUP3D_GUESS_CNN : CUP3D_GUESS_GIVEN;
+  *resid = std::sqrt(best_r2 > 0 ? best_r2 : 0.0);
   return 0;
 }
 

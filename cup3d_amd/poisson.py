@@ -138,7 +138,8 @@ class PoissonSolver:
         c.device, c.inner_tol, c.max_refine = int(device), float(inner_tol), int(max_refine)
         c.poll_every = int(poll_every)
         c.gemm_passes = int(gemm_passes)
-        c.extrapolate = int(bool(extrapolate))
+        # True = 1: the linear extrapolation; 2, 3: that many previous pressures
+        c.extrapolate = int(extrapolate)
         c.precond = {"gemm": 0, "dst": 1}[precond] if isinstance(precond, str) else int(precond)
         self.cfg = c
         h = _vp()
