@@ -136,8 +136,9 @@ def parse():
     p.add_argument("--c4-steps", type=int, default=400)
     p.add_argument("--c4-train", type=int, default=300)
     p.add_argument("--c4-fresh", type=int, default=40)
-    p.add_argument("--history", type=int, default=3,
-                   help="previous pressures the warm start extrapolates from (1 = linear)")
+    p.add_argument("--history", type=int, default=6,
+                   help="previous pressures the warm start extrapolates from (1 = linear, "
+                        "at most 6)")
     p.add_argument("--no-dst", action="store_true",
                    help="skip the leg with the sine-transform block inverse")
     p.add_argument("--fp64-dst-steps", type=int, default=4,
@@ -486,7 +487,7 @@ def bench_c4(args):
     fresh-seed sequence against the previous-pressure guess."""
     from cup3d_amd import learned
     r = learned.run(steps=args.c4_steps, train=args.c4_train, fresh=args.c4_fresh,
-                    forms=("mixed_fp16",))
+                    forms=("mixed_fp16",), history=args.history)
     held, fresh = r["held_out"]["mixed_fp16"], r["fresh_seed"]["mixed_fp16"]
     out = {"metric": "learned warm start: iterations saved vs the previous-pressure guess",
            "value": held["iterations_saved_pct"], "unit": "% iterations (held-out steps)",

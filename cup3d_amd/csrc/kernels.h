@@ -40,25 +40,26 @@ void launch_remove_mean(hipStream_t s, long long n, double* x, const double* h3,
 void launch_stream_copy(hipStream_t s, long long n4, const void* a, void* b);
 
 // Warm start from several guesses (device vectors of n doubles).
-constexpr int GUESS_MAXV = 5;  // r0 + up to three history candidates + the net's
+constexpr int HIST_MAX = 6;                // differences of previous pressures
+constexpr int GUESS_MAXV = HIST_MAX + 2;   // r0 + the history directions + the net's
 struct GuessVecs {
   const double* v[GUESS_MAXV];
 };
-struct GuessWeights {
-  double w[GUESS_MAXV];
-};
-// c = x0 + (a - b): the candidate one difference of previous pressures away;
-void launch_history_candidate(hipStream_t s, long long n, const double* x0, const double* a,
-                              const double* b, double* c);
+// d = a - b, and d[row] = 0 when row >= 0 (the bMean = 1 / 3 cell that getVec
+// zeroes): one direction of the warm start;
+void launch_guess_direction(hipStream_t s, long long n, const double* a, const double* b,
+                            long long row, double* d);
 // x[row] = 0 (nothing for row < 0): the bMean = 1 / 3 cell that getVec zeroes;
 void launch_zero_cell(hipStream_t s, double* x, long long row);
 // dots = the upper triangle of the Gram matrix of r.v[0 .. nv), row by row,
 // GUESS_MAXV (GUESS_MAXV + 1) / 2 values (those of absent vectors 0), in a
-// fixed order; part holds 15 partials per 2048 cells;
+// fixed order; part holds that many partials per 2048 cells;
 void launch_guess_gram(hipStream_t s, long long n, const GuessVecs& r, int nv, double* part,
                        double* dots);
-// x = x0 + sum_k g_k (c_k - x0), k < nc (a candidate with weight 0 is not read).
-void launch_guess_blend(hipStream_t s, long long n, const double* x0, const GuessVecs& c,
-                        const GuessWeights& g, int nc, double* x);
+// x = x0 + sum_{k < m} gt[k] (h[k-1] - h[k]) + gnet (pred - x0), h[-1] = x0,
+// and x[row] = x0[row] (row >= 0); pred is not read when gnet = 0.
+void launch_guess_blend(hipStream_t s, long long n, const double* x0, const GuessVecs& h,
+                        const double* gt, int m, const double* pred, double gnet, long long row,
+                        double* x);
 
 }  // namespace cup3d

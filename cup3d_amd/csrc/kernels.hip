@@ -227,26 +227,27 @@ void launch_copy_opt64(hipStream_t s, long long n, const KrylovVectors<double>& 
 }
 
 // ---------------------------------------------------------------------------
-// Warm start from the previous steps: candidates and their least-squares
+// Warm start from the previous steps: directions and their least-squares
 // weights (the weights themselves are solved on the host from the Gram
-// matrix of the candidates' residuals).
+// matrix of the initial residual and the directions' images under A).
 // ---------------------------------------------------------------------------
-// c = x0 + (a - b): the candidate one history difference away from x0
-__global__ __launch_bounds__(WG_THREADS) void k_history_candidate(long long n,
-                                                                  const double* __restrict__ x0,
-                                                                  const double* __restrict__ a,
-                                                                  const double* __restrict__ b,
-                                                                  double* __restrict__ c) {
+// d = a - b, zero in the bMean cell
+__global__ __launch_bounds__(WG_THREADS) void k_guess_direction(long long n,
+                                                                const double* __restrict__ a,
+                                                                const double* __restrict__ b,
+                                                                long long row,
+                                                                double* __restrict__ d) {
   for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n;
        i += (long long)gridDim.x * WG_THREADS)
-    c[i] = x0[i] + (a[i] - b[i]);
+    d[i] = i == row ? 0.0 : a[i] - b[i];
 }
 
-// Gram matrix of up to GUESS_MAXV residual vectors (upper triangle, row by
-// row: (0,0), (0,1), ..., (1,1), ...), fixed-order partial sums.
-__global__ __launch_bounds__(WG_THREADS) void k_guess_gram(long long n, GuessVecs r, int nv,
+// Gram matrix of up to GUESS_MAXV vectors (upper triangle, row by row:
+// (0,0), (0,1), ..., (1,1), ...), fixed-order partial sums; NV vectors.
+template <int NV>
+__global__ __launch_bounds__(WG_THREADS) void k_guess_gram(long long n, GuessVecs r,
                                                            double* part) {
-  constexpr int ND = GUESS_MAXV * (GUESS_MAXV + 1) / 2;
+  constexpr int ND = NV * (NV + 1) / 2;
   double v[ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k) v[k] = 0.0;
@@ -254,43 +255,57 @@ __global__ __launch_bounds__(WG_THREADS) void k_guess_gram(long long n, GuessVec
   for (int e = threadIdx.x; e < 4 * BLEN; e += WG_THREADS) {
     const long long i = base + e;
     if (i < n) {
-      double q[GUESS_MAXV];
+      double q[NV];
 #pragma unroll
-      for (int a = 0; a < GUESS_MAXV; ++a) q[a] = a < nv ? r.v[a][i] : 0.0;
+      for (int a = 0; a < NV; ++a) q[a] = r.v[a][i];
       int k = 0;
 #pragma unroll
-      for (int a = 0; a < GUESS_MAXV; ++a)
+      for (int a = 0; a < NV; ++a)
 #pragma unroll
-        for (int b = a; b < GUESS_MAXV; ++b) v[k++] += q[a] * q[b];
+        for (int b = a; b < NV; ++b) v[k++] += q[a] * q[b];
     }
   }
   grid_store_partial<ND>(v, part);
 }
 
+// Sums the partials and scatters them into the GUESS_MAXV-wide triangle
+// (the entries of absent vectors 0).
+template <int NV>
 __global__ __launch_bounds__(WG_THREADS) void k_guess_gram_fin(const double* part, int nparts,
                                                                double* dots) {
-  constexpr int ND = GUESS_MAXV * (GUESS_MAXV + 1) / 2;
+  constexpr int ND = NV * (NV + 1) / 2;
   double tot[ND];
   grid_sum_partials<ND>(part, nparts, tot);
-  if (threadIdx.x < ND) {
-#pragma unroll
-    for (int k = 0; k < ND; ++k)
-      if ((int)threadIdx.x == k) dots[k] = tot[k];
+  if (threadIdx.x == 0) {
+    int k = 0, o = 0;
+    for (int a = 0; a < GUESS_MAXV; ++a)
+      for (int b = a; b < GUESS_MAXV; ++b, ++o) dots[o] = a < NV && b < NV ? tot[k++] : 0.0;
   }
 }
 
-// x = x0 + sum_k g_k (c_k - x0) over the candidates with a nonzero weight
+struct GuessWeights {
+  double w[HIST_MAX];
+};
+
+// x = x0 + sum_{k < m} g_k (h_{k-1} - h_k) + gnet (pred - x0), h_{-1} = x0;
+// the bMean cell keeps x0's value (every direction is zero there)
 __global__ __launch_bounds__(WG_THREADS) void k_guess_blend(long long n,
                                                             const double* __restrict__ x0,
-                                                            GuessVecs c, GuessWeights g, int nc,
+                                                            GuessVecs h, GuessWeights g, int m,
+                                                            const double* __restrict__ pred,
+                                                            double gnet, long long row,
                                                             double* __restrict__ x) {
   for (long long i = (long long)blockIdx.x * WG_THREADS + threadIdx.x; i < n;
        i += (long long)gridDim.x * WG_THREADS) {
     const double b = x0[i];
-    double v = b;
-    for (int k = 0; k < nc; ++k)
-      if (g.w[k] != 0.0) v = fma(g.w[k], c.v[k][i] - b, v);
-    x[i] = v;
+    double v = b, prev = b;
+    for (int k = 0; k < m; ++k) {
+      const double hk = h.v[k][i];
+      v = fma(g.w[k], prev - hk, v);
+      prev = hk;
+    }
+    if (gnet != 0.0) v = fma(gnet, pred[i] - b, v);
+    x[i] = i == row ? b : v;
   }
 }
 
@@ -361,23 +376,43 @@ void launch_zero_cell(hipStream_t s, double* x, long long row) {
   if (row >= 0) k_zero_cell<<<1, 1, 0, s>>>(x, row);
 }
 
-void launch_history_candidate(hipStream_t s, long long n, const double* x0, const double* a,
-                              const double* b, double* c) {
-  const int nwg = (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 4096);
-  k_history_candidate<<<nwg, WG_THREADS, 0, s>>>(n, x0, a, b, c);
+void launch_guess_direction(hipStream_t s, long long n, const double* a, const double* b,
+                            long long row, double* d) {
+  const int nwg = (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 8192);
+  k_guess_direction<<<nwg, WG_THREADS, 0, s>>>(n, a, b, row, d);
+}
+
+template <int NV>
+static void guess_gram_nv(hipStream_t s, long long n, const GuessVecs& r, double* part,
+                          double* dots) {
+  const int nwg = (int)((n + 4 * BLEN - 1) / (4 * BLEN));
+  k_guess_gram<NV><<<nwg, WG_THREADS, 0, s>>>(n, r, part);
+  k_guess_gram_fin<NV><<<1, WG_THREADS, 0, s>>>(part, nwg, dots);
 }
 
 void launch_guess_gram(hipStream_t s, long long n, const GuessVecs& r, int nv, double* part,
                        double* dots) {
-  const int nwg = (int)((n + 4 * BLEN - 1) / (4 * BLEN));
-  k_guess_gram<<<nwg, WG_THREADS, 0, s>>>(n, r, nv, part);
-  k_guess_gram_fin<<<1, WG_THREADS, 0, s>>>(part, nwg, dots);
+  static_assert(GUESS_MAXV == 8, "one instance per vector count");
+  switch (nv) {
+    case 1: guess_gram_nv<1>(s, n, r, part, dots); break;
+    case 2: guess_gram_nv<2>(s, n, r, part, dots); break;
+    case 3: guess_gram_nv<3>(s, n, r, part, dots); break;
+    case 4: guess_gram_nv<4>(s, n, r, part, dots); break;
+    case 5: guess_gram_nv<5>(s, n, r, part, dots); break;
+    case 6: guess_gram_nv<6>(s, n, r, part, dots); break;
+    case 7: guess_gram_nv<7>(s, n, r, part, dots); break;
+    default: guess_gram_nv<8>(s, n, r, part, dots); break;
+  }
 }
 
-void launch_guess_blend(hipStream_t s, long long n, const double* x0, const GuessVecs& c,
-                        const GuessWeights& g, int nc, double* x) {
+void launch_guess_blend(hipStream_t s, long long n, const double* x0, const GuessVecs& h,
+                        const double* gt, int m, const double* pred, double gnet, long long row,
+                        double* x) {
+  GuessWeights g{};
+  for (int k = 0; k < m && k < HIST_MAX; ++k) g.w[k] = gt[k];
   const int nwg = (int)std::min<long long>((n + WG_THREADS - 1) / WG_THREADS, 8192);
-  k_guess_blend<<<nwg, WG_THREADS, 0, s>>>(n, x0, c, g, nc, x);
+  k_guess_blend<<<nwg, WG_THREADS, 0, s>>>(n, x0, h, g, m < HIST_MAX ? m : HIST_MAX, pred, gnet,
+                                           row, x);
 }
 
 // Fragment-packed P for the B operand: Ppk[(it*128 + kt)*64 + l] = P[4kt + l/16][16it + l%16].

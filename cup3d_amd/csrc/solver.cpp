@@ -129,9 +129,12 @@ struct cup3d_poisson {
 
   // warm start: the guesses handed in by the last cfg.extrapolate solves on
   // this topology (the caller's pressures of the previous steps), most
-  // recent first in hist[0 .. hist_count), and the candidates' Gram matrix
-  DevBuf d_hist, d_gdots;
-  double* hist[3] = {nullptr, nullptr, nullptr};
+  // recent first in hist[0 .. hist_count); the images A d_k of the time
+  // directions of the last solve (d_k of one solve is d_{k-1} of the next,
+  // so only A d_1 is new each time); the Gram matrix
+  DevBuf d_hist, d_img, d_gdots;
+  double* hist[HIST_MAX] = {};
+  double* img[HIST_MAX] = {};
   int hist_count = 0;
 };
 
@@ -274,7 +277,10 @@ int cup3d_poisson_create(const cup3d_poisson_config* cfg, cup3d_poisson_t* out) 
   if (c.max_iter <= 0) { g_create_error = "max_iter must be positive"; return -1; }
   if (c.precision < 0 || c.precision > 2) { g_create_error = "unknown precision"; return -1; }
   if (c.guess < 0 || c.guess > 2) { g_create_error = "unknown guess mode"; return -1; }
-  if (c.extrapolate < 0 || c.extrapolate > 3) { g_create_error = "extrapolate must be 0..3"; return -1; }
+  if (c.extrapolate < 0 || c.extrapolate > HIST_MAX) {
+    g_create_error = "extrapolate must be 0.." + std::to_string(HIST_MAX);
+    return -1;
+  }
   if (c.precond < 0 || c.precond > 1) { g_create_error = "precond must be 0 (GEMM) or 1 (DST)"; return -1; }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
@@ -441,16 +447,22 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
   v.z1 = base + 8 * n;
   v.z2 = base + 9 * n;
   v.b = base + 10 * n;
-  // up to 15 partials per workgroup of 2048 cells (the warm start's Gram matrix)
-  H->part_len = (size_t)((H->nb + 3) / 4 + 8) * 16;
+  // up to GUESS_MAXV (GUESS_MAXV + 1) / 2 partials per workgroup of 2048
+  // cells (the warm start's Gram matrix)
+  H->part_len = (size_t)((H->nb + 3) / 4 + 8) * (GUESS_MAXV * (GUESS_MAXV + 1) / 2);
   if (H->d_part.reserve(H->part_len * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(partials) failed");
   H->hist_count = 0;
   if (H->cfg.extrapolate > 0) {
-    if (H->d_hist.reserve((size_t)H->cfg.extrapolate * n * sizeof(double)) != hipSuccess)
+    if (H->d_hist.reserve((size_t)H->cfg.extrapolate * n * sizeof(double)) != hipSuccess ||
+        H->d_img.reserve((size_t)H->cfg.extrapolate * n * sizeof(double)) != hipSuccess)
       return fail(H, "hipMalloc(history) failed");
-    for (int k = 0; k < H->cfg.extrapolate; ++k) H->hist[k] = H->d_hist.as<double>() + k * n;
+    for (int k = 0; k < H->cfg.extrapolate; ++k) {
+      H->hist[k] = H->d_hist.as<double>() + k * n;
+      H->img[k] = H->d_img.as<double>() + k * n;
+    }
   }
-  if (!H->d_gdots.p && H->d_gdots.alloc(16 * sizeof(double)) != hipSuccess)
+  if (!H->d_gdots.p &&
+      H->d_gdots.alloc(GUESS_MAXV * (GUESS_MAXV + 1) / 2 * sizeof(double)) != hipSuccess)
     return fail(H, "hipMalloc(guess dots) failed");
   HIPCHECK(hipMemset(H->d_state.p, 0, sizeof(SolverState)));
   if (mixed_set_topology(H->mixed, H->nb, H->n, H->err) != 0) return -1;
@@ -556,121 +568,139 @@ static bool small_solve(int nk, double G[][GUESS_MAXV], const double* f, double*
 }
 
 // Warm start next to the given guess x0 (the caller's pressure of the
-// previous step): up to cfg.extrapolate candidates x0 + d_k from the
-// differences of the previous pressures, d_1 = x0 - h_1, d_2 = h_1 - h_2,
-// d_3 = h_2 - h_3, and the learned net's prediction (CUP3D_GUESS_CNN, with
-// trained weights). The guess becomes x0 + sum_k g_k (c_k - x0) with the
-// weights that minimise the FP64 residual ||b - A x|| (least squares on the
-// candidates' residuals, options: the first m time directions, m = 0..M,
-// with or without the net; a larger option replaces the best smaller one
-// only if it lowers the squared residual by at least 1 %); g = 0 is one of
-// the choices, so the result is never worse than x0, which is kept (the
-// fallback) when no candidate helps.
+// previous step), in the spirit of Fischer's projection onto previous
+// solutions (Comput. Methods Appl. Mech. Engrg. 163, 1998), here with the
+// residual norm: up to cfg.extrapolate directions from the differences of
+// the previous pressures, d_1 = x0 - h_1, d_2 = h_1 - h_2, ..., and the
+// learned net's step d_net = pred - x0 (CUP3D_GUESS_CNN, with trained
+// weights). The guess becomes x0 + sum_k g_k d_k with the weights that
+// minimise the FP64 residual ||r0 - sum_k g_k A d_k||, r0 = b - A x0 (least
+// squares on the Gram matrix of [r0, A d_1, ...], options: the first m time
+// directions, m = 0..M, with or without the net; a larger option replaces
+// the best smaller one only if it lowers the squared residual by at least
+// 1 %); g = 0 is one of the choices, so the result is never worse than x0,
+// which is kept (the fallback) when no direction helps.
 static int warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* used,
                       double* resid, double* w_net, double* w_time) {
   hipStream_t s = H->stream;
   *used = CUP3D_GUESS_GIVEN;
   *w_net = *w_time = 0.0;
+  // a failure on the way leaves history and images out of step: start over
+  struct ClearUnlessDone {
+    cup3d_poisson_t H;
+    bool done = false;
+    ~ClearUnlessDone() {
+      if (!done) H->hist_count = 0;
+    }
+  } guard{H};
   const double r_prev = increment_residual(H, d_rhs, d_x);  // v64.b, v64.x[0], r0 in v64.nu
   if (r_prev < 0) return -1;
   *resid = r_prev;
   const KrylovVectors<double>& v = H->v64;
   const double* x0 = v.x[0];
   const long long n = H->n;
-  // With bMean 1 or 3 getVec zeroes the origin cell of the guess; every
-  // candidate gets the same zero before its residual is taken, so the
-  // weights minimise the residual the solve actually starts from.
+  // With bMean 1 or 3 getVec zeroes the origin cell of the guess (x0 holds
+  // that zero); every direction is zero there too, so the weights minimise
+  // the residual the solve actually starts from.
   const long long zero_row = (H->cfg.bmean == 1 || H->cfg.bmean == 3) && H->topo.origin_block >= 0
                                  ? (long long)H->topo.origin_block * BLEN
                                  : -1;
-  // candidates (time directions first, then the net) and their residuals,
-  // in Krylov vectors the solve initialises afterwards
-  double* const tc[3] = {v.z2, v.z1, v.x[1]};
-  double* const tr[3] = {v.p, v.r, v.rhat};
-  GuessVecs cand{}, res{};
+  // the images u_k = A d_k of the time directions, then the net's: d_k of
+  // this solve is d_{k-1} of the last (h_k here is h_{k-1} there, bit for
+  // bit), so A d_2 .. A d_m are the last solve's A d_1 .. A d_{m-1}, kept
+  // in img[]; only A d_1 is new (into the slot of the dropped A d_K). Every
+  // direction passes through `dir` on its way to the SpMV.
+  double* const dir = v.t;
+  GuessVecs gv{};
+  gv.v[0] = v.nu;
   int nc = 0;
   const int K = H->cfg.extrapolate;
   const int mt = std::min(K, H->hist_count);  // time directions available
-  for (int k = 0; k < mt; ++k) {
-    launch_history_candidate(s, n, x0, k == 0 ? x0 : H->hist[k - 1], H->hist[k], tc[k]);
-    launch_zero_cell(s, tc[k], zero_row);
-    if (resid_into(H, tc[k], tr[k]) != 0) return -1;
-    cand.v[nc] = tc[k];
-    res.v[nc] = tr[k];
-    ++nc;
+  if (K > 0) {
+    double* fresh = H->img[K - 1];
+    for (int k = K - 1; k > 0; --k) H->img[k] = H->img[k - 1];
+    H->img[0] = fresh;
   }
-  if (K > 0) {  // this step's pressure becomes h_1 of the next solve
-    double* oldest = H->hist[K - 1];
-    for (int k = K - 1; k > 0; --k) H->hist[k] = H->hist[k - 1];
-    H->hist[0] = oldest;
-    HIPCHECK(hipMemcpyAsync(oldest, d_x, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    H->hist_count = std::min(H->hist_count + 1, K);
+  if (mt > 0) {
+    launch_guess_direction(s, n, x0, H->hist[0], zero_row, dir);
+    launch_spmv64(s, SPMV_PLAIN, H->g64, dir, H->img[0], nullptr, nullptr, nullptr,
+                  H->d_state.as<SolverState>(), H->d_part.as<double>());
+    if (hipGetLastError() != hipSuccess) return fail(H, "warm-start launch failed");
   }
+  for (int k = 0; k < mt; ++k) gv.v[++nc] = H->img[k];
   int inet = -1;
   if (H->cfg.guess == CUP3D_GUESS_CNN && H->cnn.have_weights) {
     if (cnn_predict(H->cnn, s, v.nu, x0, H->cnn.d_pred, H->err) != 0) return -1;
-    launch_zero_cell(s, H->cnn.d_pred, zero_row);
-    if (resid_into(H, H->cnn.d_pred, v.t) != 0) return -1;
-    inet = nc;
-    cand.v[nc] = H->cnn.d_pred;
-    res.v[nc] = v.t;
-    ++nc;
+    launch_guess_direction(s, n, H->cnn.d_pred, x0, zero_row, dir);
+    launch_spmv64(s, SPMV_PLAIN, H->g64, dir, v.z2, nullptr, nullptr, nullptr,
+                  H->d_state.as<SolverState>(), H->d_part.as<double>());
+    gv.v[nc + 1] = v.z2;
+    inet = nc++;
   }
-  if (nc == 0) return 0;
-  // Gram matrix of [r0, r_1 .. r_nc]; u_k = A (c_k - x0) = r0 - r_k
-  GuessVecs gv{};
-  gv.v[0] = v.nu;
-  for (int k = 0; k < nc; ++k) gv.v[k + 1] = res.v[k];
-  constexpr int ND = GUESS_MAXV * (GUESS_MAXV + 1) / 2;
-  double D[ND];
-  launch_guess_gram(s, n, gv, nc + 1, H->d_part.as<double>(), H->d_gdots.as<double>());
-  HIPCHECK(hipMemcpyAsync(D, H->d_gdots.p, sizeof(D), hipMemcpyDeviceToHost, s));
-  HIPCHECK(hipStreamSynchronize(s));
-  auto dot = [&](int a, int b) {  // (r_a, r_b), index 0 = r0
-    if (a > b) std::swap(a, b);
-    return D[a * GUESS_MAXV - a * (a - 1) / 2 + (b - a)];
-  };
-  const double a2 = r_prev * r_prev;
-  double Gf[GUESS_MAXV][GUESS_MAXV], ff[GUESS_MAXV];
-  for (int i = 0; i < nc; ++i) {
-    ff[i] = a2 - dot(0, i + 1);
-    for (int j = 0; j < nc; ++j) Gf[i][j] = a2 - dot(0, i + 1) - dot(0, j + 1) + dot(i + 1, j + 1);
-  }
-  // options in order of size: the first m time directions, with or without
-  // the net; each replaces the best so far only if it gains at least 1 %
-  constexpr double KEEP = 0.99;
-  double best_r2 = a2, best_g[GUESS_MAXV] = {0, 0, 0, 0, 0};
-  for (int size = 1; size <= nc; ++size) {
-    for (int with_net = 0; with_net <= (inet >= 0 ? 1 : 0); ++with_net) {
-      const int m = size - with_net;
-      if (m < 0 || m > mt) continue;
-      int idx[GUESS_MAXV], nk = 0;
-      for (int k = 0; k < m; ++k) idx[nk++] = k;
-      if (with_net) idx[nk++] = inet;
-      double G[GUESS_MAXV][GUESS_MAXV], f[GUESS_MAXV], g[GUESS_MAXV];
-      for (int i = 0; i < nk; ++i) {
-        f[i] = ff[idx[i]];
-        for (int j = 0; j < nk; ++j) G[i][j] = Gf[idx[i]][idx[j]];
+  if (nc > 0) {
+    constexpr int ND = GUESS_MAXV * (GUESS_MAXV + 1) / 2;
+    double D[ND];
+    launch_guess_gram(s, n, gv, nc + 1, H->d_part.as<double>(), H->d_gdots.as<double>());
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(D, H->d_gdots.p, sizeof(D), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    auto dot = [&](int a, int b) {  // index 0 = r0, k + 1 = u_k
+      if (a > b) std::swap(a, b);
+      return D[a * GUESS_MAXV - a * (a - 1) / 2 + (b - a)];
+    };
+    const double a2 = dot(0, 0);
+    // options in order of size: the first m time directions, with or
+    // without the net; each replaces the best so far only if it gains at
+    // least 1 %
+    constexpr double KEEP = 0.99;
+    double best_r2 = a2, best_g[GUESS_MAXV] = {};
+    for (int size = 1; size <= nc; ++size) {
+      for (int with_net = 0; with_net <= (inet >= 0 ? 1 : 0); ++with_net) {
+        const int m = size - with_net;
+        if (m < 0 || m > mt) continue;
+        int idx[GUESS_MAXV], nk = 0;
+        for (int k = 0; k < m; ++k) idx[nk++] = k;
+        if (with_net) idx[nk++] = inet;
+        double G[GUESS_MAXV][GUESS_MAXV], f[GUESS_MAXV], g[GUESS_MAXV];
+        for (int i = 0; i < nk; ++i) {
+          f[i] = dot(0, idx[i] + 1);
+          for (int j = 0; j < nk; ++j) G[i][j] = dot(idx[i] + 1, idx[j] + 1);
+        }
+        if (!small_solve(nk, G, f, g)) continue;
+        double r2 = a2;
+        for (int i = 0; i < nk; ++i) r2 -= g[i] * f[i];
+        if (!std::isfinite(r2) || !(r2 < KEEP * best_r2)) continue;
+        best_r2 = r2;
+        for (int k = 0; k < GUESS_MAXV; ++k) best_g[k] = 0;
+        for (int i = 0; i < nk; ++i) best_g[idx[i]] = g[i];
       }
-      if (!small_solve(nk, G, f, g)) continue;
-      double r2 = a2;
-      for (int i = 0; i < nk; ++i) r2 -= g[i] * f[i];
-      if (!std::isfinite(r2) || !(r2 < KEEP * best_r2)) continue;
-      best_r2 = r2;
-      for (int k = 0; k < GUESS_MAXV; ++k) best_g[k] = 0;
-      for (int i = 0; i < nk; ++i) best_g[idx[i]] = g[i];
+    }
+    if (best_r2 < a2) {
+      int m = 0;  // time directions with a weight
+      for (int k = 0; k < mt; ++k)
+        if (best_g[k] != 0) m = k + 1;
+      const double gnet = inet >= 0 ? best_g[inet] : 0.0;
+      GuessVecs hv{};
+      for (int k = 0; k < mt; ++k) hv.v[k] = H->hist[k];
+      // written to d_x, not read by the solve before getVec copies it
+      launch_guess_blend(s, n, x0, hv, best_g, m, H->cnn.d_pred, gnet, zero_row, d_x);
+      HIPCHECK(hipGetLastError());
+      *w_time = mt > 0 ? best_g[0] : 0.0;
+      *w_net = gnet;
+      *used = gnet != 0 ? CUP3D_GUESS_CNN : CUP3D_GUESS_GIVEN;
+      *resid = std::sqrt(best_r2 > 0 ? best_r2 : 0.0);
     }
   }
-  if (!(best_r2 < a2)) return 0;  // fallback: the given guess
-  GuessWeights gw{};
-  for (int k = 0; k < nc; ++k) gw.w[k] = best_g[k];
-  // a candidate with weight 0 is not read (it may hold non-finite values)
-  launch_guess_blend(s, n, x0, cand, gw, nc, d_x);
-  HIPCHECK(hipGetLastError());
-  *w_time = mt > 0 ? best_g[0] : 0.0;
-  *w_net = inet >= 0 ? best_g[inet] : 0.0;
-  *used = *w_net != 0 ? CUP3D_GUESS_CNN : CUP3D_GUESS_GIVEN;
-  *resid = std::sqrt(best_r2 > 0 ? best_r2 : 0.0);
+  // this step's given guess (x0, as getVec made it) becomes h_1 of the next
+  // solve; copied after the blend, which still reads the present history
+  if (K > 0) {
+    double* oldest = H->hist[K - 1];
+    for (int k = K - 1; k > 0; --k) H->hist[k] = H->hist[k - 1];
+    H->hist[0] = oldest;
+    HIPCHECK(hipMemcpyAsync(oldest, x0, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    H->hist_count = std::min(H->hist_count + 1, K);
+  }
+  guard.done = true;
   return 0;
 }
 

@@ -9,9 +9,10 @@ recurrence stagnates above 1e-7 on some steps of this sequence); the first
 library's own HIP kernels learn (FP32 forward/backward, Adam). The net is then judged on the remaining steps
 and on a sequence from a fresh seed by what it saves: the number of
 iterations, for the FP64 and the mixed-precision solve, against starting from
-the previous pressure; also, from the second judged step on, the linear
-extrapolation in time from the previous two pressures alone and combined
-with the net (least-squares weights on their FP64 residuals). The solve
+the previous pressure; also, once the handle holds `history` previous
+pressures, the extrapolation in time from them (least squares over their
+differences; history 1 = the linear extrapolation) alone and combined with
+the net (least-squares weights on their FP64 residuals). The solve
 always finishes to the same tolerance and falls back to the previous
 pressure whenever the net's guess has the larger residual, so a worse net
 costs one inference and two residuals, never accuracy.
@@ -37,7 +38,7 @@ import numpy as np
 
 def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, batch=256,
         lr=1e-3, res_weight=100.0, seed=1, tol=1e-7, log=sys.stderr, workload="c4",
-        fp64_max_iter=3000, forms=("fp64", "mixed_fp16")):
+        fp64_max_iter=3000, forms=("fp64", "mixed_fp16"), history=1):
     import torch
 
     from . import workloads as W
@@ -92,10 +93,11 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, bat
             base = solver_for(w, precision=form, guess="given", **kf)
             net = solver_for(w, precision=form, guess="cnn", **kf)
             net.cnn_set_weights(weights)
-            # with the extrapolation in time from the previous two pressures:
-            # alone, and together with the net (least-squares weights)
-            ext = solver_for(w, precision=form, guess="given", extrapolate=True, **kf)
-            both = solver_for(w, precision=form, guess="cnn", extrapolate=True, **kf)
+            # with the extrapolation in time from the last `history` previous
+            # pressures: alone, and together with the net (least-squares
+            # weights)
+            ext = solver_for(w, precision=form, guess="given", extrapolate=history, **kf)
+            both = solver_for(w, precision=form, guess="cnn", extrapolate=history, **kf)
             both.cnn_set_weights(weights)
             it_e, it_x, w_net, fail_e, fail_x = [], [], [], 0, 0
             it_b, it_n, used, ratio, err, raw, fail_b, fail_n = [], [], 0, [], [], [], 0, 0
@@ -116,7 +118,7 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, bat
                 re_ = ext.solve_device(rhs_[t], xe)
                 xx = sols_[t - 1].clone()
                 rx = both.solve_device(rhs_[t], xx)
-                if t > ts[0]:  # the first step has no history yet
+                if t >= ts[0] + history:  # the history is full from here on
                     it_e.append(re_["iterations"])
                     it_x.append(rx["iterations"])
                     w_net.append(rx["guess_weight_net"])
@@ -148,13 +150,16 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, bat
                 "initial_resid_ratio_median": float(np.median(ratio)),
                 "net_resid_ratio_median": float(np.median(raw)),
                 "err_vs_fp64_max": float(np.max(err)),
-                # steps after the first: extrapolation alone and with the net
-                "iterations_prev_guess_from_second": float(np.mean(it_b[1:])),
+                # the steps with a full history: extrapolation alone and with
+                # the net, against the previous pressure on the same steps
+                "extrap_history": history,
+                "iterations_prev_guess_same_steps": float(np.mean(it_b[history:])),
                 "iterations_extrap": float(np.mean(it_e)),
                 "iterations_net_extrap": float(np.mean(it_x)),
-                "iterations_saved_extrap_pct": round(100.0 * (1 - np.sum(it_e) / np.sum(it_b[1:])), 2),
+                "iterations_saved_extrap_pct": round(
+                    100.0 * (1 - np.sum(it_e) / np.sum(it_b[history:])), 2),
                 "iterations_saved_net_extrap_pct": round(
-                    100.0 * (1 - np.sum(it_x) / np.sum(it_b[1:])), 2),
+                    100.0 * (1 - np.sum(it_x) / np.sum(it_b[history:])), 2),
                 "net_weight_with_extrap_median": float(np.median(w_net)),
                 "not_converged_extrap": fail_e,
                 "not_converged_net_extrap": fail_x,
@@ -189,11 +194,14 @@ def main():
     p.add_argument("--workload", default="c4", choices=["c4", "c5"],
                    help="c4: uniform blob sequence; c5: the 4-level AMR school grid")
     p.add_argument("--fp64-max-iter", type=int, default=3000)
+    p.add_argument("--history", type=int, default=1,
+                   help="previous pressures of the extrapolation legs (1 = linear)")
     a = p.parse_args()
     if not 0 < a.train < a.steps:
         raise SystemExit("need 0 < --train < --steps")
     print(json.dumps(run(a.bpd, a.steps, a.train, a.fresh, a.train_steps, a.batch, a.lr,
-                         a.res_weight, workload=a.workload, fp64_max_iter=a.fp64_max_iter)))
+                         a.res_weight, workload=a.workload, fp64_max_iter=a.fp64_max_iter,
+                         history=a.history)))
 
 
 if __name__ == "__main__":

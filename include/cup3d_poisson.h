@@ -66,21 +66,24 @@ enum {
   CUP3D_GUESS_CNN = 2     /* learned warm start from (rhs, x on input = previous pressure);
                              falls back to the given x when its residual is larger */
 };
-/* With cfg.extrapolate = K in 1..3 (and guess GIVEN or CNN) the solver keeps
+/* With cfg.extrapolate = K in 1..6 (and guess GIVEN or CNN) the solver keeps
  * the x handed to its last K solves on the same topology (the caller's
  * pressures of the previous steps, h_1 the most recent) and tries the
- * directions d_1 = x - h_1, d_2 = h_1 - h_2, d_3 = h_2 - h_3 (those the
- * history holds). The guess used is x + sum_k g_k d_k + g_net (net - x) with
- * the weights that minimise the FP64 residual ||b - A guess|| (least squares
- * over the candidates' residuals: projection onto the recent solutions, as
- * in Fischer, CMAME 163 (1998) 193-204, in the residual norm). K = 1 with
- * g_1 = 1 is the linear extrapolation in time 2 x - h_1; K = 2 and 3 reach
- * the quadratic and cubic extrapolations. The time directions are taken
- * nested (d_1, then d_2, then d_3) and the net's as an option beside them;
- * a larger set is only taken if it lowers the squared residual by at least
- * 1 % against the best smaller one, and the given x (all weights 0) is one
- * of the choices, so the guess is never worse than x. guess_weight_time
- * reports g_1. set_topology clears the history. */
+ * directions d_1 = x - h_1, d_2 = h_1 - h_2, ..., d_K = h_{K-1} - h_K (those
+ * the history holds). The guess used is x + sum_k g_k d_k + g_net (net - x)
+ * with the weights that minimise the FP64 residual ||b - A guess|| (least
+ * squares over r0 = b - A x and the directions' images A d_k: projection onto
+ * the recent solutions, as in Fischer, CMAME 163 (1998) 193-204, in the
+ * residual norm). K = 1 with g_1 = 1 is the linear extrapolation in time
+ * 2 x - h_1; K = 2, 3, ... reach the quadratic, cubic, ... extrapolations.
+ * The time directions are taken nested (d_1, then d_2, ...) and the net's as
+ * an option beside them; a larger set is only taken if it lowers the squared
+ * residual by at least 1 % against the best smaller one, and the given x (all
+ * weights 0) is one of the choices, so the guess is never worse than x.
+ * guess_weight_time reports g_1. set_topology clears the history. Cost per
+ * solve: one FP64 operator application (A d_2 .. A d_K are the last solve's
+ * A d_1 .. A d_{K-1}, kept), one Gram reduction, one blend; memory: 2 K
+ * vectors of n doubles. */
 
 typedef struct {
   int bpd[3];        /* blocks per direction at level 0 (-bpdx/-bpdy/-bpdz) */
@@ -109,7 +112,7 @@ typedef struct {
   int poll_every;    /* iterations between host polls of the device status (0 = default) */
   int gemm_passes;   /* mixed only: 0 (or 2, 3) = input split into hi + lo, two MFMA
                         passes, linear to FP32 precision; 1 = one FP16/BF16 pass */
-  int extrapolate;   /* 0 = off (the reference contract); K = 1..3 = also extrapolate
+  int extrapolate;   /* 0 = off (the reference contract); K = 1..6 = also extrapolate
                         the guess in time from the last K solves' (see above) */
   int precond;       /* how M^-1 = blockdiag(h_b^-1 P_inv) is applied.
                         CUP3D_PRECOND_GEMM (0): P_inv x block matrix on the matrix cores

@@ -1,8 +1,8 @@
 """Warm start from the previous steps on the GPU (cfg.extrapolate): the
 linear extrapolation in time 2 p_{t-1} - p_{t-2} (K = 1) and the
-least-squares combination of the differences of the last K = 3 pressures,
-checked against what the mathematics fixes — exactness on sequences that
-are linear (K = 1) or cubic (K = 3) in time, the least-squares optimum and
+least-squares combination of the differences of the last K = 3, 5 and 6
+pressures, checked against what the mathematics fixes — exactness on
+sequences that are polynomials of degree K in time, the least-squares optimum and
 the residual it reaches computed independently with the oracle's operator,
 and never a worse residual than the given guess."""
 import numpy as np
@@ -131,75 +131,84 @@ def test_extrapolation_off_keeps_the_reference_contract(gpu):
         assert abs(rep["err_init"] - r_in) <= 1e-12 * r_in
 
 
+@pytest.mark.parametrize("K", [3, 5])
 @pytest.mark.parametrize("name", ["uniform_walls_3x2x2", "tiny_amr3_walls"])
-def test_history_exact_on_cubic_sequence(gpu, name):
-    """cfg.extrapolate = 3 on p_t = p0 + t d1 + t^2 d2 + t^3 d3: once the
-    handle holds three previous pressures (from the fifth solve on; the first
-    solve is handed the zero guess), the cubic extrapolation 4 x - 6 h1 + 4 h2
-    - h3 lies in the span of the three differences, and the least-squares
-    guess reaches it: weights (3, -3, 1) on (x - h1, h1 - h2, h2 - h3),
+def test_history_exact_on_polynomial_sequence(gpu, name, K):
+    """cfg.extrapolate = K on p_t = p0 + sum_{j=1..K} t^j d_j: once the handle
+    holds K previous pressures (from solve K + 1 on; the first solve is handed
+    the zero guess), the extrapolation of degree K, sum_{j=1..K+1} (-1)^(j+1)
+    C(K+1, j) p_{t-j}, lies in the span of the K differences, and the
+    least-squares guess reaches it: weight C(K, 1) = K on x - h1 (K = 3: the
+    weights (3, -3, 1) of the cubic extrapolation 4 x - 6 h1 + 4 h2 - h3),
     residual below 1e-6 of the given guess's."""
     blocks, bpd, h0, bc = CASES[name]
     O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
     p0 = mean_free(random_field(len(blocks), 71), blocks, h0)
-    d = [s * mean_free(random_field(len(blocks), 72 + k), blocks, h0)
-         for k, s in enumerate((0.1, 0.02, 0.004))]
+    d = [0.1 * 0.2 ** k * mean_free(random_field(len(blocks), 72 + k), blocks, h0)
+         for k in range(K)]
     S = PoissonSolver(bpd, h0, bc, bmean=0, tol_rel=0.0, tol_rhs=1e-11, max_iter=3000,
-                      extrapolate=3).set_topology(blocks)
+                      extrapolate=K).set_topology(blocks)
     x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
-    for t in range(7):
-        b = O.spmv(p0 + t * d[0] + t * t * d[1] + t ** 3 * d[2])
+    for t in range(K + 4):
+        b = O.spmv(p0 + sum(t ** (j + 1) * d[j] for j in range(K)))
         prev = x.cpu().numpy()
         rep = S.solve_device(dev(b), x)
         assert rep["status"] == 0, (t, rep)
         r_prev = np.linalg.norm(b - O.spmv(prev))
-        if t >= 4:
-            assert abs(rep["guess_weight_time"] - 3.0) < 1e-4, rep
+        if t >= K + 1:
+            assert abs(rep["guess_weight_time"] - K) < 1e-4 * K, (t, rep)
             # (the guess residual comes from |r0|^2 - g.f, exact to ~1e-8 |r0|)
-            assert rep["guess_resid"] < 1e-6 * r_prev, (rep["guess_resid"], r_prev)
+            assert rep["guess_resid"] < 1e-6 * r_prev, (t, rep["guess_resid"], r_prev)
         x.copy_(dev(mean_free(x.cpu().numpy(), blocks, h0)))
 
 
+@pytest.mark.parametrize("K", [3, 6])
 @pytest.mark.parametrize("name", ["amr3_walls", "tiny_amr3_walls"])
-def test_history_weights_are_the_least_squares_solution(gpu, name):
+def test_history_weights_are_the_least_squares_solution(gpu, name, K):
     """On a smooth sequence that is no polynomial in time, the reported guess
     residual and the weight of x - h1 are those of the least-squares problem
-    over the nested direction sets {d1}, {d1, d2}, {d1, d2, d3} (a larger set
-    taken only if it lowers the squared residual by at least 1 %), recomputed
-    on the CPU with the oracle's operator; never worse than the given guess."""
+    over the nested direction sets {d1}, {d1, d2}, ..., {d1 .. dK} (a larger
+    set taken only if it lowers the squared residual by at least 1 %),
+    recomputed on the CPU with the oracle's operator; never worse than the
+    given guess."""
     blocks, bpd, h0, bc = CASES[name]
     O = pyoracle.Oracle(blocks, bpd, h0, bc, 0)
     S = PoissonSolver(bpd, h0, bc, bmean=0, tol_rel=0.0, tol_rhs=1e-10, max_iter=3000,
-                      extrapolate=3).set_topology(blocks)
-    # eight components with their own frequencies: no three differences span
+                      extrapolate=K).set_topology(blocks)
+    # twelve components with their own frequencies: no K differences span
     # the sequence exactly (the residuals stay well above rounding)
-    base = [mean_free(random_field(len(blocks), 80 + k), blocks, h0) for k in range(8)]
+    nb = 12
+    base = [mean_free(random_field(len(blocks), 80 + k), blocks, h0) for k in range(nb)]
     x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
     handed = []
     deep = 0
-    for t in range(7):
-        c = [np.cos(0.3 * (1 + 0.25 * k) * t + k) for k in range(8)]
+    for t in range(K + 5):
+        c = [np.cos(0.3 * (1 + 0.25 * k) * t + k) for k in range(nb)]
         b = O.spmv(sum(ck * bk for ck, bk in zip(c, base)))
         xin = x.cpu().numpy()
         rep = S.solve_device(dev(b), x)
         assert rep["status"] == 0
-        hist = handed[::-1][:3]  # most recent first
+        hist = handed[::-1][:K]  # most recent first
         if hist:
             r0 = b - O.spmv(xin)
             pts = [xin] + hist
             U = [O.spmv(pts[k] - pts[k + 1]) for k in range(len(hist))]
             a2 = r0 @ r0
-            best, g1 = a2, 0.0
+            best, g1, cond = a2, 0.0, 1.0
             for m in range(1, len(U) + 1):
                 Um = np.stack(U[:m])
-                g = np.linalg.solve(Um @ Um.T, Um @ r0)
-                r2 = a2 - g @ (Um @ r0)
+                G = Um @ Um.T
+                g = np.linalg.solve(G, Um @ r0)
+                r = r0 - g @ Um
+                r2 = r @ r
                 if r2 < 0.99 * best:
-                    best, g1 = r2, g[0]
+                    best, g1, cond = r2, g[0], np.linalg.cond(G)
                     deep = max(deep, m)
-            assert abs(rep["guess_resid"] - np.sqrt(best)) <= 1e-7 * np.sqrt(best), (rep, best)
-            assert abs(rep["guess_weight_time"] - g1) <= 1e-6 * max(1.0, abs(g1)), (rep, g1)
+            assert abs(rep["guess_resid"] - np.sqrt(best)) <= 1e-7 * np.sqrt(a2), (t, rep, best)
+            # the weights are fixed to about cond(G) times the rounding of G
+            bar = max(1e-6, 1e-13 * cond) * max(1.0, abs(g1))
+            assert abs(rep["guess_weight_time"] - g1) <= bar, (t, rep, g1, cond)
             assert rep["guess_resid"] <= np.linalg.norm(r0)
         handed.append(xin)
         x.copy_(dev(mean_free(x.cpu().numpy(), blocks, h0)))
-    assert deep == 3  # the test has teeth: all three directions were taken
+    assert deep == K  # the test has teeth: all K directions were taken
