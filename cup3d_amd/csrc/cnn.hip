@@ -16,7 +16,8 @@
 // cell of the block.
 //
 // Inference (k_cnn_infer): one workgroup per block, activations in LDS as
-// FP16 channel-last [10][10][10][16] with a zero ring, each layer an implicit
+// FP16 channel-last [8][8][8][16] (the zero padding selected on read), the
+// input as the [10][10][10] patch, each layer an implicit
 // GEMM on v_mfma_f32_16x16x32_f16 (M = 16 cells, N = 16 output channels,
 // K = 27 offsets x input channels; the 8 consecutive K of a lane are 8
 // channels of one offset: one 16-byte LDS read), FP32 accumulation, bias and
@@ -45,10 +46,10 @@ __device__ __forceinline__ float gelu_grad(float x) {
   return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * expf(-0.5f * x * x) * 0.3989422804014327f;
 }
 
-// Value of field f at padded position (pz, py, px) in 0..9 of block b.
-template <typename T>
-__device__ __forceinline__ float halo_value(const T* __restrict__ f, const int* __restrict__ nbr,
-                                            int b, int pz, int py, int px) {
+// Index into a field of the cell that padded position (pz, py, px) in 0..9
+// of block b takes its value from (the halo rule above).
+__device__ __forceinline__ size_t halo_index(const int* __restrict__ nbr, int b, int pz, int py,
+                                             int px) {
   int c[3] = {px - 1, py - 1, pz - 1};
   int out = 0, dim = -1;
   for (int d = 0; d < 3; ++d)
@@ -66,7 +67,14 @@ __device__ __forceinline__ float halo_value(const T* __restrict__ f, const int* 
     }
   }
   for (int d = 0; d < 3; ++d) c[d] = c[d] < 0 ? 0 : (c[d] > 7 ? 7 : c[d]);
-  return (float)f[(size_t)blk * BLEN + c[2] * 64 + c[1] * 8 + c[0]];
+  return (size_t)blk * BLEN + c[2] * 64 + c[1] * 8 + c[0];
+}
+
+// Value of field f at padded position (pz, py, px) in 0..9 of block b.
+template <typename T>
+__device__ __forceinline__ float halo_value(const T* __restrict__ f, const int* __restrict__ nbr,
+                                            int b, int pz, int py, int px) {
+  return (float)f[halo_index(nbr, b, pz, py, px)];
 }
 
 // ---------------------------------------------------------------------------
@@ -158,9 +166,12 @@ __device__ __forceinline__ int padded(int m) {  // cell 0..511 -> padded index
   return (((m >> 6) + 1) * 10 + ((m >> 3) & 7) + 1) * 10 + (m & 7) + 1;
 }
 
-// One conv layer as implicit GEMM: in (LDS, stride CS halfs per cell), K-steps
-// KT; output tile handler f(m, n, value) for each of this wave's 8 M-tiles.
-template <int CS, int KT, bool L1, typename F>
+// One conv layer as implicit GEMM, K-steps KT; output tile handler f(m, n,
+// value) for each of this wave's 8 M-tiles. L1: `in` is the 10^3 input patch
+// (8 halfs per cell, valid conv); otherwise `in` holds the block's 512 cells
+// (CNN_C halfs each, no halo) and reads outside the block are the zero
+// padding of the "same" conv (selected, not stored).
+template <int KT, bool L1, typename F>
 __device__ __forceinline__ void conv_layer(const _Float16* __restrict__ in,
                                            const half8_t* __restrict__ wfrag, int w, int l,
                                            F&& f) {
@@ -168,6 +179,9 @@ __device__ __forceinline__ void conv_layer(const _Float16* __restrict__ in,
   half8_t bq[KT];
 #pragma unroll
   for (int kt = 0; kt < KT; ++kt) bq[kt] = wfrag[kt * 64 + l];
+  half8_t zero;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) zero[e] = (_Float16)0.f;
   for (int t = 0; t < 8; ++t) {
     const int tile = 8 * w + t;
     const int m = 16 * tile + lr;  // this lane's A row (cell)
@@ -177,16 +191,19 @@ __device__ __forceinline__ void conv_layer(const _Float16* __restrict__ in,
     for (int kt = 0; kt < KT; ++kt) {
       const int o = L1 ? 4 * kt + g : 2 * kt + (g >> 1);
       const int c0 = L1 ? 0 : 8 * (g & 1);
-      half8_t a;
+      half8_t a = zero;
       if (o < 27) {
         const int dz = o / 9, dy = (o / 3) % 3, dx = o % 3;
-        // L1 input is the 10^3 patch (valid conv); later layers read the
-        // padded activations, cell (z,y,x) stored at (z+1,y+1,x+1)
-        const int p = ((mz + dz) * 10 + (my + dy)) * 10 + (mx + dx);
-        a = *reinterpret_cast<const half8_t*>(&in[p * CS + c0]);
-      } else {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) a[e] = (_Float16)0.f;
+        if (L1) {
+          const int p = ((mz + dz) * 10 + (my + dy)) * 10 + (mx + dx);
+          a = *reinterpret_cast<const half8_t*>(&in[p * 8]);
+        } else {
+          const int z = mz + dz - 1, y = my + dy - 1, x = mx + dx - 1;
+          const bool inside = (unsigned)z < 8u && (unsigned)y < 8u && (unsigned)x < 8u;
+          const int p = inside ? (z * 8 + y) * 8 + x : 0;
+          const half8_t v = *reinterpret_cast<const half8_t*>(&in[p * CNN_C + c0]);
+          a = inside ? v : zero;
+        }
       }
       acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bq[kt], acc, 0, 0, 0);
     }
@@ -196,6 +213,11 @@ __device__ __forceinline__ void conv_layer(const _Float16* __restrict__ in,
   }
 }
 
+// LDS of the inference: the input patch (1000 cells x 8 halfs), later the
+// second layer's activations (512 x 16), and the first layer's (512 x 16):
+// 32 KiB per workgroup, five workgroups per CU.
+constexpr int CNN_IN_HALFS = PADC * 8 > BLEN * CNN_C ? PADC * 8 : BLEN * CNN_C;
+
 __global__ __launch_bounds__(WG_THREADS) void k_cnn_infer(int nb, const int* __restrict__ nbr,
                                                           const double* __restrict__ h,
                                                           const double* __restrict__ res,
@@ -204,53 +226,58 @@ __global__ __launch_bounds__(WG_THREADS) void k_cnn_infer(int nb, const int* __r
                                                           const _Float16* __restrict__ wpk,
                                                           const float* __restrict__ w,
                                                           double* __restrict__ out) {
-  __shared__ __attribute__((aligned(16))) _Float16 bufA[PADC * CNN_C];
-  __shared__ __attribute__((aligned(16))) _Float16 bufB[PADC * CNN_C];
+  __shared__ __attribute__((aligned(16))) _Float16 bufIn[CNN_IN_HALFS];  // input, then act2
+  __shared__ __attribute__((aligned(16))) _Float16 bufA[BLEN * CNN_C];   // act1
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   if (b >= nb) return;
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
   const double s_r = scal[0];
   const float inv_sp = (float)(1.0 / scal[1]);
   const float inv_hs = (float)(1.0 / (h[b] * s_r));
-  // input: 8-channel cells (2 real), and a zero-ringed act1 buffer
-  for (int p = tid; p < PADC; p += WG_THREADS) {
-    const int pz = p / 100, py = (p / 10) % 10, px = p % 10;
-    half8_t v;
+  // input: 8-channel cells (2 real); the indices of a thread's (up to 4)
+  // positions first, then all 8 loads, so the gather costs two memory round
+  // trips, not one per position
+  constexpr int NP = (PADC + WG_THREADS - 1) / WG_THREADS;
+  size_t gi[NP];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = (_Float16)0.f;
-    v[0] = (_Float16)(halo_value(res, nbr, b, pz, py, px) * inv_hs);
-    v[1] = (_Float16)(halo_value(prev, nbr, b, pz, py, px) * inv_sp);
-    *reinterpret_cast<half8_t*>(&bufB[p * 8]) = v;
+  for (int j = 0; j < NP; ++j) {
+    const int p = tid + j * WG_THREADS;
+    gi[j] = p < PADC ? halo_index(nbr, b, p / 100, (p / 10) % 10, p % 10) : (size_t)b * BLEN;
   }
-  for (int i = tid; i < PADC * CNN_C / 8; i += WG_THREADS) {
-    half8_t z;
+  double rv[NP], pv[NP];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) z[e] = (_Float16)0.f;
-    reinterpret_cast<half8_t*>(bufA)[i] = z;
+  for (int j = 0; j < NP; ++j) {
+    rv[j] = res[gi[j]];
+    pv[j] = prev[gi[j]];
+  }
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int p = tid + j * WG_THREADS;
+    if (p < PADC) {
+      half8_t v;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = (_Float16)0.f;
+      v[0] = (_Float16)((float)rv[j] * inv_hs);
+      v[1] = (_Float16)((float)pv[j] * inv_sp);
+      *reinterpret_cast<half8_t*>(&bufIn[p * 8]) = v;
+    }
   }
   __syncthreads();
   const half8_t* wf = reinterpret_cast<const half8_t*>(wpk);
-  conv_layer<8, CNN_KT1, true>(bufB, wf, wv, l, [&](int m, int n, float v) {
-    bufA[padded(m) * CNN_C + n] = (_Float16)gelu(v + w[CNN_OB1 + n]);
+  conv_layer<CNN_KT1, true>(bufIn, wf, wv, l, [&](int m, int n, float v) {
+    bufA[m * CNN_C + n] = (_Float16)gelu(v + w[CNN_OB1 + n]);
   });
   __syncthreads();
-  for (int i = tid; i < PADC * CNN_C / 8; i += WG_THREADS) {
-    half8_t z;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) z[e] = (_Float16)0.f;
-    reinterpret_cast<half8_t*>(bufB)[i] = z;
-  }
-  __syncthreads();
-  conv_layer<CNN_C, CNN_KT2, false>(bufA, wf + CNN_KT1 * 64, wv, l, [&](int m, int n, float v) {
-    bufB[padded(m) * CNN_C + n] = (_Float16)gelu(v + w[CNN_OB2 + n]);
+  conv_layer<CNN_KT2, false>(bufA, wf + CNN_KT1 * 64, wv, l, [&](int m, int n, float v) {
+    bufIn[m * CNN_C + n] = (_Float16)gelu(v + w[CNN_OB2 + n]);
   });
   __syncthreads();
   const float b3 = w[CNN_OB3];
   const size_t off = (size_t)b * BLEN;
-  conv_layer<CNN_C, CNN_KT3, false>(bufB, wf + (CNN_KT1 + CNN_KT2) * 64, wv, l,
-                                    [&](int m, int n, float v) {
-                                      if (n == 0) out[off + m] = prev[off + m] + s_r * (double)(v + b3);
-                                    });
+  conv_layer<CNN_KT3, false>(bufIn, wf + (CNN_KT1 + CNN_KT2) * 64, wv, l,
+                             [&](int m, int n, float v) {
+                               if (n == 0) out[off + m] = prev[off + m] + s_r * (double)(v + b3);
+                             });
 }
 
 // ---------------------------------------------------------------------------
