@@ -14,7 +14,9 @@ pressure is the solution of the step before (blob centres 0.005 earlier).
 --workload c2 selects the uniform 256x128x128 walls grid (256 blobs and a
 penalised sphere), c3 the 3-level AMR sphere grid, c1 the 32^3 periodic
 Fourier case. The forms timed on the same steps, each with the same
-iteration cap (3000):
+iteration cap (3000), each run the way the caller runs its time steps: every
+solve is handed the form's own solution of the step before, its
+volume-weighted mean removed (PressureProjection.cpp:325-431):
   * fp64: the FP64 drop-in (the reference algorithm, FP64 MFMA block-Jacobi
     GEMM); its steps that do not reach the tolerance are counted;
   * prev_guess: the chosen form — FP16 split-precision MFMA GEMM with FP32
@@ -27,8 +29,9 @@ iteration cap (3000):
   * extrapolation_linear: the same from the linear extrapolation in time of
     the previous two pressures (no net);
   * extrapolation_only: the same from the least-squares combination of the
-    differences of the last --history (3) pressures (no net): the linear,
-    quadratic and cubic extrapolations are in its span;
+    differences of the last --history (6) pressures (no net): the
+    extrapolations in time up to that degree are in its span, the depth
+    taken is scored against the rounding of the kept pressures;
   * the headline `value`: the learned guess and that extrapolation
     together, weighted by FP64 least squares on their residuals (fallback:
     the previous pressure); the handle's history is primed with the steps
@@ -138,7 +141,7 @@ def parse():
     p.add_argument("--c4-fresh", type=int, default=40)
     p.add_argument("--history", type=int, default=6,
                    help="previous pressures the warm start extrapolates from (1 = linear, "
-                        "at most 6)")
+                        "at most 8)")
     p.add_argument("--no-dst", action="store_true",
                    help="skip the leg with the sine-transform block inverse")
     p.add_argument("--fp64-dst-steps", type=int, default=4,
@@ -182,14 +185,17 @@ def in_solve_profile(S, enable=None):
 
 def time_form(w, form, bmean, seq, warmup, err_fn, weights=None, profile=False, tol=1e-7,
               extrapolate=False, log=None, precond="gemm"):
-    """Solve every step of the sequence with `form`, each handed the previous
-    step's reference pressure (given net weights, the learned guess with that
-    pressure as the fallback; with `extrapolate`, also the extrapolation in
-    time from the pressure handed to the solve before); time the steps after
-    the first `warmup`. Returns the handle and, per timed step, (ms, report,
-    error against the step's reference solution). With `profile`, the library
-    also times K1 .. K5 with HIP events on its stream during the timed solves
-    (one iteration in eight) and the handle carries them as `prof`."""
+    """Solve every step of the sequence with `form` the way the caller runs
+    its time steps (PressureProjection.cpp:325-431): each solve is handed the
+    form's own solution of the step before, its volume-weighted mean removed
+    (the first from the reference pressure of the step before the sequence);
+    given net weights, the learned guess with that pressure as the fallback;
+    with `extrapolate`, also the extrapolation in time from the pressures
+    handed to the solves before. Time the steps after the first `warmup`.
+    Returns the handle and, per timed step, (ms, report, error against the
+    step's reference solution). With `profile`, the library also times K1 ..
+    K5 with HIP events on its stream during the timed solves (one iteration
+    in eight) and the handle carries them as `prof`."""
     import torch
     from cup3d_amd.poisson import solver_for
     S = solver_for(w, tol_rel=0.0, tol_rhs=tol, precision=form, bmean=bmean, max_iter=MAX_ITER,
@@ -199,16 +205,16 @@ def time_form(w, form, bmean, seq, warmup, err_fn, weights=None, profile=False, 
         S.cnn_set_weights(weights)
     if profile:
         in_solve_profile(S, True)
-    x = torch.empty_like(seq[0]["p_prev"])
+    x = seq[0]["p_prev"].clone()
     out = []
     for k, st in enumerate(seq):
         if profile and k == warmup:
             in_solve_profile(S)  # drop the warm-up samples
-        x.copy_(st["p_prev"])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         rep = S.solve_device(st["rhs"], x)
         dt = time.perf_counter() - t0
+        S.remove_mean(x)  # the caller's projection after the solve; x is the next guess
         if k >= warmup:
             out.append((dt * 1e3, rep, err_fn(x, st["x_ref"])))
         if log:
@@ -814,9 +820,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "form": args.form, "cells": w.ncells,
                    "blocks": int(len(w.blocks)), "tolerance": f"||b-Ax||/||b|| <= {tol:g}",
-                   "steps": "consecutive time steps t = warmup+1 .. warmup+steps",
-                   "warm_start": (f"learned CNN guess + extrapolation in time from the last "
-                                  f"{args.history} pressures, FP64 least-squares weights; "
+                   "steps": "consecutive time steps t = warmup+1 .. warmup+steps, each solve "
+                            "handed the form's own previous solution, mean removed",
+                   "warm_start": (f"learned CNN guess + extrapolation in time from up to "
+                                  f"{args.history} previous pressures, FP64 least-squares "
+                                  "weights, depth scored against the history's rounding; "
                                   "previous-step pressure as fallback"
                                   if learned else "previous-step pressure"),
                    "bMeanConstraint": args.bmean, "max_iter": MAX_ITER,

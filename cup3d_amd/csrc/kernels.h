@@ -40,7 +40,7 @@ void launch_remove_mean(hipStream_t s, long long n, double* x, const double* h3,
 void launch_stream_copy(hipStream_t s, long long n4, const void* a, void* b);
 
 // Warm start from several guesses (device vectors of n doubles).
-constexpr int HIST_MAX = 6;                // differences of previous pressures
+constexpr int HIST_MAX = 8;                // differences of previous pressures
 constexpr int GUESS_MAXV = HIST_MAX + 2;   // r0 + the history directions + the net's
 struct GuessVecs {
   const double* v[GUESS_MAXV];

@@ -392,7 +392,7 @@ static void guess_gram_nv(hipStream_t s, long long n, const GuessVecs& r, double
 
 void launch_guess_gram(hipStream_t s, long long n, const GuessVecs& r, int nv, double* part,
                        double* dots) {
-  static_assert(GUESS_MAXV == 8, "one instance per vector count");
+  static_assert(GUESS_MAXV == 10, "one instance per vector count");
   switch (nv) {
     case 1: guess_gram_nv<1>(s, n, r, part, dots); break;
     case 2: guess_gram_nv<2>(s, n, r, part, dots); break;
@@ -401,7 +401,9 @@ void launch_guess_gram(hipStream_t s, long long n, const GuessVecs& r, int nv, d
     case 5: guess_gram_nv<5>(s, n, r, part, dots); break;
     case 6: guess_gram_nv<6>(s, n, r, part, dots); break;
     case 7: guess_gram_nv<7>(s, n, r, part, dots); break;
-    default: guess_gram_nv<8>(s, n, r, part, dots); break;
+    case 8: guess_gram_nv<8>(s, n, r, part, dots); break;
+    case 9: guess_gram_nv<9>(s, n, r, part, dots); break;
+    default: guess_gram_nv<10>(s, n, r, part, dots); break;
   }
 }
 

@@ -136,6 +136,10 @@ struct cup3d_poisson {
   double* hist[HIST_MAX] = {};
   double* img[HIST_MAX] = {};
   int hist_count = 0;
+  // the FP64 true residual of each kept pressure (that of the solve it came
+  // from, when the caller handed that solution back), and of the last solve
+  double hist_rho[HIST_MAX] = {};
+  double last_resid = -1.0;  // < 0: unknown
 };
 
 namespace {
@@ -452,6 +456,7 @@ int cup3d_poisson_set_topology(cup3d_poisson_t H, int64_t nblocks, const cup3d_b
   H->part_len = (size_t)((H->nb + 3) / 4 + 8) * (GUESS_MAXV * (GUESS_MAXV + 1) / 2);
   if (H->d_part.reserve(H->part_len * sizeof(double)) != hipSuccess) return fail(H, "hipMalloc(partials) failed");
   H->hist_count = 0;
+  H->last_resid = -1.0;
   if (H->cfg.extrapolate > 0) {
     if (H->d_hist.reserve((size_t)H->cfg.extrapolate * n * sizeof(double)) != hipSuccess ||
         H->d_img.reserve((size_t)H->cfg.extrapolate * n * sizeof(double)) != hipSuccess)
@@ -528,13 +533,6 @@ static double increment_residual(cup3d_poisson_t H, const double* d_rhs, const d
   return std::sqrt(H->h_state->true_rr);
 }
 
-// r = b - A x for b in v64.b (FP64).
-static int resid_into(cup3d_poisson_t H, const double* d_x, double* r) {
-  launch_spmv64(H->stream, SPMV_RESID, H->g64, d_x, nullptr, H->v64.b, r, nullptr,
-                H->d_state.as<SolverState>(), H->d_part.as<double>());
-  return hipGetLastError() == hipSuccess ? 0 : fail(H, "residual launch failed");
-}
-
 // Solves the nk x nk system G g = f (symmetric positive semi-definite, nk
 // <= GUESS_MAXV - 1) by Gaussian elimination with partial pivoting; false if
 // it is singular to rounding.
@@ -567,6 +565,15 @@ static bool small_solve(int nk, double G[][GUESS_MAXV], const double* f, double*
   return true;
 }
 
+// Weight of the history-noise estimate in the warm start's option score
+// (see warm_start): a unit of residual carried in from the kept pressures'
+// rounding counts as much as NOISE_WEIGHT units of the extrapolation's own
+// residual. With 0 the deepest option wins whenever its residual is 1 %
+// lower; on moving-blob sequences solved to 1e-7 and fed back (the AMR
+// school, uniform 256x128x128 and AMR sphere grids) 120..300 selects the
+// depths that need the fewest iterations (4, 5, 5 of 6).
+constexpr double NOISE_WEIGHT = 200.0;
+
 // Warm start next to the given guess x0 (the caller's pressure of the
 // previous step), in the spirit of Fischer's projection onto previous
 // solutions (Comput. Methods Appl. Mech. Engrg. 163, 1998), here with the
@@ -581,10 +588,11 @@ static bool small_solve(int nk, double G[][GUESS_MAXV], const double* f, double*
 // 1 %); g = 0 is one of the choices, so the result is never worse than x0,
 // which is kept (the fallback) when no direction helps.
 static int warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* used,
-                      double* resid, double* w_net, double* w_time) {
+                      double* resid, double* w_net, double* w_time, int* depth) {
   hipStream_t s = H->stream;
   *used = CUP3D_GUESS_GIVEN;
   *w_net = *w_time = 0.0;
+  *depth = 0;
   // a failure on the way leaves history and images out of step: start over
   struct ClearUnlessDone {
     cup3d_poisson_t H;
@@ -649,11 +657,34 @@ static int warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* 
       return D[a * GUESS_MAXV - a * (a - 1) / 2 + (b - a)];
     };
     const double a2 = dot(0, 0);
+    // The kept pressures carry the rounding of their own solves: residuals
+    // rho_j of about the tolerance, whose error is the smooth part the
+    // Krylov process removes slowest. An option x0 + sum_k g_k d_k takes
+    // them with coefficients c_0 = 1 + g_1 (- g_net), c_j = g_{j+1} - g_j,
+    // c_m = -g_m on x0, h_1 .. h_m (binomial for the extrapolations: their
+    // norm grows like 2^m), so it carries their noise with a residual of
+    // about sqrt(sum_j c_j^2 rho_j^2). Each option is scored by its
+    // residual plus NOISE_WEIGHT times that estimate (in squares): deeper
+    // extrapolation is taken only while what it removes exceeds the noise
+    // it brings.
+    const double rho0 = H->last_resid > 0 ? H->last_resid : 0.0;
+    auto noise2 = [&](const double* g, int m, double gnet) {
+      double c0 = 1.0 - gnet + (m > 0 ? g[0] : 0.0);
+      double v = c0 * c0 * rho0 * rho0;
+      for (int j = 1; j <= m; ++j) {
+        const double cj = (j < m ? g[j] : 0.0) - g[j - 1];
+        v += cj * cj * H->hist_rho[j - 1] * H->hist_rho[j - 1];
+      }
+      return v;
+    };
+    constexpr double NW2 = NOISE_WEIGHT * NOISE_WEIGHT;
     // options in order of size: the first m time directions, with or
-    // without the net; each replaces the best so far only if it gains at
-    // least 1 %
+    // without the net; each replaces the best so far only if it lowers the
+    // score by at least 1 %
     constexpr double KEEP = 0.99;
     double best_r2 = a2, best_g[GUESS_MAXV] = {};
+    double best_score = a2 + NW2 * rho0 * rho0;
+    int best_m = 0;
     for (int size = 1; size <= nc; ++size) {
       for (int with_net = 0; with_net <= (inet >= 0 ? 1 : 0); ++with_net) {
         const int m = size - with_net;
@@ -669,16 +700,17 @@ static int warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* 
         if (!small_solve(nk, G, f, g)) continue;
         double r2 = a2;
         for (int i = 0; i < nk; ++i) r2 -= g[i] * f[i];
-        if (!std::isfinite(r2) || !(r2 < KEEP * best_r2)) continue;
+        const double score = r2 + NW2 * noise2(g, m, with_net ? g[nk - 1] : 0.0);
+        if (!std::isfinite(score) || !(score < KEEP * best_score)) continue;
+        best_score = score;
         best_r2 = r2;
+        best_m = m;
         for (int k = 0; k < GUESS_MAXV; ++k) best_g[k] = 0;
         for (int i = 0; i < nk; ++i) best_g[idx[i]] = g[i];
       }
     }
-    if (best_r2 < a2) {
-      int m = 0;  // time directions with a weight
-      for (int k = 0; k < mt; ++k)
-        if (best_g[k] != 0) m = k + 1;
+    if (best_score < a2 + NW2 * rho0 * rho0) {
+      const int m = best_m;  // time directions taken
       const double gnet = inet >= 0 ? best_g[inet] : 0.0;
       GuessVecs hv{};
       for (int k = 0; k < mt; ++k) hv.v[k] = H->hist[k];
@@ -689,14 +721,19 @@ static int warm_start(cup3d_poisson_t H, const double* d_rhs, double* d_x, int* 
       *w_net = gnet;
       *used = gnet != 0 ? CUP3D_GUESS_CNN : CUP3D_GUESS_GIVEN;
       *resid = std::sqrt(best_r2 > 0 ? best_r2 : 0.0);
+      *depth = m;
     }
   }
   // this step's given guess (x0, as getVec made it) becomes h_1 of the next
   // solve; copied after the blend, which still reads the present history
   if (K > 0) {
     double* oldest = H->hist[K - 1];
-    for (int k = K - 1; k > 0; --k) H->hist[k] = H->hist[k - 1];
+    for (int k = K - 1; k > 0; --k) {
+      H->hist[k] = H->hist[k - 1];
+      H->hist_rho[k] = H->hist_rho[k - 1];
+    }
     H->hist[0] = oldest;
+    H->hist_rho[0] = H->last_resid > 0 ? H->last_resid : 0.0;
     HIPCHECK(hipMemcpyAsync(oldest, x0, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
     H->hist_count = std::min(H->hist_count + 1, K);
   }
@@ -717,11 +754,13 @@ int cup3d_poisson_solve_device(cup3d_poisson_t H, const double* d_rhs, double* d
   int guess_used = H->cfg.guess;
   double guess_resid = -1;
   double w_net = 0, w_time = 0;
+  int depth = 0;
   if (H->cfg.guess == CUP3D_GUESS_ZERO) x0 = nullptr;
   int st;
   if (H->cfg.guess == CUP3D_GUESS_CNN || (H->cfg.extrapolate && H->cfg.guess == CUP3D_GUESS_GIVEN)) {
     // learned and/or extrapolated warm start, the given guess as the fallback
-    if (warm_start(H, d_rhs, d_x, &guess_used, &guess_resid, &w_net, &w_time) != 0) return -1;
+    if (warm_start(H, d_rhs, d_x, &guess_used, &guess_resid, &w_net, &w_time, &depth) != 0)
+      return -1;
   }
   if (H->cfg.precision == CUP3D_PREC_FP64)
     st = solve_fp64(H, d_rhs, d_x, x0, &local);
@@ -739,6 +778,10 @@ int cup3d_poisson_solve_device(cup3d_poisson_t H, const double* d_rhs, double* d
   if (guess_resid >= 0) local.guess_resid = guess_resid;
   local.guess_weight_net = w_net;
   local.guess_weight_time = w_time;
+  local.guess_depth = depth;
+  // what the next warm start takes as the noise of this solution
+  const double r_true = local.true_rel_resid * local.rhs_norm;
+  H->last_resid = std::isfinite(r_true) ? r_true : -1.0;
   if (rep) *rep = local;
   return st;
 }

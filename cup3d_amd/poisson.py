@@ -41,7 +41,7 @@ class Report(ctypes.Structure):
                 ("rhs_norm", ctypes.c_double), ("true_rel_resid", ctypes.c_double),
                 ("solve_ms", ctypes.c_double), ("guess_used", ctypes.c_int),
                 ("guess_resid", ctypes.c_double), ("guess_weight_net", ctypes.c_double),
-                ("guess_weight_time", ctypes.c_double)]
+                ("guess_weight_time", ctypes.c_double), ("guess_depth", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -66,7 +66,7 @@ enum {
   CUP3D_GUESS_CNN = 2     /* learned warm start from (rhs, x on input = previous pressure);
                              falls back to the given x when its residual is larger */
 };
-/* With cfg.extrapolate = K in 1..6 (and guess GIVEN or CNN) the solver keeps
+/* With cfg.extrapolate = K in 1..8 (and guess GIVEN or CNN) the solver keeps
  * the x handed to its last K solves on the same topology (the caller's
  * pressures of the previous steps, h_1 the most recent) and tries the
  * directions d_1 = x - h_1, d_2 = h_1 - h_2, ..., d_K = h_{K-1} - h_K (those
@@ -77,10 +77,16 @@ enum {
  * residual norm). K = 1 with g_1 = 1 is the linear extrapolation in time
  * 2 x - h_1; K = 2, 3, ... reach the quadratic, cubic, ... extrapolations.
  * The time directions are taken nested (d_1, then d_2, ...) and the net's as
- * an option beside them; a larger set is only taken if it lowers the squared
- * residual by at least 1 % against the best smaller one, and the given x (all
- * weights 0) is one of the choices, so the guess is never worse than x.
- * guess_weight_time reports g_1. set_topology clears the history. Cost per
+ * an option beside them. Each option is scored by its squared residual plus
+ * 200^2 times the rounding it carries in from the kept pressures,
+ * sum_j c_j^2 rho_j^2 (c_j its coefficients on x, h_1, ...; rho_j the true
+ * residual of the solve each came from, assuming the caller hands the
+ * solutions back), which stops the extrapolation where its binomial weights
+ * would amplify that rounding; a larger set is only taken if it lowers the
+ * score by at least 1 % against the best smaller one, and the given x (all
+ * weights 0) is one of the choices, so the guess's residual is never above
+ * x's. guess_weight_time reports g_1, guess_depth the number of time
+ * directions taken. set_topology clears the history. Cost per
  * solve: one FP64 operator application (A d_2 .. A d_K are the last solve's
  * A d_1 .. A d_{K-1}, kept), one Gram reduction, one blend; memory: 2 K
  * vectors of n doubles. */
@@ -112,7 +118,7 @@ typedef struct {
   int poll_every;    /* iterations between host polls of the device status (0 = default) */
   int gemm_passes;   /* mixed only: 0 (or 2, 3) = input split into hi + lo, two MFMA
                         passes, linear to FP32 precision; 1 = one FP16/BF16 pass */
-  int extrapolate;   /* 0 = off (the reference contract); K = 1..6 = also extrapolate
+  int extrapolate;   /* 0 = off (the reference contract); K = 1..8 = also extrapolate
                         the guess in time from the last K solves' (see above) */
   int precond;       /* how M^-1 = blockdiag(h_b^-1 P_inv) is applied.
                         CUP3D_PRECOND_GEMM (0): P_inv x block matrix on the matrix cores
@@ -145,6 +151,7 @@ typedef struct {
   double guess_resid;    /* ||b - A x0||_2 of the guess used */
   double guess_weight_net;   /* g_net of the warm start (0: the net's guess not used) */
   double guess_weight_time;  /* g_time of the warm start (0: no extrapolation) */
+  int guess_depth;           /* time directions of the warm start taken (0: none) */
 } cup3d_solve_report;
 
 /* Create a solver on cfg->device. Returns 0 or < 0. */

@@ -64,7 +64,7 @@ def test_create_rejects_bad_config():
 
 @pytest.mark.parametrize("field,value,message", [
     ("bmean", 4, b"bmean"), ("max_iter", 0, b"max_iter"), ("precision", 3, b"precision"),
-    ("guess", 3, b"guess"), ("extrapolate", 7, b"extrapolate"), ("h0", 0.0, b"h0"),
+    ("guess", 3, b"guess"), ("extrapolate", 9, b"extrapolate"), ("h0", 0.0, b"h0"),
     ("precond", 2, b"precond")])
 def test_create_rejects_each_bad_field(field, value, message):
     """Every configuration field is checked before any device is touched."""
