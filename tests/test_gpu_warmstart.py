@@ -239,3 +239,32 @@ def test_history_weights_are_the_least_squares_solution(gpu, name, K, tol):
         assert deep == K
     else:
         assert noise_decided > 0
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed_fp16"])
+@pytest.mark.parametrize("bmean", [1, 3])
+def test_history_with_zeroed_origin_cell(gpu, prec, bmean):
+    """With bMean 1 or 3 getVec zeroes the origin cell of the guess, and every
+    warm-start direction is zero there too: the solve starts from the residual
+    the weights minimised (its own err_init equals the reported guess
+    residual), never above the given guess's (a handle without the history,
+    same inputs)."""
+    blocks, bpd, h0, bc = CASES["tiny_amr3_walls"]
+    O = pyoracle.Oracle(blocks, bpd, h0, bc, bmean)
+    kw = dict(bmean=bmean, tol_rel=0.0, tol_rhs=1e-9, precision=prec, max_iter=3000)
+    S = PoissonSolver(bpd, h0, bc, extrapolate=3, **kw).set_topology(blocks)
+    P = PoissonSolver(bpd, h0, bc, **kw).set_topology(blocks)
+    base = [random_field(len(blocks), 90 + k) for k in range(4)]
+    x = torch.zeros(O.n, dtype=torch.float64, device=gpu)
+    taken = 0
+    for t in range(7):
+        c = [np.cos(0.3 * (1 + 0.25 * k) * t + k) for k in range(4)]
+        b = dev(O.spmv(sum(ck * bk for ck, bk in zip(c, base))))
+        xg = x.clone()
+        rep = S.solve_device(b, x)
+        rep2 = P.solve_device(b, xg)
+        assert rep["status"] == 0 and rep2["status"] == 0, (t, rep, rep2)
+        assert abs(rep["err_init"] - rep["guess_resid"]) <= 1e-8 * rep2["err_init"], (t, rep)
+        assert rep["err_init"] <= rep2["err_init"] * (1 + 1e-12), (t, rep, rep2)
+        taken += rep["guess_depth"] > 0
+    assert taken >= 3  # the test has teeth: the history was used
