@@ -489,7 +489,7 @@ def bench_cpu_full(args):
 def bench_c4(args):
     """The learned-warm-start protocol on the uniform 256x128x128 blob
     sequence (cup3d_amd.learned): train on the first --c4-train triples,
-    judge the mixed form's iterations on the remaining steps and on a
+    evaluate the mixed form's iterations on the remaining steps and on a
     fresh-seed sequence against the previous-pressure guess."""
     from cup3d_amd import learned
     r = learned.run(steps=args.c4_steps, train=args.c4_train, fresh=args.c4_fresh,

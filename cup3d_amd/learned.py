@@ -1,4 +1,4 @@
-"""Learned warm start on a sequence of pressure solves: generate, train, judge.
+"""Learned warm start on a sequence of pressure solves: generate, train, evaluate.
 
 The sequence is the uniform 256x128x128 walls workload with 64 Gaussian
 blobs (sigma 0.04) moving 0.005 per step (``workloads.c4_uniform_sequence``).
@@ -6,7 +6,7 @@ Each step is solved to ||b - A x|| / ||b|| <= 1e-7 (FP64 true residual) from
 the previous step's pressure by the FP64-refined solve (the FP64 drop-in's
 recurrence stagnates above 1e-7 on some steps of this sequence); the first
 ``--train`` steps give the training triples (rhs_t, p_{t-1}, p_t), which the
-library's own HIP kernels learn (FP32 forward/backward, Adam). The net is then judged on the remaining steps
+library's own HIP kernels learn (FP32 forward/backward, Adam). The net is then evaluated on the remaining steps
 and on a sequence from a fresh seed by what it saves: the number of
 iterations, for the FP64 and the mixed-precision solve, against starting from
 the previous pressure; also, once the handle holds `history` previous
@@ -86,7 +86,7 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, bat
           f"{train_s:.1f} s", file=log, flush=True)
     weights = ref.cnn_get_weights()
 
-    def judge(rhs_, sols_, ts):
+    def evaluate(rhs_, sols_, ts):
         out = {}
         for form in forms:
             kf = dict(kw, max_iter=fp64_max_iter) if form == "fp64" else kw
@@ -166,9 +166,9 @@ def run(bpd=(32, 16, 16), steps=400, train=300, fresh=40, train_steps=20000, bat
             }
         return out
 
-    held = judge(rhs, sols, range(train + 1, steps + 1))
+    held = evaluate(rhs, sols, range(train + 1, steps + 1))
     frhs, fsols, _ = sequence(fresh, W.FRESH_SEED)
-    fr = judge(frhs, fsols, range(1, fresh + 1))
+    fr = evaluate(frhs, fsols, range(1, fresh + 1))
     return {
         "workload": w.name, "cells": w.ncells, "steps": steps, "train_triples": train,
         "train_steps": train_steps, "batch_blocks": batch, "lr": lr, "res_weight": res_weight,

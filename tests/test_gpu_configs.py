@@ -11,10 +11,10 @@ on this grid) and with a reference solution refined to 1e-10.
 Learned warm start: the uniform 256x128x128 walls sequence of 64 moving blobs
 (4.19 M cells), a shortened form of the 300 / 100 / fresh-seed protocol: the
 net is trained by the library's kernels on 40 (rhs, p_prev, p) triples,
-judged by the iterations the mixed solve needs from its guess against the
+evaluated by the iterations the mixed solve needs from its guess against the
 previous pressure, every solve finished to the same tolerance: first on the
 8 validation steps after the training window, which decide whether the net
-is used at all (it is judged by iterations saved, not by its residual), then
+is used at all (it is evaluated by iterations saved, not by its residual), then
 on the 16 held-out steps after those and on 4 steps of a fresh seed. Its
 backward kernels are checked against the FP64 oracle on fields of that
 sequence.
@@ -158,7 +158,7 @@ def c4_run(gpu):
     weights = ref.cnn_get_weights()
     frhs, fsols = sequence(N_FRESH, W.FRESH_SEED)
 
-    def judge(rhs_, sols_, ts):
+    def evaluate(rhs_, sols_, ts):
         base = solver_for(w, precision="mixed_fp16", guess="given", **kw)
         net = solver_for(w, precision="mixed_fp16", guess="cnn", **kw)
         net.cnn_set_weights(weights)
@@ -171,9 +171,9 @@ def c4_run(gpu):
             rows.append((rb, rn))
         return rows
 
-    val = judge(rhs, sols, range(N_TRAIN + 1, N_TRAIN + N_VAL + 1))
-    held = judge(rhs, sols, range(N_TRAIN + N_VAL + 1, N_TRAIN + N_VAL + N_HELD + 1))
-    fresh = judge(frhs, fsols, range(1, N_FRESH + 1))
+    val = evaluate(rhs, sols, range(N_TRAIN + 1, N_TRAIN + N_VAL + 1))
+    held = evaluate(rhs, sols, range(N_TRAIN + N_VAL + 1, N_TRAIN + N_VAL + N_HELD + 1))
+    fresh = evaluate(frhs, fsols, range(1, N_FRESH + 1))
     return dict(w=w, rhs=rhs, sols=sols, l0=l0, l1=l1, weights=weights, val=val, held=held,
                 fresh=fresh)
 
@@ -206,7 +206,7 @@ def test_c4_iterations_saved(c4_run):
     validation steps; the held-out steps and the fresh seed then measure the
     raw net and the gated method. Whatever the net does, the gated method
     costs at most the noise of the iteration counts over the previous
-    pressure on the held-out steps (the gate is judged on data the held-out
+    pressure on the held-out steps (the gate is evaluated on data the held-out
     steps never saw)."""
     s_v, b_v, n_v = _saved(c4_run["val"])
     s_h, b_h, n_h = _saved(c4_run["held"])
@@ -223,7 +223,7 @@ def test_c4_iterations_saved(c4_run):
           f"fresh seed: {b_f:.1f} -> {n_f:.1f} ({s_f:+.1f} %)")
     # the net's guess lowers the starting residual where it is taken ...
     assert ratio <= 1.0
-    # ... and the method judged by iterations never pays for a net that
+    # ... and the method evaluated by iterations never pays for a net that
     # does not help: +/- 3 % is the spread of these counts between
     # equivalent guesses
     assert it_g <= 1.03 * it_b, (it_g, it_b)
